@@ -1,0 +1,154 @@
+/*
+ * tomo_hip.h -- C ABI of libtomo_hip.so, the MI355X (gfx950) support-tomography sweep.
+ *
+ * Two entry-point families:
+ *
+ *  1. The drop-in legacy shim.  Same ten-pointer signature, header-array layout, return value
+ *     and getter set as the reference DLL that cfms_tomo binds with ctypes
+ *     (cfms_tomo/Tomo_Shell_vc_src/Tomo_Shell2026.h:19-35; bound at
+ *     cfms_tomo/tomo_dll_interface.py:66-77).  Outputs are owned by the library and stay valid
+ *     until the next TomoSh_HIP call or OnDestroy, as in the reference (Tomo_Shell2026.cpp:19-35,
+ *     73-87).  Not re-entrant (global state), like the reference.
+ *
+ *  2. A handle API (re-entrant per handle, one HIP stream per handle, caller-owned outputs,
+ *     status codes instead of exit()).  It replaces the same reference entry point
+ *     (_TomoSh_Function_Call, Tomo_Shell2026.cpp:119-249) split into set-up, sweep and readback.
+ *
+ * Units and options follow the reference: Float32Info[12] and Int32Info[11] in the order packed
+ * by tomo_dll_interface.py:115-124 and decoded by S3DPrinterInfo::Set (S3DPrinterInfo.cpp:124-145);
+ * vertex coordinates are voxel indices (STomoVoxelSpaceInfo.cpp:97-105).
+ */
+#ifndef TOMO_HIP_H
+#define TOMO_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------------------------- */
+#define TOMO_OK 0
+#define TOMO_E_INVALID_ARG (-1)
+#define TOMO_E_HIP (-2)
+#define TOMO_E_OOM (-3)
+#define TOMO_E_NO_DEVICE (-4)
+#define TOMO_E_STATE (-5)
+
+/* ---- per-orientation flags (tomo_record.flags) ------------------------------------------- */
+#define TOMO_F_FP32_INEXACT 1   /* a float32 running total may round: |terms| sum > 2^24      */
+#define TOMO_F_INT16_WRAP 2     /* per-slot |z| sum > 32767 or a z/nZ outside int16 (the      */
+                                /* pre-fix reference wrapped there)                           */
+#define TOMO_F_FOOTPRINT_OOB 4  /* an entry fell outside the grid and was dropped (the        */
+                                /* reference writes out of bounds: undefined behaviour)       */
+#define TOMO_F_CAP_DROP 8       /* a slot held 63 distinct rows and dropped one               */
+#define TOMO_F_SENTINEL_DROP 16 /* a floor row was skipped because its slot was full          */
+#define TOMO_F_RANK_TIE 32      /* a triangle put two pixels into one slot (insertion order   */
+                                /* between them is not tracked; matters only for CAP_DROP)    */
+#define TOMO_F_UNRESOLVED 64    /* a slot exceeded the exact-resolution capacity              */
+
+/* Bed types (Tomo_types.h:64-69). */
+#define TOMO_BED_NONE 0
+#define TOMO_BED_BRIM 1
+#define TOMO_BED_RAFT 2
+#define TOMO_BED_SKIRT 3
+
+/* Per-orientation result.  Integer totals are exact; the f* fields are the reference's float32
+ * running totals (STomoVolMassInfo.h:24-30), identical to them unless TOMO_F_FP32_INEXACT. */
+typedef struct {
+  int64_t Va, Vb, Vo, Vss, Vnv, Vtc, Vbed; /* Vbed = number of bed rows (each at z = 1)        */
+  int64_t n_entries;  /* pixel rows after rasterisation, summed over slots                    */
+  int64_t n_slots;    /* slots touched by rasterisation                                       */
+  int64_t n_final;    /* rows after pairing floor rows and the bed                            */
+  uint64_t chk_pre;   /* sum over slots of (slot id + 1) * rows after rasterisation           */
+  uint64_t chk_final; /* same after pairing and bed                                          */
+  int32_t flags;
+  int32_t n_bed;
+  float fVa, fVb, fVtc, fVnv, fVss, fVo, fVbed;
+  float Mo, Mss, Mtotal; /* VolToMass (STomoVolMassInfo.cpp:42-81)                           */
+  float mat4x4[16];      /* placement transform, row-major (STomoSh_Base.cpp:111-123)        */
+} tomo_record;
+
+typedef struct tomo_handle tomo_handle;
+
+/* ---- handle API ---------------------------------------------------------------------------- */
+
+/* Create a sweep context on the current HIP device.  grid: slot grid edge (the reference CPU
+ * engine is fixed at 256, STomoVoxelSpaceInfo.h:10; 0 means Int32Info[3] = nVoxel).  Replaces
+ * the option unpacking of S3DPrinterInfo::Set (S3DPrinterInfo.cpp:124-145). */
+int tomo_create(const float *float32_info_x12, const int32_t *int32_info_x11, int32_t grid,
+                tomo_handle **out);
+
+/* Upload a mesh (host pointers, copied): tri[nTri*3] int32, vtx[nVtx*3], vtx_nrm[nVtx*3] float32.
+ * Replaces the borrowed rpTri0/rpVtx0/rpVtxNrm0 pointers of S3DPrinterInfo::Set
+ * (S3DPrinterInfo.cpp:179-182). */
+int tomo_set_mesh(tomo_handle *h, int32_t n_tri, const int32_t *tri, int32_t n_vtx,
+                  const float *vtx, const float *vtx_nrm);
+
+/* Upload orientations: ypr[n*3] radians (yaw, pitch, roll) as TomoSh_INT3 takes them
+ * (Tomo_Shell2026.cpp:104-106); rotation matrices are built on the host with cosf/sinf
+ * (SMatrix4f.cpp:213-240). */
+int tomo_set_orientations(tomo_handle *h, int32_t n, const float *ypr);
+
+/* Enqueue the sweep of orientations [begin, end) on the handle's stream (asynchronous). */
+int tomo_run(tomo_handle *h, int32_t begin, int32_t end);
+
+/* Wait for the handle's stream. */
+int tomo_sync(tomo_handle *h);
+
+/* Copy records [begin, end) of the last run to out[end-begin] and finish them on the host
+ * (float totals, masses, transform).  Synchronises. */
+int tomo_get_records(tomo_handle *h, int32_t begin, int32_t end, tomo_record *out);
+
+/* set_orientations + run + get_records for all n orientations. */
+int tomo_sweep(tomo_handle *h, int32_t n, const float *ypr, tomo_record *out);
+
+/* One orientation with per-slot dumps (host arrays of grid*grid int32, slot id x*grid + y; any
+ * may be NULL): rows after rasterisation, rows at the end, slot Vo and Vss.  For parity tests
+ * against the reference's per-slot state (STomoSh_INT3.cpp:366-395). */
+int tomo_debug_slots(tomo_handle *h, const float ypr[3], tomo_record *rec, int32_t *cnt_pre,
+                     int32_t *cnt_final, int32_t *slot_vo, int32_t *slot_vss);
+
+/* Persistent work-group count for tomo_run (0 = automatic).  Launch-shape knob for tuning. */
+int tomo_set_launch(tomo_handle *h, int32_t n_groups);
+
+/* Duration in milliseconds of the last tomo_run's kernel, from HIP events on the handle's
+ * stream (negative if unavailable). */
+float tomo_last_kernel_ms(tomo_handle *h);
+
+/* Grid edge and work-group count actually used. */
+int tomo_get_info(tomo_handle *h, int32_t *grid, int32_t *n_groups, int64_t *scratch_bytes);
+
+int tomo_destroy(tomo_handle *h);
+
+/* Message for the last error on this thread (never NULL). */
+const char *tomo_last_error(void);
+
+/* First strict minimum, initial value 1e5; 0 for n <= 1 (_find1stOptimal,
+ * Tomo_Shell2026.cpp:89-97). */
+int32_t tomo_find_first_optimal(int32_t n, const float *mss);
+
+/* ---- legacy drop-in shim (Tomo_Shell2026.h:19-35) ----------------------------------------- */
+
+/* Same arguments and return (optimal orientation index) as TomoSh_INT3 / TomoSh_CUDA
+ * (Tomo_Shell2026.h:33-35).  Convex-hull and triangle-normal arrays are accepted and unused, as
+ * in the INT3 engine.  Returns a negative status code on error instead of exiting. */
+int32_t TomoSh_HIP(float *float32_info_x12, int32_t *int32_info_x11, float *YPR, int32_t *tri,
+                   float *vtx, float *vtx_nrm, float *tri_nrm, int32_t *chull_tri,
+                   float *chull_vtx, float *chull_trinrm);
+
+float *getMss(void);         /* float32[nYPR] (Tomo_Shell2026.cpp:67)        */
+float *getMo(void);          /* float32[nYPR] (Tomo_Shell2026.cpp:68)        */
+float *getVtc(void);         /* float32[nYPR] (Tomo_Shell2026.cpp:69)        */
+float *getVolMassInfo(void); /* float32[22] of the optimum (:70, :244)      */
+float *getMat4x4(void);      /* float32[16] of the optimum (:71, :239-242)  */
+/* Render export of the optimum's pixel groups (Tomo_Shell2026.cpp:65-66, 221-236):
+ * rows {x, y, z, nx, ny, nz} int16 per pixel. */
+int32_t getnData2i(int16_t pixel_type);
+int16_t *getpData2i(int16_t pixel_type);
+void OnDestroy(void); /* frees the shim's outputs (Tomo_Shell2026.cpp:73-87) */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,93 @@
+/*
+ * tomo_oracle.h -- CPU restatement of the reference's INT3 support-tomography path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This oracle is the checker for the HIP library: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The product path
+ * (libtomo_hip.so and the tomo_shell2026_amd package) never calls it.
+ *
+ * It restates, in plain C, the per-orientation pipeline of the reference CPU engine
+ * (Tomo_Shell2026.cpp:99-117  thread_func: Rotate -> Pixelize -> Pairing -> GenerateBed ->
+ * Calculate) in "brief slot pairing" mode (Tomo_types.h:22).  File:line citations in the .c
+ * file are relative to cfms_tomo/Tomo_Shell_vc_src/ of the reference.
+ */
+#ifndef TOMO_ORACLE_H
+#define TOMO_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Slot row type bits (Tomo_types.h:72-82). */
+enum {
+  ORC_AL = 1, ORC_BE = 2, ORC_SSB = 4, ORC_SSA = 8, ORC_SS = 16, ORC_BED = 32,
+  ORC_VO = 64, ORC_VSS = 128, ORC_TC = 256, ORC_NVB = 512, ORC_NVA = 1024
+};
+
+/* Per-orientation flags. */
+enum {
+  ORC_F_FP32_INEXACT = 1,   /* a float32 running total could round (sum of |terms| > 2^24) */
+  ORC_F_INT16_WRAP = 2,     /* a per-slot partial sum or a z value leaves int16 range      */
+  ORC_F_FOOTPRINT_OOB = 4,  /* an entry fell outside the grid (reference: unchecked write)  */
+  ORC_F_CAP_DROP = 8,       /* a slot reached 63 distinct keys and dropped an entry         */
+  ORC_F_SENTINEL_DROP = 16  /* a floor sentinel was skipped because the slot was full       */
+};
+
+/* Options: Float32Info[12] and Int32Info[11] in the reference's order
+ * (tomo_dll_interface.py:115-124; S3DPrinterInfo.cpp:124-145), plus the grid edge. */
+typedef struct {
+  float f32[12];   /* dVoxel, theta_c, area, wall, Fcore, Fclad, Fss, Css, rho, BedInner, BedOuter, BedThick */
+  int32_t i32[11]; /* verbose, explicitSS, shell, nVoxel, nTri, nVtx, nYPR, nCHtri, nCHvtx, BedType, shell_um */
+  int32_t grid;    /* slot grid edge (the reference CPU hard-wires 256: STomoVoxelSpaceInfo.h:10) */
+} orc_opts;
+
+typedef struct {
+  int64_t Va, Vb, Vo, Vss, Vnv, Vtc, Vbed; /* exact integer totals (Vbed = number of bed rows at z=1) */
+  int64_t n_entries;      /* sum of per-slot pixel counts after Pixelize                     */
+  int64_t n_slots;        /* slots touched by Pixelize                                        */
+  int64_t n_final;        /* sum of per-slot pixel counts after Pairing sentinels and bed     */
+  uint64_t chk_pre;       /* sum over slots (id+1)*count after Pixelize                        */
+  uint64_t chk_final;     /* sum over slots (id+1)*count at the end                            */
+  int32_t flags;
+  int32_t n_bed;          /* bed rows written                                                  */
+  /* The reference's float32 running totals (STomoVolMassInfo.h:24-30), summed in its order. */
+  float fVa, fVb, fVtc, fVnv, fVss, fVo, fVbed;
+  float Mo, Mss, Mtotal;
+  float mat4x4[16];
+} orc_record;
+
+/* R = Rz(roll) Ry(pitch) Rx(yaw) as 3x3 row-major floats (SMatrix4f.cpp:213-240). */
+void orc_ypr_matrix(float yaw, float pitch, float roll, float R[9]);
+
+/* Integer nZ cut points equivalent to the float tests of createShadowBriefly
+ * (STomoSh_INT3.cpp:749-790): SSB iff nZ <= *cut_ssb, NVB iff nZ >= *cut_nvb (for beta rows). */
+void orc_theta_cuts(float theta_c, int32_t *cut_ssb, int32_t *cut_nvb);
+
+/* Sweep nYPR orientations (ypr: nYPR*3 floats, radians).  nthreads <= 0 -> 1.
+ * Returns 0 on success, negative on bad arguments. */
+int orc_sweep(const orc_opts *opt, int32_t nTri, const int32_t *tri, int32_t nVtx,
+              const float *vtx, const float *vnrm, int32_t nYPR, const float *ypr,
+              orc_record *out, int32_t nthreads);
+
+/* One orientation with per-slot dumps (each array grid*grid, slot id = x*grid + y; any may be
+ * NULL): pixel count after Pixelize, count at the end, slot Vo and Vss from vslotPair. */
+int orc_one(const orc_opts *opt, int32_t nTri, const int32_t *tri, int32_t nVtx,
+            const float *vtx, const float *vnrm, const float ypr[3], orc_record *out,
+            int32_t *cnt_pre, int32_t *cnt_final, int32_t *slot_vo, int32_t *slot_vss);
+
+/* Pair one hand-made slot (rows of z and nZ, any order, distinct keys) the way vslotPair does
+ * (STomoSh_INT3.cpp:1041-1062).  Returns the new slot length; writes Vo/Vss and row types. */
+int orc_pair_slot(int32_t n, const int16_t *z, const int16_t *nz, float theta_c, int32_t explicit_ss,
+                  int32_t *vo, int32_t *vss, int16_t *z_out, int16_t *nz_out, int16_t *type_out);
+
+/* VolToMass (STomoVolMassInfo.cpp:42-81) on the record's float totals. */
+void orc_vol_to_mass(const orc_opts *opt, orc_record *r);
+
+/* First strict minimum with initial value 1e5 (Tomo_Shell2026.cpp:89-97). */
+int32_t orc_find_first_optimal(int32_t n, const float *mss);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

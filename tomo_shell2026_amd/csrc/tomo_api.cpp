@@ -1,0 +1,608 @@
+// tomo_api.cpp -- host side of libtomo_hip.so: option decoding, device buffers, launches,
+// record finishing (float totals, VolToMass, transform) and the legacy drop-in shim.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/tomo_hip.h"
+#include "tomo_device.h"
+
+namespace tomo {
+hipError_t launch_sweep(const SweepParams &p, int n_groups, hipStream_t stream);
+}
+
+using namespace tomo;
+
+namespace {
+
+thread_local std::string g_err = "";
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                      \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess) return fail(TOMO_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr float kMargin = 0.001f;
+
+struct Options {
+  float f32[12];
+  int32_t i32[11];
+  int32_t grid;
+  float dvoxel, theta_c, area, wall, fcore, fclad, fss, css, rho, bed_in, bed_out, bed_thick;
+  int explicit_ss, shell_mesh, bed_type, bed_r, do_shell;
+  float shell_t, shell_off, h2;
+  int32_t cut_ssb, cut_nvb;
+};
+
+// S3DPrinterInfo::Set (S3DPrinterInfo.cpp:124-145) and the constants triVoxel derives
+// (STomoSh_INT3.cpp:1160-1163).
+int decode_options(const float *f32, const int32_t *i32, int32_t grid, Options &o) {
+  if (!f32 || !i32) return fail(TOMO_E_INVALID_ARG, "null option array");
+  std::memcpy(o.f32, f32, sizeof(o.f32));
+  std::memcpy(o.i32, i32, sizeof(o.i32));
+  o.dvoxel = f32[0]; o.theta_c = f32[1]; o.area = f32[2]; o.wall = f32[3];
+  o.fcore = f32[4]; o.fclad = f32[5]; o.fss = f32[6]; o.css = f32[7]; o.rho = f32[8];
+  o.bed_in = f32[9]; o.bed_out = f32[10]; o.bed_thick = f32[11];
+  o.explicit_ss = (float)i32[1] > kMargin;
+  o.shell_mesh = (float)i32[2] > kMargin;
+  o.shell_t = (float)i32[10] * 0.001f;
+  o.bed_type = i32[9];
+  o.grid = grid > 0 ? grid : (i32[3] > 0 ? i32[3] : 256);
+  if (o.grid < 8 || o.grid > 4096) return fail(TOMO_E_INVALID_ARG, "grid edge out of range [8, 4096]");
+  if (o.bed_type < 0 || o.bed_type > 3) return fail(TOMO_E_INVALID_ARG, "BedType must be 0..3");
+  o.bed_r = (int)o.bed_out;
+  if (o.bed_type != 0 && o.bed_r > kMaxBedR)
+    return fail(TOMO_E_INVALID_ARG, "BedOuterBound above the supported maximum (16)");
+  o.h2 = o.dvoxel * 0.5f;
+  o.shell_off = (float)(int)(o.shell_t + 0.999f);
+  o.do_shell = o.shell_mesh && o.shell_t > kMargin;
+  // createShadowBriefly's float tests (STomoSh_INT3.cpp:755-776) as integer cut points.
+  const float thr = (float)(-1.0 * std::sin((double)o.theta_c));
+  int32_t cs = -32769, cn = 32768;
+  for (int32_t n = -32768; n <= 32767; n++) {
+    const float f = (float)n / 1000.0f;
+    if (f < thr) cs = n;
+    if (f > thr && n < cn) cn = n;
+  }
+  o.cut_ssb = cs;
+  o.cut_nvb = cn;
+  return TOMO_OK;
+}
+
+// SMatrix4f::YPR (SMatrix4f.cpp:213-240), host libm.
+void ypr_matrix(float yaw, float pitch, float roll, float *R) {
+  const float ca = cosf(roll), cb = cosf(pitch), cg = cosf(yaw);
+  const float sa = sinf(roll), sb = sinf(pitch), sg = sinf(yaw);
+  R[0] = ca * cb;
+  R[1] = ca * sb * sg - sa * cg;
+  R[2] = ca * sb * cg + sa * sg;
+  R[3] = sa * cb;
+  R[4] = sa * sb * sg + ca * cg;
+  R[5] = sa * sb * cg - ca * sg;
+  R[6] = -sb;
+  R[7] = cb * sg;
+  R[8] = cb * cg;
+}
+
+// STomoVolMassInfo::VolToMass (STomoVolMassInfo.cpp:42-81); out[22] in the struct's order
+// (STomoVolMassInfo.h:24-30).
+void vol_to_mass(const Options &o, float *vm) {
+  float &Vss = vm[4], &Vss_clad = vm[5], &Vss_core = vm[6], &Ass_clad = vm[7];
+  float &Vo = vm[8], &Vo_clad = vm[9], &Vo_core = vm[10];
+  float &Vbed = vm[11], &Mbed = vm[12], &Mss = vm[13], &Mss_clad = vm[14], &Mss_core = vm[15];
+  float &Mo = vm[16], &Mo_clad = vm[17], &Mo_core = vm[18], &Mtotal = vm[19];
+  Vo_clad = Vo_core = Mo_clad = Mo_core = Mo = 0.f;
+  Vss_clad = Vss_core = Ass_clad = Mss_clad = Mss_core = Mss = 0.f;
+  Mbed = Mtotal = 0.f;
+  Vo_clad = o.area * o.wall;
+  Vo_core = Vo - Vo_clad;
+  Mo_clad = Vo_clad * o.fclad * o.rho;
+  Mo_core = Vo_core * o.fcore * o.rho;
+  Mo = Mo_core + Mo_clad;
+  if (o.explicit_ss) {
+    if (Vss > kMargin) {
+      const float radius = powf(Vss / (4.f / 3.f * 3.141592f), 1.f / 3.f);
+      Ass_clad = (float)(4. * 3.141592 * (double)radius * (double)radius);
+      Vss_clad = Ass_clad * o.wall;
+      if (Vss_clad > Vss) Vss_clad = Vss;
+      Vss_core = Vss - Vss_clad;
+      Mss_clad = Vss_clad * o.fclad * o.rho;
+      Mss_core = Vss_core * o.fcore * o.rho;
+      Mss = Mss_core + Mss_clad;
+    }
+  } else {
+    Mss = Vss * o.fss * o.css * o.rho;
+  }
+  Mbed = Vbed * o.wall * o.rho;
+  Mss += Mbed;
+  Mtotal = Mo + Mss;
+}
+
+ScratchLayout make_layout(int32_t G) {
+  ScratchLayout L{};
+  const int64_t G2 = (int64_t)G * G;
+  const int32_t pool = (int32_t)(G2 > 65536 ? G2 : 65536);
+  int64_t at = 0;
+  auto take = [&](int64_t bytes) {
+    const int64_t here = at;
+    at += (bytes + 255) & ~int64_t(255);
+    return here;
+  };
+  L.count = take(4 * G2);
+  L.keys = take(4 * G2 * kInline);
+  L.ranks = take(4 * G2 * kInline);
+  L.bedinfo = take(4 * G2);
+  L.big = take(4 * G2);
+  L.off = take(4 * G2);
+  L.cursor = take(4 * G2);
+  L.pool_slot = take(4 * (int64_t)pool);
+  L.pool_key = take(4 * (int64_t)pool);
+  L.pool_rank = take(4 * (int64_t)pool);
+  L.cmp_key = take(4 * (int64_t)pool);
+  L.cmp_rank = take(4 * (int64_t)pool);
+  L.bytes = at;
+  L.pool = pool;
+  return L;
+}
+
+}  // namespace
+
+struct tomo_handle {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool timed = false;
+  Options opt{};
+  // mesh
+  int32_t n_tri = 0, n_vtx = 0;
+  float center[3] = {0, 0, 0};
+  int32_t *d_tri = nullptr;
+  float *d_soa = nullptr;  // 6 * n_vtx: x, y, z, nx, ny, nz
+  float *d_bed_dist = nullptr;
+  // orientations
+  int32_t n_orient = 0;
+  std::vector<float> rot;
+  float *d_rot = nullptr;
+  DevRecord *d_rec = nullptr;
+  int32_t rec_cap = 0;
+  // scratch
+  ScratchLayout lay{};
+  uint8_t *d_scratch = nullptr;
+  int32_t scratch_groups = 0;
+  int32_t groups_req = 0;
+  int32_t last_groups = 0;
+  int32_t last_begin = 0, last_end = 0;
+  std::vector<float> vbed_seq;
+};
+
+namespace {
+
+int ensure_scratch(tomo_handle *h, int32_t groups) {
+  if (groups <= h->scratch_groups) return TOMO_OK;
+  if (h->d_scratch) {
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipFree(h->d_scratch));
+    h->d_scratch = nullptr;
+    h->scratch_groups = 0;
+  }
+  const size_t bytes = (size_t)h->lay.bytes * (size_t)groups;
+  if (hipMalloc(&h->d_scratch, bytes) != hipSuccess) {
+    h->d_scratch = nullptr;
+    return fail(TOMO_E_OOM, "scratch allocation failed");
+  }
+  // Slot counters must start at zero; every orientation leaves them at zero again.
+  HIP_TRY(hipMemsetAsync(h->d_scratch, 0, bytes, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->scratch_groups = groups;
+  return TOMO_OK;
+}
+
+int32_t auto_groups(tomo_handle *h, int32_t n) {
+  int cus = 256;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, h->device) == hipSuccess && prop.multiProcessorCount > 0)
+    cus = prop.multiProcessorCount;
+  int32_t g = h->groups_req > 0 ? h->groups_req : cus * 4;
+  // keep scratch within a quarter of free memory
+  size_t freeb = 0, total = 0;
+  if (hipMemGetInfo(&freeb, &total) == hipSuccess && h->lay.bytes > 0) {
+    const size_t cap = (freeb / 4) / (size_t)h->lay.bytes;
+    if (cap >= 1 && (size_t)g > cap && h->scratch_groups < g) g = (int32_t)cap;
+  }
+  if (g > n) g = n;
+  if (g < 1) g = 1;
+  return g;
+}
+
+void finish_record(const tomo_handle *h, const DevRecord &d, const float *R, tomo_record &r) {
+  std::memset(&r, 0, sizeof(r));
+  r.Va = d.Va; r.Vb = d.Vb; r.Vo = d.Vo; r.Vss = d.Vss; r.Vnv = d.Vnv; r.Vtc = d.Vtc;
+  r.Vbed = d.Vbed; r.n_entries = d.n_entries; r.n_slots = d.n_slots; r.n_final = d.n_final;
+  r.chk_pre = d.chk_pre; r.chk_final = d.chk_final; r.n_bed = d.n_bed;
+  int32_t flags = d.flags;
+  const int64_t lim = (int64_t)1 << 24;
+  if (d.aVo > lim || d.aVss > lim || d.aVa > lim || d.aVb > lim || d.aVnv > lim || d.aVtc > lim)
+    flags |= TOMO_F_FP32_INEXACT;
+  r.flags = flags;
+  // Integer partial sums below 2^24 are exact in float32, so the reference's running totals
+  // equal the integers (Pairing :386-392 and Calculate :1233-1240 accumulate ints into floats).
+  r.fVa = (float)d.Va; r.fVb = (float)d.Vb; r.fVtc = (float)d.Vtc; r.fVnv = (float)d.Vnv;
+  r.fVss = (float)d.Vss; r.fVo = (float)d.Vo;
+  const size_t nb = (size_t)(d.n_bed < 0 ? 0 : d.n_bed);
+  r.fVbed = nb < h->vbed_seq.size() ? h->vbed_seq[nb] : (float)nb * h->opt.bed_thick;
+  float vm[22] = {};
+  vm[0] = r.fVa; vm[1] = r.fVb; vm[2] = r.fVtc; vm[3] = r.fVnv; vm[4] = r.fVss; vm[8] = r.fVo;
+  vm[11] = r.fVbed;
+  vol_to_mass(h->opt, vm);
+  r.Mo = vm[16]; r.Mss = vm[13]; r.Mtotal = vm[19];
+  // mat4x4 (STomoSh_Base.cpp:111-123)
+  const float m = -h->center[0], n = -h->center[1], o = -h->center[2];
+  const float tr[3] = {-d.kx, -d.ky, -d.kz};
+  for (int row = 0; row < 3; row++) {
+    r.mat4x4[row * 4 + 0] = R[row * 3 + 0];
+    r.mat4x4[row * 4 + 1] = R[row * 3 + 1];
+    r.mat4x4[row * 4 + 2] = R[row * 3 + 2];
+    r.mat4x4[row * 4 + 3] = tr[row] + R[row * 3 + 0] * m + R[row * 3 + 1] * n + R[row * 3 + 2] * o;
+  }
+  r.mat4x4[15] = 1.0f;
+}
+
+void vm_from_record(const tomo_handle *h, const tomo_record &r, float *vm) {
+  std::memset(vm, 0, sizeof(float) * 22);
+  vm[0] = r.fVa; vm[1] = r.fVb; vm[2] = r.fVtc; vm[3] = r.fVnv; vm[4] = r.fVss; vm[8] = r.fVo;
+  vm[11] = r.fVbed;
+  vol_to_mass(h->opt, vm);
+}
+
+SweepParams base_params(tomo_handle *h) {
+  SweepParams p{};
+  p.tri = h->d_tri;
+  p.vx = h->d_soa;
+  p.vy = h->d_soa + h->n_vtx;
+  p.vz = h->d_soa + 2 * (size_t)h->n_vtx;
+  p.nx = h->d_soa + 3 * (size_t)h->n_vtx;
+  p.ny = h->d_soa + 4 * (size_t)h->n_vtx;
+  p.nz = h->d_soa + 5 * (size_t)h->n_vtx;
+  p.n_tri = h->n_tri;
+  p.n_vtx = h->n_vtx;
+  p.cx = h->center[0]; p.cy = h->center[1]; p.cz = h->center[2];
+  p.grid = h->opt.grid;
+  p.bed_out = h->opt.bed_out;
+  p.bed_in = h->opt.bed_in;
+  p.h2 = h->opt.h2;
+  p.shell_off = h->opt.shell_off;
+  p.do_shell = h->opt.do_shell;
+  p.pp.cut_ssb = h->opt.cut_ssb;
+  p.pp.cut_nvb = h->opt.cut_nvb;
+  p.pp.explicit_ss = h->opt.explicit_ss;
+  p.pp.bed_type = h->opt.bed_type;
+  p.pp.bed_r = h->opt.bed_r;
+  p.bed_dist = h->d_bed_dist;
+  p.lay = h->lay;
+  return p;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *tomo_last_error(void) { return g_err.c_str(); }
+
+int32_t tomo_find_first_optimal(int32_t n, const float *mss) {
+  if (n <= 1 || !mss) return 0;
+  int32_t best = 0;
+  float mv = 1e5f;
+  for (int32_t i = 0; i < n; i++)
+    if (mss[i] < mv) {
+      mv = mss[i];
+      best = i;
+    }
+  return best;
+}
+
+int tomo_create(const float *f32, const int32_t *i32, int32_t grid, tomo_handle **out) {
+  if (!out) return fail(TOMO_E_INVALID_ARG, "null output handle");
+  *out = nullptr;
+  Options o;
+  int rc = decode_options(f32, i32, grid, o);
+  if (rc) return rc;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(TOMO_E_NO_DEVICE, "no HIP device");
+  tomo_handle *h = new (std::nothrow) tomo_handle();
+  if (!h) return fail(TOMO_E_OOM, "handle allocation failed");
+  h->opt = o;
+  if (hipGetDevice(&h->device) != hipSuccess ||
+      hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
+    tomo_destroy(h);
+    return fail(TOMO_E_HIP, "stream/event creation failed");
+  }
+  h->lay = make_layout(o.grid);
+  // IsBedCandidate distances dist2D (STomoSh_INT3.cpp:1326-1329) for dx, dy in [-r, r).
+  const int r = o.bed_r > 0 ? o.bed_r : 1;
+  std::vector<float> dist((size_t)(2 * r) * (2 * r));
+  for (int dx = -r; dx < r; dx++)
+    for (int dy = -r; dy < r; dy++) {
+      const float fx = (float)dx, fy = (float)dy;
+      dist[(size_t)(dx + r) * (2 * r) + (dy + r)] = sqrtf(fx * fx + fy * fy);
+    }
+  if (hipMalloc(&h->d_bed_dist, dist.size() * sizeof(float)) != hipSuccess ||
+      hipMemcpy(h->d_bed_dist, dist.data(), dist.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+    tomo_destroy(h);
+    return fail(TOMO_E_HIP, "bed table upload failed");
+  }
+  // Calculate adds sumType(Bed) * BedThickness slot by slot (STomoSh_INT3.cpp:1242): the float
+  // total after k bed rows is a k-fold running sum.
+  const size_t g2 = (size_t)o.grid * o.grid;
+  h->vbed_seq.resize(g2 + 1);
+  float acc = 0.0f;
+  h->vbed_seq[0] = 0.0f;
+  for (size_t k = 1; k <= g2; k++) {
+    acc += (float)1 * o.bed_thick;
+    h->vbed_seq[k] = acc;
+  }
+  *out = h;
+  return TOMO_OK;
+}
+
+int tomo_set_mesh(tomo_handle *h, int32_t n_tri, const int32_t *tri, int32_t n_vtx,
+                  const float *vtx, const float *vtx_nrm) {
+  if (!h) return fail(TOMO_E_INVALID_ARG, "null handle");
+  if (n_tri < 0 || n_vtx < 0 || (n_tri > 0 && (!tri || !vtx || !vtx_nrm)) || (n_vtx > 0 && (!vtx || !vtx_nrm)))
+    return fail(TOMO_E_INVALID_ARG, "bad mesh arguments");
+  for (int64_t i = 0; i < (int64_t)n_tri * 3; i++)
+    if (tri[i] < 0 || tri[i] >= n_vtx) return fail(TOMO_E_INVALID_ARG, "triangle index out of range");
+  if (n_tri >= (1 << 30)) return fail(TOMO_E_INVALID_ARG, "too many triangles");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  if (h->d_tri) { HIP_TRY(hipFree(h->d_tri)); h->d_tri = nullptr; }
+  if (h->d_soa) { HIP_TRY(hipFree(h->d_soa)); h->d_soa = nullptr; }
+  h->n_tri = n_tri;
+  h->n_vtx = n_vtx;
+  // raw AABB centre (STomoAABB3Df.cpp:42-72): float add, then *0.5 in double
+  float mn[3] = {1e5f, 1e5f, 1e5f}, mx[3] = {-1e5f, -1e5f, -1e5f};
+  for (int32_t v = 0; v < n_vtx; v++)
+    for (int k = 0; k < 3; k++) {
+      const float c = vtx[(size_t)v * 3 + k];
+      mn[k] = (mn[k] <= c) ? mn[k] : c;
+      mx[k] = (mx[k] >= c) ? mx[k] : c;
+    }
+  for (int k = 0; k < 3; k++) h->center[k] = (float)((double)(mn[k] + mx[k]) * 0.5);
+  std::vector<float> soa((size_t)6 * (n_vtx > 0 ? n_vtx : 1));
+  for (int32_t v = 0; v < n_vtx; v++)
+    for (int k = 0; k < 3; k++) {
+      soa[(size_t)k * n_vtx + v] = vtx[(size_t)v * 3 + k];
+      soa[(size_t)(3 + k) * n_vtx + v] = vtx_nrm[(size_t)v * 3 + k];
+    }
+  HIP_TRY(hipMalloc(&h->d_soa, soa.size() * sizeof(float)));
+  HIP_TRY(hipMemcpy(h->d_soa, soa.data(), soa.size() * sizeof(float), hipMemcpyHostToDevice));
+  HIP_TRY(hipMalloc(&h->d_tri, sizeof(int32_t) * 3 * (size_t)(n_tri > 0 ? n_tri : 1)));
+  if (n_tri > 0)
+    HIP_TRY(hipMemcpy(h->d_tri, tri, sizeof(int32_t) * 3 * (size_t)n_tri, hipMemcpyHostToDevice));
+  return TOMO_OK;
+}
+
+int tomo_set_orientations(tomo_handle *h, int32_t n, const float *ypr) {
+  if (!h) return fail(TOMO_E_INVALID_ARG, "null handle");
+  if (n < 0 || (n > 0 && !ypr)) return fail(TOMO_E_INVALID_ARG, "bad orientation arguments");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->rot.assign((size_t)9 * (n > 0 ? n : 1), 0.0f);
+  for (int32_t i = 0; i < n; i++)
+    ypr_matrix(ypr[(size_t)i * 3 + 0], ypr[(size_t)i * 3 + 1], ypr[(size_t)i * 3 + 2], &h->rot[(size_t)i * 9]);
+  if (n > h->rec_cap) {
+    if (h->d_rec) { HIP_TRY(hipFree(h->d_rec)); h->d_rec = nullptr; }
+    if (h->d_rot) { HIP_TRY(hipFree(h->d_rot)); h->d_rot = nullptr; }
+    h->rec_cap = 0;
+    if (hipMalloc(&h->d_rec, sizeof(DevRecord) * (size_t)n) != hipSuccess ||
+        hipMalloc(&h->d_rot, sizeof(float) * 9 * (size_t)n) != hipSuccess)
+      return fail(TOMO_E_OOM, "orientation buffers");
+    h->rec_cap = n;
+  }
+  if (n > 0)
+    HIP_TRY(hipMemcpy(h->d_rot, h->rot.data(), sizeof(float) * 9 * (size_t)n, hipMemcpyHostToDevice));
+  h->n_orient = n;
+  return TOMO_OK;
+}
+
+int tomo_set_launch(tomo_handle *h, int32_t n_groups) {
+  if (!h || n_groups < 0) return fail(TOMO_E_INVALID_ARG, "bad launch arguments");
+  h->groups_req = n_groups;
+  return TOMO_OK;
+}
+
+int tomo_run(tomo_handle *h, int32_t begin, int32_t end) {
+  if (!h) return fail(TOMO_E_INVALID_ARG, "null handle");
+  if (begin < 0 || end > h->n_orient || begin > end) return fail(TOMO_E_INVALID_ARG, "bad orientation range");
+  if (!h->d_soa || !h->d_tri) return fail(TOMO_E_STATE, "no mesh");
+  h->last_begin = begin;
+  h->last_end = end;
+  h->timed = false;
+  const int32_t n = end - begin;
+  if (n == 0) return TOMO_OK;
+  const int32_t groups = auto_groups(h, n);
+  int rc = ensure_scratch(h, groups);
+  if (rc) return rc;
+  SweepParams p = base_params(h);
+  p.rot = h->d_rot + (size_t)begin * 9;
+  p.n_orient = n;
+  p.rec = h->d_rec + begin;
+  p.scratch = h->d_scratch;
+  HIP_TRY(hipEventRecord(h->ev0, h->stream));
+  HIP_TRY(launch_sweep(p, groups, h->stream));
+  HIP_TRY(hipEventRecord(h->ev1, h->stream));
+  h->timed = true;
+  h->last_groups = groups;
+  return TOMO_OK;
+}
+
+int tomo_sync(tomo_handle *h) {
+  if (!h) return fail(TOMO_E_INVALID_ARG, "null handle");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return TOMO_OK;
+}
+
+int tomo_get_records(tomo_handle *h, int32_t begin, int32_t end, tomo_record *out) {
+  if (!h || (!out && end > begin)) return fail(TOMO_E_INVALID_ARG, "null argument");
+  if (begin < 0 || end > h->n_orient || begin > end) return fail(TOMO_E_INVALID_ARG, "bad orientation range");
+  const int32_t n = end - begin;
+  if (n == 0) return TOMO_OK;
+  std::vector<DevRecord> d((size_t)n);
+  HIP_TRY(hipMemcpyAsync(d.data(), h->d_rec + begin, sizeof(DevRecord) * (size_t)n, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  for (int32_t i = 0; i < n; i++) finish_record(h, d[(size_t)i], &h->rot[(size_t)(begin + i) * 9], out[i]);
+  return TOMO_OK;
+}
+
+int tomo_sweep(tomo_handle *h, int32_t n, const float *ypr, tomo_record *out) {
+  int rc = tomo_set_orientations(h, n, ypr);
+  if (rc) return rc;
+  rc = tomo_run(h, 0, n);
+  if (rc) return rc;
+  return tomo_get_records(h, 0, n, out);
+}
+
+float tomo_last_kernel_ms(tomo_handle *h) {
+  if (!h || !h->timed) return -1.0f;
+  float ms = -1.0f;
+  if (hipEventSynchronize(h->ev1) != hipSuccess) return -1.0f;
+  if (hipEventElapsedTime(&ms, h->ev0, h->ev1) != hipSuccess) return -1.0f;
+  return ms;
+}
+
+int tomo_get_info(tomo_handle *h, int32_t *grid, int32_t *n_groups, int64_t *scratch_bytes) {
+  if (!h) return fail(TOMO_E_INVALID_ARG, "null handle");
+  if (grid) *grid = h->opt.grid;
+  if (n_groups) *n_groups = h->last_groups;
+  if (scratch_bytes) *scratch_bytes = (int64_t)h->lay.bytes * h->scratch_groups;
+  return TOMO_OK;
+}
+
+int tomo_debug_slots(tomo_handle *h, const float ypr[3], tomo_record *rec, int32_t *cnt_pre,
+                     int32_t *cnt_final, int32_t *slot_vo, int32_t *slot_vss) {
+  if (!h || !ypr || !rec) return fail(TOMO_E_INVALID_ARG, "null argument");
+  if (!h->d_soa || !h->d_tri) return fail(TOMO_E_STATE, "no mesh");
+  int rc = tomo_set_orientations(h, 1, ypr);
+  if (rc) return rc;
+  rc = ensure_scratch(h, 1);
+  if (rc) return rc;
+  const size_t g2 = (size_t)h->opt.grid * h->opt.grid;
+  int32_t *d = nullptr;
+  HIP_TRY(hipMalloc(&d, sizeof(int32_t) * 4 * g2));
+  HIP_TRY(hipMemsetAsync(d, 0, sizeof(int32_t) * 4 * g2, h->stream));
+  SweepParams p = base_params(h);
+  p.rot = h->d_rot;
+  p.n_orient = 1;
+  p.rec = h->d_rec;
+  p.scratch = h->d_scratch;
+  p.dbg_pre = d;
+  p.dbg_final = d + g2;
+  p.dbg_vo = d + 2 * g2;
+  p.dbg_vss = d + 3 * g2;
+  hipError_t e = launch_sweep(p, 1, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  int32_t *outs[4] = {cnt_pre, cnt_final, slot_vo, slot_vss};
+  for (int k = 0; k < 4 && e == hipSuccess; k++)
+    if (outs[k]) e = hipMemcpy(outs[k], d + k * g2, sizeof(int32_t) * g2, hipMemcpyDeviceToHost);
+  hipFree(d);
+  if (e != hipSuccess) return fail(TOMO_E_HIP, std::string("debug sweep: ") + hipGetErrorString(e));
+  h->last_begin = 0;
+  h->last_end = 1;
+  return tomo_get_records(h, 0, 1, rec);
+}
+
+int tomo_destroy(tomo_handle *h) {
+  if (!h) return TOMO_OK;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  if (h->d_scratch) hipFree(h->d_scratch);
+  if (h->d_rec) hipFree(h->d_rec);
+  if (h->d_rot) hipFree(h->d_rot);
+  if (h->d_tri) hipFree(h->d_tri);
+  if (h->d_soa) hipFree(h->d_soa);
+  if (h->d_bed_dist) hipFree(h->d_bed_dist);
+  if (h->ev0) hipEventDestroy(h->ev0);
+  if (h->ev1) hipEventDestroy(h->ev1);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+  return TOMO_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Legacy shim: global outputs, freed on the next call or OnDestroy (Tomo_Shell2026.cpp:19-87).
+static std::vector<float> g_mss, g_mo, g_vtc;
+static float g_vm[22];
+static float g_mat[16];
+static bool g_have = false;
+static std::vector<std::vector<int16_t>> g_pix(11);
+
+int32_t TomoSh_HIP(float *f32, int32_t *i32, float *YPR, int32_t *tri, float *vtx, float *vtx_nrm,
+                   float *tri_nrm, int32_t *chull_tri, float *chull_vtx, float *chull_trinrm) {
+  (void)tri_nrm; (void)chull_tri; (void)chull_vtx; (void)chull_trinrm;
+  OnDestroy();
+  if (!f32 || !i32) return fail(TOMO_E_INVALID_ARG, "null header array");
+  const int32_t n_tri = i32[4], n_vtx = i32[5], n_ypr = i32[6];
+  if (n_ypr < 0 || (n_ypr > 0 && !YPR)) return fail(TOMO_E_INVALID_ARG, "bad YPR");
+  tomo_handle *h = nullptr;
+  int rc = tomo_create(f32, i32, 0, &h);
+  if (rc) return rc;
+  rc = tomo_set_mesh(h, n_tri, tri, n_vtx, vtx, vtx_nrm);
+  std::vector<tomo_record> rec((size_t)(n_ypr > 0 ? n_ypr : 1));
+  if (!rc) rc = tomo_sweep(h, n_ypr, YPR, rec.data());
+  if (rc) {
+    tomo_destroy(h);
+    return rc;
+  }
+  g_mss.assign((size_t)n_ypr + 2, 0.0f);
+  g_mo.assign((size_t)n_ypr + 2, 0.0f);
+  g_vtc.assign((size_t)n_ypr + 2, 0.0f);
+  for (int32_t i = 0; i < n_ypr; i++) {
+    g_mss[i] = rec[i].Mss;
+    g_mo[i] = rec[i].Mo;
+    g_vtc[i] = rec[i].fVtc;
+  }
+  const int32_t opt = tomo_find_first_optimal(n_ypr, g_mss.data());
+  if (n_ypr > 0) {
+    vm_from_record(h, rec[opt], g_vm);
+    std::memcpy(g_mat, rec[opt].mat4x4, sizeof(g_mat));
+  }
+  g_have = true;
+  tomo_destroy(h);
+  return opt;
+}
+
+float *getMss(void) { return g_have ? g_mss.data() : nullptr; }
+float *getMo(void) { return g_have ? g_mo.data() : nullptr; }
+float *getVtc(void) { return g_have ? g_vtc.data() : nullptr; }
+float *getVolMassInfo(void) { return g_vm; }
+float *getMat4x4(void) { return g_have ? g_mat : nullptr; }
+
+int32_t getnData2i(int16_t t) {
+  if (t < 0 || t >= 11) return 0;
+  return (int32_t)(g_pix[(size_t)t].size() / 6);
+}
+
+int16_t *getpData2i(int16_t t) {
+  if (t < 0 || t >= 11 || g_pix[(size_t)t].empty()) return nullptr;
+  return g_pix[(size_t)t].data();
+}
+
+void OnDestroy(void) {
+  g_mss.clear();
+  g_mo.clear();
+  g_vtc.clear();
+  std::memset(g_vm, 0, sizeof(g_vm));
+  std::memset(g_mat, 0, sizeof(g_mat));
+  for (auto &v : g_pix) v.clear();
+  g_have = false;
+}
+
+}  // extern "C"

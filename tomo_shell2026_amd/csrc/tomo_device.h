@@ -1,0 +1,60 @@
+// tomo_device.h -- launch parameters and device record shared by the HIP kernels and the host
+// side of libtomo_hip.so.
+#pragma once
+#include <stdint.h>
+
+#include "tomo_slot.h"
+
+namespace tomo {
+
+constexpr int kThreads = 256;          // work-group size: 4 waves of 64
+constexpr int kWaves = kThreads / 64;
+constexpr int kSmall = 8;              // slots with <= 8 entries are paired in registers
+constexpr int kInline = 16;            // entries stored inline per slot (layer-major)
+constexpr int kBigCap = 512;           // entries a single slot may hold for exact resolution
+constexpr int kMaxBedR = 16;           // int(BedOuterBound) supported by the bed kernel
+
+// Raw device record, one per orientation.  Totals are exact 64-bit integers; the host turns
+// them into the reference's float totals and masses.
+struct DevRecord {
+  int64_t Va, Vb, Vo, Vss, Vnv, Vtc, Vbed;
+  int64_t n_entries, n_slots, n_final;
+  uint64_t chk_pre, chk_final;
+  int64_t aVo, aVss, aVa, aVb, aVnv, aVtc;  // sums of |per-slot value| (float exactness test)
+  int32_t flags, n_bed;
+  float kx, ky, kz;                          // placement offsets (rotated AABB min - Bo, - Bo, 0)
+  int32_t pad;
+};
+
+// Per-work-group scratch, carved from one device allocation.  G2 = grid * grid.
+struct ScratchLayout {
+  int64_t count, keys, ranks, bedinfo, big, off, cursor, pool_slot, pool_key, pool_rank, cmp_key,
+      cmp_rank;  // byte offsets inside one work-group's region
+  int64_t bytes; // bytes per work-group region
+  int32_t pool;  // overflow pool capacity (entries)
+};
+
+struct SweepParams {
+  // mesh (structure of arrays, raw coordinates in voxel units)
+  const int32_t *tri;
+  const float *vx, *vy, *vz, *nx, *ny, *nz;
+  int32_t n_tri, n_vtx;
+  float cx, cy, cz;        // raw AABB centre (STomoSh_Base.cpp:75-82)
+  // orientation set
+  const float *rot;        // [n_orient][9] row-major rotation matrices
+  int32_t n_orient;
+  // options
+  int32_t grid;
+  float bed_out, bed_in, h2, shell_off;
+  int32_t do_shell;
+  PairParams pp;
+  const float *bed_dist;   // [2r][2r] float distances sqrtf(dx^2 + dy^2), dx,dy in [-r, r)
+  // scratch and outputs
+  uint8_t *scratch;
+  ScratchLayout lay;
+  DevRecord *rec;
+  // optional per-slot dumps for one orientation (grid*grid each, zero-filled by the host)
+  int32_t *dbg_pre, *dbg_final, *dbg_vo, *dbg_vss;
+};
+
+}  // namespace tomo

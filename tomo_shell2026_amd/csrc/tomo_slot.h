@@ -1,0 +1,206 @@
+// tomo_slot.h -- per-slot sort / pair / shadow logic shared by the HIP kernels and the host
+// unit test (compiled for both: TOMO_HD expands to __host__ __device__ under hipcc).
+//
+// A slot entry is a packed 32-bit key: ((z + 32768) << 16) | (nZ + 32768) with z and nZ the
+// int16 values SetBit stores (STomoVoxelSpaceInfo.cpp:179-180).  Descending key order is the
+// order sortSlotByZ produces: z descending, ties by nZ descending (STomoSh_INT3.cpp:398-446).
+//
+// Semantics restated from the brief vslotPair (STomoSh_INT3.cpp:1041-1062):
+//   splitAlBe (:493-513)            alpha iff nZ >= 1, beta iff nZ <= -1
+//   matchAlBePairBriefly (:583-611) an alpha is kept iff the previous alpha/beta row is a beta
+//                                   (or none); a beta is kept iff the previous one is an alpha
+//   createVoPixels (:934-960)       Vo = max(sum z(alpha) - sum z(beta), 0)
+//   createShadowBriefly (:749-812)  SSB/NVB open on kept betas below/above the theta cut,
+//                                   SSA/NVA close on the next kept alpha; open shadows get a
+//                                   z = 0 floor row while the slot has fewer than 63 rows
+//   createVss_Explicit (:962-987)   Vss = sum z(SSB) - sum z(SSA)
+//   createTCPixels + createVss_Implicit (:734-747, :989-1025)
+//   Calculate / sumType (:1211-1255, :679-694) per-slot Va, Vb, Vnv, Vtc.
+#pragma once
+#include <stdint.h>
+
+#ifdef __HIPCC__
+#define TOMO_HD __host__ __device__ __forceinline__
+#else
+#define TOMO_HD inline
+#endif
+
+namespace tomo {
+
+enum : int32_t {
+  T_AL = 1, T_BE = 2, T_SSB = 4, T_SSA = 8, T_SS = 16, T_BED = 32, T_VO = 64, T_VSS = 128,
+  T_TC = 256, T_NVB = 512, T_NVA = 1024
+};
+
+enum : int32_t {
+  F_FP32_INEXACT = 1, F_INT16_WRAP = 2, F_FOOTPRINT_OOB = 4, F_CAP_DROP = 8,
+  F_SENTINEL_DROP = 16, F_RANK_TIE = 32, F_UNRESOLVED = 64
+};
+
+constexpr int kMaxPix = 63;  // pixel rows per slot (nSlotCapacityHeight - 1)
+
+TOMO_HD uint32_t make_key(int32_t z16, int32_t nz16) {
+  return ((uint32_t)(z16 + 32768) << 16) | (uint32_t)(nz16 + 32768);
+}
+TOMO_HD int32_t key_z(uint32_t k) { return (int32_t)(k >> 16) - 32768; }
+TOMO_HD int32_t key_nz(uint32_t k) { return (int32_t)(k & 0xffffu) - 32768; }
+
+struct PairParams {
+  int32_t cut_ssb;   // beta rows with nZ <= cut_ssb are support-structure casters
+  int32_t cut_nvb;   // beta rows with nZ >= cut_nvb are non-vertical casters
+  int32_t explicit_ss;
+  int32_t bed_type;  // 0 none, 1 brim, 2 raft, 3 skirt
+  int32_t bed_r;     // int(BedOuterBound)
+};
+
+// Per-slot result.
+struct SlotResult {
+  int32_t vo, vss, va, vb, vnv, vtc;  // pairing and Calculate sums for this slot
+  int32_t npre;      // distinct rows after Pixelize
+  int32_t n;         // rows after Pairing (distinct + floor rows)
+  int32_t sabs;      // sum |z| over those rows
+  int32_t last_z, last_type;
+  int32_t has10;     // a (z = 1, nZ = 0) row exists (a bed row would merge into it)
+  int32_t flags;
+};
+
+// Bed bookkeeping packed per slot: n (7 bits) | has10 | src | possible | conv.
+enum : uint32_t { BI_HAS10 = 1u << 7, BI_SRC = 1u << 8, BI_POSS = 1u << 9, BI_CONV = 1u << 10 };
+
+// GenerateBed / IsBedCandidate slot tests (STomoSh_INT3.cpp:1331-1373, 1408-1428):
+//  src      : the bottom row is at z = 0 and carries an alpha/beta/SSA/SS bit;
+//  possible : empty slot, or bottom row at z = 0 that is exactly NVA/Vo/Vss, or (raft) a
+//             beta/SSA bottom;
+//  conv     : a raft source that will take a bed row on its own turn and so stops being a
+//             source for every slot scanned after it.
+TOMO_HD uint32_t bed_info(const SlotResult &r, const PairParams &pp) {
+  uint32_t b = (uint32_t)r.n;
+  if (r.has10) b |= BI_HAS10;
+  const int32_t solid = T_AL | T_BE | T_SSA | T_SS;
+  bool src = r.n > 0 && r.last_z == 0 && (r.last_type & solid);
+  bool poss;
+  if (r.n == 0) {
+    poss = true;
+  } else if (r.last_z == 0) {
+    poss = (r.last_type == T_NVA || r.last_type == T_VO || r.last_type == T_VSS);
+    if (r.last_type & solid) poss = false;
+    if (pp.bed_type == 2 && (r.last_type & (T_BE | T_SSA))) poss = true;
+  } else {
+    poss = false;
+  }
+  bool conv = src && pp.bed_type == 2 && (r.last_type & (T_BE | T_SSA)) && pp.bed_r >= 1 &&
+              r.n < kMaxPix && !r.has10;
+  if (src) b |= BI_SRC;
+  if (poss) b |= BI_POSS;
+  if (conv) b |= BI_CONV;
+  return b;
+}
+
+// Pair N register-resident keys, the first c valid (c <= N <= 63), any order, duplicates
+// allowed.  Fully unrolled so the keys stay in registers.
+template <int N>
+TOMO_HD void pair_keys(uint32_t (&k)[N], int c, const PairParams &pp, SlotResult &r) {
+  // Sort descending.  N == 8 uses a 19-comparator network; other N an odd-even transposition
+  // network (used by the host unit test for larger N).
+  if constexpr (N == 8) {
+#define TOMO_CX(a, b)                                   \
+  {                                                     \
+    uint32_t hi = k[a] > k[b] ? k[a] : k[b];            \
+    uint32_t lo = k[a] > k[b] ? k[b] : k[a];            \
+    k[a] = hi;                                          \
+    k[b] = lo;                                          \
+  }
+    TOMO_CX(0, 2) TOMO_CX(1, 3) TOMO_CX(4, 6) TOMO_CX(5, 7)
+    TOMO_CX(0, 4) TOMO_CX(1, 5) TOMO_CX(2, 6) TOMO_CX(3, 7)
+    TOMO_CX(0, 1) TOMO_CX(2, 3) TOMO_CX(4, 5) TOMO_CX(6, 7)
+    TOMO_CX(2, 4) TOMO_CX(3, 5)
+    TOMO_CX(1, 4) TOMO_CX(3, 6)
+    TOMO_CX(1, 2) TOMO_CX(3, 4) TOMO_CX(5, 6)
+#undef TOMO_CX
+  } else {
+#pragma unroll
+    for (int pass = 0; pass < N; pass++) {
+#pragma unroll
+      for (int i = pass & 1; i + 1 < N; i += 2) {
+        uint32_t hi = k[i] > k[i + 1] ? k[i] : k[i + 1];
+        uint32_t lo = k[i] > k[i + 1] ? k[i + 1] : k[i];
+        k[i] = hi;
+        k[i + 1] = lo;
+      }
+    }
+  }
+  const uint32_t k10 = make_key(1, 0);
+  int npre = 0;
+  int32_t sabs = 0, has10 = 0;
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    bool d = (i < c) && (i == 0 || k[i] != k[i - 1]);
+    if (d) {
+      npre++;
+      int32_t z = key_z(k[i]);
+      sabs += z < 0 ? -z : z;
+      if (k[i] == k10) has10 = 1;
+    }
+  }
+  r.npre = npre;
+  r.sabs = sabs;
+  r.has10 = has10;
+  r.flags = 0;
+  r.vo = r.vss = r.va = r.vb = r.vnv = r.vtc = 0;
+  if (npre <= 1) {  // Pairing skips slots with a single pixel (STomoSh_INT3.cpp:386)
+    r.n = npre;
+    r.last_z = npre ? key_z(k[0]) : 0;
+    r.last_type = 0;
+    return;
+  }
+  bool started = false, ex = false, im = false, tc_done = false;
+  int32_t al = 0, be = 0, ssb = 0, ssa = 0, nvb = 0, nva = 0, tc = 0;
+  int32_t last_z = 0, last_type = 0;
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    bool d = (i < c) && (i == 0 || k[i] != k[i - 1]);
+    if (d) {
+      int32_t z = key_z(k[i]), nz = key_nz(k[i]);
+      bool isA = nz >= 1, isB = nz <= -1;
+      bool kA = false, kB = false;
+      if (!started && isA) { kA = true; started = true; }
+      else if (started && isB) { kB = true; started = false; }
+      int32_t ty = kA ? T_AL : (kB ? T_BE : 0);
+      if (kA) al += z;
+      if (kB) be += z;
+      if (kB) {
+        if (!ex && nz <= pp.cut_ssb) { ty |= T_SSB; ex = true; ssb += z; }
+        if (!im && nz >= pp.cut_nvb) { ty |= T_NVB; im = true; nvb += z; }
+      } else if (kA) {
+        if (ex) { ty |= T_SSA; ex = false; ssa += z; }
+        if (im) { ty |= T_NVA; im = false; nva += z; }
+        if (!pp.explicit_ss && !tc_done) { ty |= T_TC; tc_done = true; tc += z; }
+      }
+      last_z = z;
+      last_type = ty;
+    }
+  }
+  int n = npre;
+  if (ex) {
+    if (n < kMaxPix) { n++; last_z = 0; last_type = T_SSA; } else r.flags |= F_SENTINEL_DROP;
+  }
+  if (im) {
+    if (n < kMaxPix) { n++; last_z = 0; last_type = T_NVA; } else r.flags |= F_SENTINEL_DROP;
+  }
+  r.n = n;
+  r.last_z = last_z;
+  r.last_type = last_type;
+  r.vo = al > be ? al - be : 0;
+  r.va = al;
+  r.vb = be;
+  r.vnv = nvb - nva;
+  if (pp.explicit_ss) {
+    r.vss = ssb - ssa;
+    r.vtc = 0;
+  } else {
+    r.vss = -al + be + tc - nvb + nva;
+    r.vtc = tc;
+  }
+}
+
+}  // namespace tomo

@@ -1,0 +1,189 @@
+"""Seeded synthetic workloads for the support-tomography sweep.
+
+Meshes and orientation grids for the benchmark configurations:
+  * icosphere(subdiv=5, r=50 mm): 20,480 triangles, 1 mm voxels, 128 x 128 grid,
+    yaw 0..350 x pitch 0..180 in 10 degree steps (684 orientations);
+  * displaced lat-long sphere: 69,120 triangles (240 longitudes x 145 latitude bands),
+    base radius 60 mm perturbed by real spherical harmonics l = 1..8 with coefficients
+    N(0, 4 mm)/l, 0.5 mm voxels, 256 x 256 grid, yaw 0..359 x pitch 0..60 in 1 degree steps
+    (21,960 orientations).
+Meshes are expressed in voxel units (pre-scaled by 1/voxel), because the reference uses vertex
+coordinates directly as voxel indices (STomoVoxelSpaceInfo.cpp:97-105) with dVoxel pinned to 1.
+"""
+import math
+
+import numpy as np
+from scipy import special
+
+from .mesh import TomoMesh, mesh_area, vertex_normals
+
+
+# ---------------------------------------------------------------- options (Float32Info / Int32Info)
+def default_f32(area, theta_deg=60.0, bed_inner=0.0, bed_outer=2.0, bed_thick=0.3 + 0.27 + 2 * 0.2):
+    """Float32Info[12] in the order of tomo_dll_interface.py:115-118 with SlicingOptions defaults
+    (tomo_slicingOptions.py:41-58)."""
+    return np.array([1.0, theta_deg * np.pi / 180.0, area, 0.8, 0.15, 1.0, 0.2, 1.0, 0.001121,
+                     bed_inner, bed_outer, bed_thick], dtype=np.float32)
+
+
+BED_NONE, BED_BRIM, BED_RAFT, BED_SKIRT = 0, 1, 2, 3
+
+
+def default_i32(n_tri, n_vtx, n_ypr, grid=256, explicit_ss=True, shell=False, bed_type=BED_RAFT,
+                shell_um=500):
+    """Int32Info[11] in the order of tomo_dll_interface.py:120-124."""
+    return np.array([0, int(explicit_ss), int(shell), grid, n_tri, n_vtx, n_ypr, 0, 0, bed_type,
+                     shell_um], dtype=np.int32)
+
+
+# ---------------------------------------------------------------- orientation grids
+def ypr_grid(yaw_deg, pitch_deg):
+    """Rows pitch-major, yaw fastest, roll 0 (tomo_dll_interface.py:80-85 ordering).
+    Angles are float32(deg * pi / 180) computed in float64."""
+    yaw = np.asarray(yaw_deg, np.float64) * np.pi / 180.0
+    pit = np.asarray(pitch_deg, np.float64) * np.pi / 180.0
+    yy, pp = np.meshgrid(yaw, pit)
+    return np.column_stack([yy.ravel(), pp.ravel(), np.zeros(yy.size)]).astype(np.float32)
+
+
+def reference_ypr_grid(theta_step_deg):
+    """The drop-in grid of TomoDLLInterface.__init__ (tomo_dll_interface.py:80-85): yaw and pitch
+    both linspace(0, 2 pi, 360/step + 1, float32), roll 0."""
+    n = int(360 / theta_step_deg) + 1
+    r = np.linspace(0.0, 2 * np.pi, num=n, endpoint=True, dtype=np.float32)
+    xx, yy, zz = np.meshgrid(r, r, np.zeros(1))
+    return np.column_stack([xx.ravel(), yy.ravel(), zz.ravel()]).astype(np.float32)
+
+
+# ---------------------------------------------------------------- meshes
+def icosphere(subdiv, radius):
+    t = (1.0 + 5 ** 0.5) / 2.0
+    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t],
+                  [0, -1, -t], [0, 1, -t], [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]],
+                 np.float64)
+    f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9],
+                  [5, 11, 4], [11, 10, 2], [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2],
+                  [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5], [2, 4, 11], [6, 2, 10],
+                  [8, 6, 7], [9, 8, 1]], np.int64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    for _ in range(subdiv):
+        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+        e = np.sort(e, axis=1)
+        uniq, inv = np.unique(e, axis=0, return_inverse=True)
+        inv = inv.ravel()
+        mid = v[uniq[:, 0]] + v[uniq[:, 1]]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        base = len(v)
+        v = np.concatenate([v, mid])
+        nf = len(f)
+        m01 = base + inv[:nf]
+        m12 = base + inv[nf:2 * nf]
+        m20 = base + inv[2 * nf:]
+        f = np.concatenate([
+            np.stack([f[:, 0], m01, m20], 1), np.stack([f[:, 1], m12, m01], 1),
+            np.stack([f[:, 2], m20, m12], 1), np.stack([m01, m12, m20], 1)])
+    return v * radius, f
+
+
+def real_sph_harm(l, m, polar, azim):
+    """Real orthonormal spherical harmonic Y_lm."""
+    y = special.sph_harm_y(l, abs(m), polar, azim)
+    if m == 0:
+        return y.real
+    s = math.sqrt(2.0) * (-1) ** m
+    return s * (y.real if m > 0 else y.imag)
+
+
+def displaced_latlong_sphere(n_lon=240, n_bands=145, radius=60.0, lmax=8, sigma=4.0, seed=1):
+    """Closed lat-long sphere with 2*n_lon*(n_bands-1) triangles, radius perturbed by real SH."""
+    rng = np.random.default_rng(seed)
+    polar_rings = np.arange(1, n_bands) * np.pi / n_bands
+    azim = np.arange(n_lon) * 2 * np.pi / n_lon
+    P, A = np.meshgrid(polar_rings, azim, indexing="ij")
+    polar = np.concatenate([[0.0], P.ravel(), [np.pi]])
+    az = np.concatenate([[0.0], A.ravel(), [0.0]])
+    r = np.full(polar.shape, radius, np.float64)
+    for l in range(1, lmax + 1):
+        for m in range(-l, l + 1):
+            a = rng.normal(0.0, sigma) / l
+            r += a * real_sph_harm(l, m, polar, az)
+    v = np.stack([r * np.sin(polar) * np.cos(az), r * np.sin(polar) * np.sin(az), r * np.cos(polar)], 1)
+    nr = n_bands - 1
+    top, bot = 0, 1 + nr * n_lon
+
+    def ring(k, j):
+        return 1 + k * n_lon + (j % n_lon)
+
+    faces = []
+    for j in range(n_lon):
+        faces.append([top, ring(0, j), ring(0, j + 1)])
+    for k in range(nr - 1):
+        for j in range(n_lon):
+            a, b = ring(k, j), ring(k, j + 1)
+            c, d = ring(k + 1, j), ring(k + 1, j + 1)
+            faces.append([a, c, d])
+            faces.append([a, d, b])
+    for j in range(n_lon):
+        faces.append([bot, ring(nr - 1, j + 1), ring(nr - 1, j)])
+    return v, np.array(faces, np.int64)
+
+
+def fit_to_grid(v, grid, bed_outer=2.0, margin=2.0):
+    """Uniform scale (<= 1) about the centroid so the mesh footprint fits the grid for every
+    orientation: the placed mesh spans [Bo, Bo + extent] in x and y (STomoSh_Base.cpp:103-109),
+    and the extent is at most the mesh diameter about its centre."""
+    c = 0.5 * (v.min(0) + v.max(0))
+    rmax = np.linalg.norm(v - c, axis=1).max()
+    limit = grid - bed_outer - margin
+    s = min(1.0, limit / (2.0 * rmax))
+    return c + (v - c) * s, s
+
+
+class Workload:
+    """A mesh in voxel units plus its options and orientation grid."""
+
+    def __init__(self, name, vertices, faces, grid, ypr, voxel_mm, scale_note="", bed_type=BED_RAFT,
+                 explicit_ss=True, shell=False, shell_um=500):
+        self.name = name
+        self.vtx = np.ascontiguousarray(vertices, np.float32)
+        self.tri = np.ascontiguousarray(faces, np.int32)
+        self.vnrm = np.ascontiguousarray(vertex_normals(vertices, faces), np.float32)
+        self.grid = int(grid)
+        self.ypr = np.ascontiguousarray(ypr, np.float32)
+        self.voxel_mm = voxel_mm
+        self.scale_note = scale_note
+        self.area = float(mesh_area(vertices, faces))
+        self.bed_type = bed_type
+        self.explicit_ss = explicit_ss
+        self.shell = shell
+        self.shell_um = shell_um
+
+    @property
+    def f32(self):
+        return default_f32(self.area)
+
+    @property
+    def i32(self):
+        return default_i32(len(self.tri), len(self.vtx), len(self.ypr), grid=self.grid,
+                           explicit_ss=self.explicit_ss, shell=self.shell, bed_type=self.bed_type,
+                           shell_um=self.shell_um)
+
+    def mesh(self):
+        return TomoMesh(self.vtx.astype(np.float64), self.tri, self.vnrm, self.name)
+
+
+def icosphere_workload(bed_type=BED_RAFT):
+    """20,480-triangle icosphere, r = 50 mm, 1 mm voxels, 128^2 grid, 684 orientations."""
+    v, f = icosphere(5, 50.0)
+    ypr = ypr_grid(np.arange(0, 360, 10), np.arange(0, 181, 10))
+    return Workload("icosphere20480_r50_h1_g128_10deg", v, f, 128, ypr, 1.0, bed_type=bed_type)
+
+
+def displaced_sphere_workload(bed_type=BED_RAFT, seed=1, yaw_step=1, pitch_max=60):
+    """69,120-triangle displaced sphere, 0.5 mm voxels, 256^2 grid, yaw 0..359 x pitch 0..60."""
+    v, f = displaced_latlong_sphere(seed=seed)
+    v = v / 0.5
+    v, s = fit_to_grid(v, 256)
+    note = "" if s == 1.0 else f"scaled by {s:.4f} to fit the 256 grid"
+    ypr = ypr_grid(np.arange(0, 360, yaw_step), np.arange(0, pitch_max + 1, 1))
+    return Workload("dsphere69120_h0.5_g256_1deg", v, f, 256, ypr, 0.5, note, bed_type=bed_type)
