@@ -1,0 +1,32 @@
+// Host build of the product's per-slot pairing (tomo_slot.h) for CPU unit tests.
+#include "../tomo_shell2026_amd/csrc/tomo_slot.h"
+
+using namespace tomo;
+
+template <int N>
+static int run(const uint32_t *keys, int c, int cut_ssb, int cut_nvb, int explicit_ss, int bed_type,
+               int bed_r, int32_t *out) {
+  uint32_t k[N];
+  for (int i = 0; i < N; i++) k[i] = i < c ? keys[i] : 0u;
+  PairParams pp{cut_ssb, cut_nvb, explicit_ss, bed_type, bed_r};
+  SlotResult r;
+  pair_keys<N>(k, c, pp, r);
+  out[0] = r.vo; out[1] = r.vss; out[2] = r.va; out[3] = r.vb; out[4] = r.vnv; out[5] = r.vtc;
+  out[6] = r.npre; out[7] = r.n; out[8] = r.sabs; out[9] = r.last_z; out[10] = r.last_type;
+  out[11] = r.has10; out[12] = r.flags; out[13] = (int32_t)bed_info(r, pp);
+  for (int i = 0; i < N; i++) out[14 + i] = (int32_t)k[i];
+  return 0;
+}
+
+extern "C" {
+// Pair c keys with the 8-entry register network (c <= 8) or the generic network (c <= 63).
+int harness_pair8(const uint32_t *keys, int c, int cs, int cn, int ex, int bt, int br, int32_t *out) {
+  if (c < 0 || c > 8) return -1;
+  return run<8>(keys, c, cs, cn, ex, bt, br, out);
+}
+int harness_pair63(const uint32_t *keys, int c, int cs, int cn, int ex, int bt, int br, int32_t *out) {
+  if (c < 0 || c > 63) return -1;
+  return run<63>(keys, c, cs, cn, ex, bt, br, out);
+}
+uint32_t harness_make_key(int z, int nz) { return make_key(z, nz); }
+}

@@ -1,0 +1,143 @@
+"""GPU parity: libtomo_hip.so against the CPU oracle, bit-exact on integer counts and on the
+reference's float totals / masses wherever the float total is exact.
+
+Every case prints its name before touching the GPU so a failing run names it.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from tests import _cases as C
+from tomo_shell2026_amd import _lib, synth
+
+pytestmark = pytest.mark.gpu
+
+NTHREADS = min(16, os.cpu_count() or 1)
+
+
+def announce(name):
+    print(f"[gpu-case] {name}", flush=True)
+
+
+def run_gpu(case, launch=0):
+    with _lib.Sweep(case["f32"], case["i32"], case["grid"]) as s:
+        s.set_mesh(case["tri"], case["vtx"], case["vnrm"])
+        if launch:
+            s.set_launch(launch)
+        return s.sweep(case["ypr"])
+
+
+def run_ref(O, case):
+    return O.sweep(case["f32"], case["i32"], case["grid"], case["tri"], case["vtx"], case["vnrm"],
+                   case["ypr"], nthreads=NTHREADS)
+
+
+def workload_case(w, ypr=None):
+    return dict(f32=w.f32, i32=w.i32, grid=w.grid, tri=w.tri, vtx=w.vtx, vnrm=w.vnrm,
+                ypr=w.ypr if ypr is None else ypr)
+
+
+def assert_parity(O, case, label):
+    announce(label)
+    g = run_gpu(case)
+    r = run_ref(O, case)
+    bad = C.compare_records(g, r, label + ": ")
+    assert not bad, "\n".join(bad)
+    assert _lib.find_first_optimal(g["Mss"]) == O.find_first_optimal(r["Mss"])
+    return g, r
+
+
+@pytest.mark.parametrize("bed", [synth.BED_NONE, synth.BED_RAFT, synth.BED_BRIM, synth.BED_SKIRT])
+def test_icosphere_684(oracle_mod, bed):
+    w = synth.icosphere_workload(bed_type=bed)
+    g, r = assert_parity(oracle_mod, workload_case(w), f"icosphere20480 bed={bed}")
+    assert (g["flags"] & ~32 == 0).all()
+
+
+def test_icosphere_implicit_ss(oracle_mod):
+    w = synth.icosphere_workload(bed_type=synth.BED_RAFT)
+    w.explicit_ss = False
+    assert_parity(oracle_mod, workload_case(w), "icosphere implicit SS")
+
+
+def test_displaced_sphere_slice(oracle_mod):
+    w = synth.displaced_sphere_workload()
+    idx = np.arange(0, len(w.ypr), 61)
+    g, r = assert_parity(oracle_mod, workload_case(w, w.ypr[idx]), "dsphere69120 every 61st orientation")
+    # saturated slots occur in this workload: the 63-row cap order is exercised
+    assert ((r["flags"] & 8) != 0).any()
+
+
+def test_per_slot_dumps(oracle_mod):
+    w = synth.displaced_sphere_workload()
+    announce("per-slot dumps dsphere69120")
+    with _lib.Sweep(w.f32, w.i32, w.grid) as s:
+        s.set_mesh(w.tri, w.vtx, w.vnrm)
+        for i in (0, 1000, 7777, 21959):
+            rec, pre, fin, vo, vss = s.debug_slots(w.ypr[i])
+            rr, rpre, rfin, rvo, rvss = oracle_mod.one(w.f32, w.i32, w.grid, w.tri, w.vtx, w.vnrm, w.ypr[i])
+            np.testing.assert_array_equal(pre, rpre)
+            np.testing.assert_array_equal(vo, rvo)
+            np.testing.assert_array_equal(vss, rvss)
+            np.testing.assert_array_equal(fin, rfin)
+
+
+def test_stacked_planes_cap(oracle_mod):
+    v, f = C.stacked_planes()
+    case = C.make(v, f, 160, C.coarse_ypr(45, 90), bed=synth.BED_RAFT)
+    g, r = assert_parity(oracle_mod, case, "stacked planes (63-row cap)")
+    assert ((r["flags"] & 8) != 0).any()
+
+
+def test_tiny_triangles_fallback(oracle_mod):
+    v, f = C.tiny_triangles()
+    assert_parity(oracle_mod, C.make(v, f, 64, C.coarse_ypr(60)), "tiny triangles")
+
+
+def test_box_shell_mode(oracle_mod):
+    v, f = C.box((0, 0, 0), (20, 20, 20))
+    assert_parity(oracle_mod, C.make(v, f, 64, C.coarse_ypr(15), shell=True, shell_um=1500,
+                                     bed=synth.BED_RAFT), "box shell")
+
+
+def test_footprint_out_of_grid(oracle_mod):
+    v, f = C.box((0, 0, 0), (40, 30, 10))
+    g, r = assert_parity(oracle_mod, C.make(v, f, 32, C.coarse_ypr(30)), "box larger than grid")
+    assert ((r["flags"] & 4) != 0).all()
+
+
+def test_empty_mesh(oracle_mod):
+    announce("empty mesh")
+    case = C.make(np.zeros((0, 3)), np.zeros((0, 3), np.int64), 32, C.coarse_ypr(90))
+    g = run_gpu(case)
+    r = run_ref(oracle_mod, case)
+    assert not C.compare_records(g, r)
+    assert (g["Vo"] == 0).all() and (g["n_slots"] == 0).all()
+
+
+def test_launch_shapes_agree(oracle_mod):
+    w = synth.icosphere_workload()
+    case = workload_case(w, w.ypr[:50])
+    announce("launch shapes")
+    ref = run_gpu(case)
+    for groups in (1, 7, 64):
+        g = run_gpu(case, launch=groups)
+        assert not C.compare_records(g, ref), groups
+
+
+def test_full_sweep_properties():
+    """configs-size sweep: 21,960 orientations of the 69,120-triangle mesh."""
+    w = synth.displaced_sphere_workload()
+    announce("dsphere69120 full 1-degree sweep x2")
+    with _lib.Sweep(w.f32, w.i32, w.grid) as s:
+        s.set_mesh(w.tri, w.vtx, w.vnrm)
+        a = s.sweep(w.ypr)
+        b = s.sweep(w.ypr)
+    # deterministic: two runs bit-identical (reference acceptance: TODO_CUDA_issues_2026-07-04.md:101-104)
+    assert a.tobytes() == b.tobytes()
+    assert (a["Vss"] >= 0).all()
+    # object volume is orientation-independent up to rasterisation error (~1%)
+    vo = a["Vo"].astype(np.float64)
+    assert vo.std() / vo.mean() < 0.01
+    assert (a["flags"] & (4 | 64)).sum() == 0

@@ -1,0 +1,187 @@
+"""ctypes binding of libtomo_hip.so (declared in include/tomo_hip.h).
+
+The library is the only compute path: if it is missing or cannot be loaded this module raises,
+there is no CPU fallback.
+"""
+import ctypes as ct
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtomo_hip.so")
+
+TOMO_OK = 0
+F_FP32_INEXACT, F_INT16_WRAP, F_FOOTPRINT_OOB, F_CAP_DROP = 1, 2, 4, 8
+F_SENTINEL_DROP, F_RANK_TIE, F_UNRESOLVED = 16, 32, 64
+
+RECORD_DTYPE = np.dtype(
+    [("Va", "<i8"), ("Vb", "<i8"), ("Vo", "<i8"), ("Vss", "<i8"), ("Vnv", "<i8"),
+     ("Vtc", "<i8"), ("Vbed", "<i8"), ("n_entries", "<i8"), ("n_slots", "<i8"),
+     ("n_final", "<i8"), ("chk_pre", "<u8"), ("chk_final", "<u8"), ("flags", "<i4"),
+     ("n_bed", "<i4"), ("fVa", "<f4"), ("fVb", "<f4"), ("fVtc", "<f4"), ("fVnv", "<f4"),
+     ("fVss", "<f4"), ("fVo", "<f4"), ("fVbed", "<f4"), ("Mo", "<f4"), ("Mss", "<f4"),
+     ("Mtotal", "<f4"), ("mat4x4", "<f4", (16,))], align=True)
+assert RECORD_DTYPE.itemsize == 208
+
+# Every symbol include/tomo_hip.h declares.
+EXPORTS = (
+    "tomo_create", "tomo_set_mesh", "tomo_set_orientations", "tomo_run", "tomo_sync",
+    "tomo_get_records", "tomo_sweep", "tomo_debug_slots", "tomo_set_launch",
+    "tomo_last_kernel_ms", "tomo_get_info", "tomo_destroy", "tomo_last_error",
+    "tomo_find_first_optimal", "TomoSh_HIP", "getMss", "getMo", "getVtc", "getVolMassInfo",
+    "getMat4x4", "getnData2i", "getpData2i", "OnDestroy",
+)
+
+_lib = None
+
+
+class TomoError(RuntimeError):
+    pass
+
+
+def load():
+    """Load libtomo_hip.so (build it first with `python -m tomo_shell2026_amd.build`)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise TomoError(f"{LIB_PATH} is not built; run python -m tomo_shell2026_amd.build")
+    L = ct.CDLL(LIB_PATH)
+    P, I32, F = ct.c_void_p, ct.c_int32, ct.c_float
+    sig = {
+        "tomo_create": ((P, P, I32, P), ct.c_int),
+        "tomo_set_mesh": ((P, I32, P, I32, P, P), ct.c_int),
+        "tomo_set_orientations": ((P, I32, P), ct.c_int),
+        "tomo_run": ((P, I32, I32), ct.c_int),
+        "tomo_sync": ((P,), ct.c_int),
+        "tomo_get_records": ((P, I32, I32, P), ct.c_int),
+        "tomo_sweep": ((P, I32, P, P), ct.c_int),
+        "tomo_debug_slots": ((P, P, P, P, P, P, P), ct.c_int),
+        "tomo_set_launch": ((P, I32), ct.c_int),
+        "tomo_last_kernel_ms": ((P,), F),
+        "tomo_get_info": ((P, P, P, P), ct.c_int),
+        "tomo_destroy": ((P,), ct.c_int),
+        "tomo_last_error": ((), ct.c_char_p),
+        "tomo_find_first_optimal": ((I32, P), I32),
+        "TomoSh_HIP": ((P,) * 10, I32),
+        "getMss": ((), ct.POINTER(ct.c_float)),
+        "getMo": ((), ct.POINTER(ct.c_float)),
+        "getVtc": ((), ct.POINTER(ct.c_float)),
+        "getVolMassInfo": ((), ct.POINTER(ct.c_float)),
+        "getMat4x4": ((), ct.POINTER(ct.c_float)),
+        "getnData2i": ((ct.c_int16,), I32),
+        "getpData2i": ((ct.c_int16,), ct.POINTER(ct.c_int16)),
+        "OnDestroy": ((), None),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def last_error():
+    return load().tomo_last_error().decode(errors="replace")
+
+
+def check(rc, what):
+    if rc != TOMO_OK:
+        raise TomoError(f"{what} failed ({rc}): {last_error()}")
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data_as(ct.c_void_p)
+
+
+class Sweep:
+    """One sweep context (a tomo_handle): options, a mesh, an orientation set."""
+
+    def __init__(self, f32, i32, grid=0):
+        L = load()
+        self._f32 = np.ascontiguousarray(f32, np.float32)
+        self._i32 = np.ascontiguousarray(i32, np.int32)
+        if self._f32.size != 12 or self._i32.size != 11:
+            raise ValueError("Float32Info must have 12 entries and Int32Info 11")
+        h = ct.c_void_p()
+        check(L.tomo_create(ptr(self._f32), ptr(self._i32), int(grid), ct.byref(h)), "tomo_create")
+        self._h = h
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().tomo_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_mesh(self, tri, vtx, vnrm):
+        tri = np.ascontiguousarray(tri, np.int32).reshape(-1, 3)
+        vtx = np.ascontiguousarray(vtx, np.float32).reshape(-1, 3)
+        vnrm = np.ascontiguousarray(vnrm, np.float32).reshape(-1, 3)
+        if vtx.shape != vnrm.shape:
+            raise ValueError("vertex and normal arrays differ in shape")
+        check(load().tomo_set_mesh(self._h, tri.shape[0], ptr(tri), vtx.shape[0], ptr(vtx), ptr(vnrm)),
+              "tomo_set_mesh")
+
+    def set_orientations(self, ypr):
+        ypr = np.ascontiguousarray(ypr, np.float32).reshape(-1, 3)
+        check(load().tomo_set_orientations(self._h, ypr.shape[0], ptr(ypr)), "tomo_set_orientations")
+        self.n = ypr.shape[0]
+
+    def set_launch(self, n_groups):
+        check(load().tomo_set_launch(self._h, int(n_groups)), "tomo_set_launch")
+
+    def run(self, begin=0, end=None):
+        end = self.n if end is None else end
+        check(load().tomo_run(self._h, int(begin), int(end)), "tomo_run")
+
+    def sync(self):
+        check(load().tomo_sync(self._h), "tomo_sync")
+
+    def records(self, begin=0, end=None):
+        end = self.n if end is None else end
+        out = np.zeros(end - begin, RECORD_DTYPE)
+        check(load().tomo_get_records(self._h, int(begin), int(end), ptr(out)), "tomo_get_records")
+        return out
+
+    def sweep(self, ypr):
+        self.set_orientations(ypr)
+        self.run()
+        return self.records()
+
+    def kernel_ms(self):
+        return float(load().tomo_last_kernel_ms(self._h))
+
+    def info(self):
+        g = ct.c_int32()
+        n = ct.c_int32()
+        b = ct.c_int64()
+        check(load().tomo_get_info(self._h, ct.byref(g), ct.byref(n), ct.byref(b)), "tomo_get_info")
+        return {"grid": g.value, "n_groups": n.value, "scratch_bytes": b.value}
+
+    def debug_slots(self, ypr3):
+        y = np.ascontiguousarray(ypr3, np.float32).reshape(3)
+        info = self.info()
+        g2 = info["grid"] ** 2
+        rec = np.zeros(1, RECORD_DTYPE)
+        arrs = [np.zeros(g2, np.int32) for _ in range(4)]
+        check(load().tomo_debug_slots(self._h, ptr(y), ptr(rec), *[ptr(a) for a in arrs]),
+              "tomo_debug_slots")
+        return (rec[0],) + tuple(arrs)
+
+
+def find_first_optimal(mss):
+    m = np.ascontiguousarray(mss, np.float32)
+    return int(load().tomo_find_first_optimal(m.size, ptr(m)))

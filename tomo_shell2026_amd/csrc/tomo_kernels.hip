@@ -157,7 +157,14 @@ __device__ __forceinline__ void raster_tri(Emit &em, const SweepParams &p, int32
   float mny = (v0y <= v1y) ? v0y : v1y; mny = (mny <= v2y) ? mny : v2y;
   float mxx = (v0x >= v1x) ? v0x : v1x; mxx = (mxx >= v2x) ? mxx : v2x;
   float mxy = (v0y >= v1y) ? v0y : v1y; mxy = (mxy >= v2y) ? mxy : v2y;
-  const int32_t x0 = (int32_t)mnx, y0 = (int32_t)mny, x1 = (int32_t)mxx, y1 = (int32_t)mxy;
+  int32_t x0 = (int32_t)mnx, y0 = (int32_t)mny, x1 = (int32_t)mxx, y1 = (int32_t)mxy;
+  // Pixels far outside the grid can only yield dropped (out-of-grid) entries: bound the loop so
+  // a wild vertex cannot stall the work-group, and report the footprint overflow.
+  const int32_t lo = -64, hi = p.grid + 64;
+  if (x0 < lo || y0 < lo || x1 > hi || y1 > hi) {
+    em.flags |= F_FOOTPRINT_OOB;
+    x0 = max(x0, lo); y0 = max(y0, lo); x1 = min(x1, hi); y1 = min(y1, hi);
+  }
   const float e0x = v1x - v0x, e0y = v1y - v0y;
   const float e1x = v2x - v0x, e1y = v2y - v0y;
   const float d00 = e0x * e0x + e0y * e0y + 0.0f;
