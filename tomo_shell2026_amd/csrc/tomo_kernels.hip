@@ -564,6 +564,8 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(SweepParams p) {
       account(acc, r, s, p);
       t.bedinfo[s] = bed_info(r, p.pp);
     }
+    // Large slots reset their counters below; every wave must be done reading counters above.
+    __syncthreads();
     // ---- (3b) pair large slots, one wave each ----------------------------------------------
     for (uint32_t b = wave; b < nbig; b += kWaves) {
       const int32_t s = t.big[b];
@@ -660,7 +662,7 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(SweepParams p) {
       R.kx = kx; R.ky = ky; R.kz = kz;
       R.pad = 0;
     }
-    __syncthreads();
+    wg_sync_global();
   }
 }
 
