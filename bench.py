@@ -50,7 +50,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--groups", type=int, default=0, help="persistent work-groups (0 = auto)")
+    ap.add_argument("--batch", type=int, default=0, help="orientations in flight (0 = auto)")
     ap.add_argument("--cpu-stride", type=int, default=11)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -65,8 +65,8 @@ def main():
     s = _lib.Sweep(w.f32, w.i32, w.grid)
     s.set_mesh(w.tri, w.vtx, w.vnrm)
     s.set_orientations(w.ypr)
-    if args.groups:
-        s.set_launch(args.groups)
+    if args.batch:
+        s.set_launch(args.batch)
     rec = None
     for _ in range(max(args.warmup, 0)):
         s.run()
@@ -105,7 +105,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": w.name, "triangles": int(len(w.tri)), "vertices": int(len(w.vtx)),
                    "orientations": n, "grid": w.grid, "voxel_mm": w.voxel_mm, "bed": "raft",
-                   "support": "explicit", "note": w.scale_note, "work_groups": info["n_groups"]},
+                   "support": "explicit", "note": w.scale_note, "batch": info["batch"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "kernel_ms": round(k_ms, 3), "algorithmic_bytes_per_orientation": round(b_orient),

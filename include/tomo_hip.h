@@ -109,15 +109,22 @@ int tomo_sweep(tomo_handle *h, int32_t n, const float *ypr, tomo_record *out);
 int tomo_debug_slots(tomo_handle *h, const float ypr[3], tomo_record *rec, int32_t *cnt_pre,
                      int32_t *cnt_final, int32_t *slot_vo, int32_t *slot_vss);
 
-/* Persistent work-group count for tomo_run (0 = automatic).  Launch-shape knob for tuning. */
-int tomo_set_launch(tomo_handle *h, int32_t n_groups);
+/* Orientations in flight per batch for tomo_run (0 = automatic; each needs one slot-table
+ * region of scratch memory).  Launch-shape knob for tuning. */
+int tomo_set_launch(tomo_handle *h, int32_t batch);
 
 /* Duration in milliseconds of the last tomo_run's kernel, from HIP events on the handle's
  * stream (negative if unavailable). */
 float tomo_last_kernel_ms(tomo_handle *h);
 
-/* Grid edge and work-group count actually used. */
-int tomo_get_info(tomo_handle *h, int32_t *grid, int32_t *n_groups, int64_t *scratch_bytes);
+/* Enable (1) or disable (0) per-stage timing (HIP events between kernels) in later tomo_run
+ * calls, and read the last run's milliseconds summed over batches for: [0] rotated-AABB
+ * reduction, [1] rasterisation, [2] overflow compaction, [3] pairing, [4] bed; [5..7] = 0. */
+int tomo_set_phase_profile(tomo_handle *h, int32_t enable);
+int tomo_phase_profile(tomo_handle *h, double *ms8);
+
+/* Grid edge, batch size of the last run, and scratch bytes held. */
+int tomo_get_info(tomo_handle *h, int32_t *grid, int32_t *batch, int64_t *scratch_bytes);
 
 int tomo_destroy(tomo_handle *h);
 

@@ -28,7 +28,7 @@ assert RECORD_DTYPE.itemsize == 208
 EXPORTS = (
     "tomo_create", "tomo_set_mesh", "tomo_set_orientations", "tomo_run", "tomo_sync",
     "tomo_get_records", "tomo_sweep", "tomo_debug_slots", "tomo_set_launch",
-    "tomo_last_kernel_ms", "tomo_get_info", "tomo_destroy", "tomo_last_error",
+    "tomo_last_kernel_ms", "tomo_get_info", "tomo_set_phase_profile", "tomo_phase_profile", "tomo_destroy", "tomo_last_error",
     "tomo_find_first_optimal", "TomoSh_HIP", "getMss", "getMo", "getVtc", "getVolMassInfo",
     "getMat4x4", "getnData2i", "getpData2i", "OnDestroy",
 )
@@ -61,6 +61,8 @@ def load():
         "tomo_set_launch": ((P, I32), ct.c_int),
         "tomo_last_kernel_ms": ((P,), F),
         "tomo_get_info": ((P, P, P, P), ct.c_int),
+        "tomo_set_phase_profile": ((P, I32), ct.c_int),
+        "tomo_phase_profile": ((P, P), ct.c_int),
         "tomo_destroy": ((P,), ct.c_int),
         "tomo_last_error": ((), ct.c_char_p),
         "tomo_find_first_optimal": ((I32, P), I32),
@@ -169,7 +171,17 @@ class Sweep:
         n = ct.c_int32()
         b = ct.c_int64()
         check(load().tomo_get_info(self._h, ct.byref(g), ct.byref(n), ct.byref(b)), "tomo_get_info")
-        return {"grid": g.value, "n_groups": n.value, "scratch_bytes": b.value}
+        return {"grid": g.value, "batch": n.value, "scratch_bytes": b.value}
+
+    PHASES = ("aabb", "raster", "compact", "pair", "bed")
+
+    def set_phase_profile(self, enable=True):
+        check(load().tomo_set_phase_profile(self._h, int(bool(enable))), "tomo_set_phase_profile")
+
+    def phase_profile(self):
+        out = np.zeros(8, np.float64)
+        check(load().tomo_phase_profile(self._h, ptr(out)), "tomo_phase_profile")
+        return {k: float(out[i]) for i, k in enumerate(self.PHASES)}
 
     def debug_slots(self, ypr3):
         y = np.ascontiguousarray(ypr3, np.float32).reshape(3)

@@ -13,7 +13,7 @@
 #include "tomo_device.h"
 
 namespace tomo {
-hipError_t launch_sweep(const SweepParams &p, int n_groups, hipStream_t stream);
+hipError_t launch_stage(int stage, const SweepParams &p, hipStream_t stream);
 }
 
 using namespace tomo;
@@ -140,8 +140,7 @@ ScratchLayout make_layout(int32_t G) {
     return here;
   };
   L.count = take(4 * G2);
-  L.keys = take(4 * G2 * kInline);
-  L.ranks = take(4 * G2 * kInline);
+  L.ent = take(8 * G2 * kInline);
   L.bedinfo = take(4 * G2);
   L.big = take(4 * G2);
   L.off = take(4 * G2);
@@ -176,49 +175,59 @@ struct tomo_handle {
   float *d_rot = nullptr;
   DevRecord *d_rec = nullptr;
   int32_t rec_cap = 0;
-  // scratch
+  // scratch: one slot-table region per in-flight orientation
   ScratchLayout lay{};
   uint8_t *d_scratch = nullptr;
-  int32_t scratch_groups = 0;
-  int32_t groups_req = 0;
-  int32_t last_groups = 0;
+  OrientHdr *d_hdr = nullptr;
+  int32_t scratch_batch = 0;
+  int32_t batch_req = 0;
+  int32_t last_batch = 0;
   int32_t last_begin = 0, last_end = 0;
   std::vector<float> vbed_seq;
+  // per-stage timing (events around every stage of every batch)
+  bool phases = false;
+  std::vector<hipEvent_t> stage_ev;
+  std::vector<int> stage_of;
 };
 
 namespace {
 
-int ensure_scratch(tomo_handle *h, int32_t groups) {
-  if (groups <= h->scratch_groups) return TOMO_OK;
-  if (h->d_scratch) {
-    HIP_TRY(hipStreamSynchronize(h->stream));
-    HIP_TRY(hipFree(h->d_scratch));
-    h->d_scratch = nullptr;
-    h->scratch_groups = 0;
-  }
-  const size_t bytes = (size_t)h->lay.bytes * (size_t)groups;
+void free_stage_events(tomo_handle *h) {
+  for (hipEvent_t e : h->stage_ev) hipEventDestroy(e);
+  h->stage_ev.clear();
+  h->stage_of.clear();
+}
+
+int ensure_scratch(tomo_handle *h, int32_t batch) {
+  if (batch <= h->scratch_batch) return TOMO_OK;
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  if (h->d_scratch) { HIP_TRY(hipFree(h->d_scratch)); h->d_scratch = nullptr; }
+  if (h->d_hdr) { HIP_TRY(hipFree(h->d_hdr)); h->d_hdr = nullptr; }
+  h->scratch_batch = 0;
+  const size_t bytes = (size_t)h->lay.bytes * (size_t)batch;
   if (hipMalloc(&h->d_scratch, bytes) != hipSuccess) {
     h->d_scratch = nullptr;
     return fail(TOMO_E_OOM, "scratch allocation failed");
   }
-  // Slot counters must start at zero; every orientation leaves them at zero again.
+  if (hipMalloc(&h->d_hdr, sizeof(OrientHdr) * (size_t)batch) != hipSuccess) {
+    h->d_hdr = nullptr;
+    return fail(TOMO_E_OOM, "header allocation failed");
+  }
+  // Slot counters start at zero; the kernels restore that before a region is reused.
   HIP_TRY(hipMemsetAsync(h->d_scratch, 0, bytes, h->stream));
+  HIP_TRY(hipMemsetAsync(h->d_hdr, 0, sizeof(OrientHdr) * (size_t)batch, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
-  h->scratch_groups = groups;
+  h->scratch_batch = batch;
   return TOMO_OK;
 }
 
-int32_t auto_groups(tomo_handle *h, int32_t n) {
-  int cus = 256;
-  hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, h->device) == hipSuccess && prop.multiProcessorCount > 0)
-    cus = prop.multiProcessorCount;
-  int32_t g = h->groups_req > 0 ? h->groups_req : cus * 4;
-  // keep scratch within a quarter of free memory
+// Orientations in flight per batch: as requested, else 2048, within a quarter of free memory.
+int32_t auto_batch(tomo_handle *h, int32_t n) {
+  int32_t g = h->batch_req > 0 ? h->batch_req : 2048;
   size_t freeb = 0, total = 0;
-  if (hipMemGetInfo(&freeb, &total) == hipSuccess && h->lay.bytes > 0) {
+  if (g > h->scratch_batch && hipMemGetInfo(&freeb, &total) == hipSuccess && h->lay.bytes > 0) {
     const size_t cap = (freeb / 4) / (size_t)h->lay.bytes;
-    if (cap >= 1 && (size_t)g > cap && h->scratch_groups < g) g = (int32_t)cap;
+    if (cap >= 1 && (size_t)g > cap) g = (int32_t)cap;
   }
   if (g > n) g = n;
   if (g < 1) g = 1;
@@ -229,7 +238,7 @@ void finish_record(const tomo_handle *h, const DevRecord &d, const float *R, tom
   std::memset(&r, 0, sizeof(r));
   r.Va = d.Va; r.Vb = d.Vb; r.Vo = d.Vo; r.Vss = d.Vss; r.Vnv = d.Vnv; r.Vtc = d.Vtc;
   r.Vbed = d.Vbed; r.n_entries = d.n_entries; r.n_slots = d.n_slots; r.n_final = d.n_final;
-  r.chk_pre = d.chk_pre; r.chk_final = d.chk_final; r.n_bed = d.n_bed;
+  r.chk_pre = d.chk_pre; r.chk_final = d.chk_final; r.n_bed = (int32_t)d.Vbed;
   int32_t flags = d.flags;
   const int64_t lim = (int64_t)1 << 24;
   if (d.aVo > lim || d.aVss > lim || d.aVa > lim || d.aVb > lim || d.aVnv > lim || d.aVtc > lim)
@@ -239,7 +248,7 @@ void finish_record(const tomo_handle *h, const DevRecord &d, const float *R, tom
   // equal the integers (Pairing :386-392 and Calculate :1233-1240 accumulate ints into floats).
   r.fVa = (float)d.Va; r.fVb = (float)d.Vb; r.fVtc = (float)d.Vtc; r.fVnv = (float)d.Vnv;
   r.fVss = (float)d.Vss; r.fVo = (float)d.Vo;
-  const size_t nb = (size_t)(d.n_bed < 0 ? 0 : d.n_bed);
+  const size_t nb = (size_t)(d.Vbed < 0 ? 0 : d.Vbed);
   r.fVbed = nb < h->vbed_seq.size() ? h->vbed_seq[nb] : (float)nb * h->opt.bed_thick;
   float vm[22] = {};
   vm[0] = r.fVa; vm[1] = r.fVb; vm[2] = r.fVtc; vm[3] = r.fVnv; vm[4] = r.fVss; vm[8] = r.fVo;
@@ -289,8 +298,50 @@ SweepParams base_params(tomo_handle *h) {
   p.pp.bed_type = h->opt.bed_type;
   p.pp.bed_r = h->opt.bed_r;
   p.bed_dist = h->d_bed_dist;
+  p.bed_on = (h->opt.bed_type != 0 && h->opt.bed_r >= 1) ? 1 : 0;
   p.lay = h->lay;
+  p.hdr = h->d_hdr;
+  p.scratch = h->d_scratch;
+  const int32_t g2 = h->opt.grid * h->opt.grid;
+  p.tri_groups = (h->n_tri + kTriPerGroup - 1) / kTriPerGroup;
+  p.pair_tiles = (g2 + kPairTile - 1) / kPairTile;
+  p.bed_bands = (h->opt.grid + kBandRows - 1) / kBandRows;
   return p;
+}
+
+// Enqueue the five stages for orientations [begin, end) in batches of `batch`.
+int enqueue(tomo_handle *h, int32_t begin, int32_t end, int32_t batch, int32_t *dbg4, size_t g2) {
+  for (int32_t b0 = begin; b0 < end; b0 += batch) {
+    SweepParams p = base_params(h);
+    p.n_batch = (end - b0) < batch ? (end - b0) : batch;
+    p.rot = h->d_rot + (size_t)b0 * 9;
+    p.rec = h->d_rec + b0;
+    if (dbg4) {
+      p.dbg_pre = dbg4;
+      p.dbg_final = dbg4 + g2;
+      p.dbg_vo = dbg4 + 2 * g2;
+      p.dbg_vss = dbg4 + 3 * g2;
+    }
+    if (p.tri_groups == 0) p.tri_groups = 1;
+    for (int stage = 0; stage < 5; stage++) {
+      if (h->phases) {
+        hipEvent_t e;
+        HIP_TRY(hipEventCreate(&e));
+        HIP_TRY(hipEventRecord(e, h->stream));
+        h->stage_ev.push_back(e);
+        h->stage_of.push_back(stage);
+      }
+      HIP_TRY(launch_stage(stage, p, h->stream));
+    }
+  }
+  if (h->phases) {
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e));
+    HIP_TRY(hipEventRecord(e, h->stream));
+    h->stage_ev.push_back(e);
+    h->stage_of.push_back(-1);
+  }
+  return TOMO_OK;
 }
 
 }  // namespace
@@ -414,9 +465,9 @@ int tomo_set_orientations(tomo_handle *h, int32_t n, const float *ypr) {
   return TOMO_OK;
 }
 
-int tomo_set_launch(tomo_handle *h, int32_t n_groups) {
-  if (!h || n_groups < 0) return fail(TOMO_E_INVALID_ARG, "bad launch arguments");
-  h->groups_req = n_groups;
+int tomo_set_launch(tomo_handle *h, int32_t batch) {
+  if (!h || batch < 0) return fail(TOMO_E_INVALID_ARG, "bad launch arguments");
+  h->batch_req = batch;
   return TOMO_OK;
 }
 
@@ -429,19 +480,17 @@ int tomo_run(tomo_handle *h, int32_t begin, int32_t end) {
   h->timed = false;
   const int32_t n = end - begin;
   if (n == 0) return TOMO_OK;
-  const int32_t groups = auto_groups(h, n);
-  int rc = ensure_scratch(h, groups);
+  const int32_t batch = auto_batch(h, n);
+  int rc = ensure_scratch(h, batch);
   if (rc) return rc;
-  SweepParams p = base_params(h);
-  p.rot = h->d_rot + (size_t)begin * 9;
-  p.n_orient = n;
-  p.rec = h->d_rec + begin;
-  p.scratch = h->d_scratch;
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  free_stage_events(h);
   HIP_TRY(hipEventRecord(h->ev0, h->stream));
-  HIP_TRY(launch_sweep(p, groups, h->stream));
+  rc = enqueue(h, begin, end, batch, nullptr, 0);
+  if (rc) return rc;
   HIP_TRY(hipEventRecord(h->ev1, h->stream));
   h->timed = true;
-  h->last_groups = groups;
+  h->last_batch = batch;
   return TOMO_OK;
 }
 
@@ -479,11 +528,11 @@ float tomo_last_kernel_ms(tomo_handle *h) {
   return ms;
 }
 
-int tomo_get_info(tomo_handle *h, int32_t *grid, int32_t *n_groups, int64_t *scratch_bytes) {
+int tomo_get_info(tomo_handle *h, int32_t *grid, int32_t *batch, int64_t *scratch_bytes) {
   if (!h) return fail(TOMO_E_INVALID_ARG, "null handle");
   if (grid) *grid = h->opt.grid;
-  if (n_groups) *n_groups = h->last_groups;
-  if (scratch_bytes) *scratch_bytes = (int64_t)h->lay.bytes * h->scratch_groups;
+  if (batch) *batch = h->last_batch;
+  if (scratch_bytes) *scratch_bytes = (int64_t)h->lay.bytes * h->scratch_batch;
   return TOMO_OK;
 }
 
@@ -498,18 +547,19 @@ int tomo_debug_slots(tomo_handle *h, const float ypr[3], tomo_record *rec, int32
   const size_t g2 = (size_t)h->opt.grid * h->opt.grid;
   int32_t *d = nullptr;
   HIP_TRY(hipMalloc(&d, sizeof(int32_t) * 4 * g2));
-  HIP_TRY(hipMemsetAsync(d, 0, sizeof(int32_t) * 4 * g2, h->stream));
-  SweepParams p = base_params(h);
-  p.rot = h->d_rot;
-  p.n_orient = 1;
-  p.rec = h->d_rec;
-  p.scratch = h->d_scratch;
-  p.dbg_pre = d;
-  p.dbg_final = d + g2;
-  p.dbg_vo = d + 2 * g2;
-  p.dbg_vss = d + 3 * g2;
-  hipError_t e = launch_sweep(p, 1, h->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  hipError_t e = hipMemsetAsync(d, 0, sizeof(int32_t) * 4 * g2, h->stream);
+  const bool was = h->phases;
+  h->phases = false;
+  if (e == hipSuccess) {
+    rc = enqueue(h, 0, 1, 1, d, g2);
+    if (rc) {
+      h->phases = was;
+      hipFree(d);
+      return rc;
+    }
+    e = hipStreamSynchronize(h->stream);
+  }
+  h->phases = was;
   int32_t *outs[4] = {cnt_pre, cnt_final, slot_vo, slot_vss};
   for (int k = 0; k < 4 && e == hipSuccess; k++)
     if (outs[k]) e = hipMemcpy(outs[k], d + k * g2, sizeof(int32_t) * g2, hipMemcpyDeviceToHost);
@@ -520,9 +570,31 @@ int tomo_debug_slots(tomo_handle *h, const float ypr[3], tomo_record *rec, int32
   return tomo_get_records(h, 0, 1, rec);
 }
 
+int tomo_set_phase_profile(tomo_handle *h, int32_t enable) {
+  if (!h) return fail(TOMO_E_INVALID_ARG, "null handle");
+  h->phases = enable != 0;
+  return TOMO_OK;
+}
+
+int tomo_phase_profile(tomo_handle *h, double *ms8) {
+  if (!h || !ms8) return fail(TOMO_E_INVALID_ARG, "null argument");
+  if (h->stage_ev.size() < 2) return fail(TOMO_E_STATE, "no stage timing recorded");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  for (int k = 0; k < 8; k++) ms8[k] = 0.0;
+  for (size_t i = 0; i + 1 < h->stage_ev.size(); i++) {
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, h->stage_ev[i], h->stage_ev[i + 1]));
+    const int st = h->stage_of[i];
+    if (st >= 0 && st < 8) ms8[st] += ms;
+  }
+  return TOMO_OK;
+}
+
 int tomo_destroy(tomo_handle *h) {
   if (!h) return TOMO_OK;
   if (h->stream) hipStreamSynchronize(h->stream);
+  free_stage_events(h);
+  if (h->d_hdr) hipFree(h->d_hdr);
   if (h->d_scratch) hipFree(h->d_scratch);
   if (h->d_rec) hipFree(h->d_rec);
   if (h->d_rot) hipFree(h->d_rot);

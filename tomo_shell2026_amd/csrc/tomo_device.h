@@ -1,4 +1,4 @@
-// tomo_device.h -- launch parameters and device record shared by the HIP kernels and the host
+// tomo_device.h -- launch parameters and device records shared by the HIP kernels and the host
 // side of libtomo_hip.so.
 #pragma once
 #include <stdint.h>
@@ -13,9 +13,14 @@ constexpr int kSmall = 8;              // slots with <= 8 entries are paired in 
 constexpr int kInline = 16;            // entries stored inline per slot (layer-major)
 constexpr int kBigCap = 512;           // entries a single slot may hold for exact resolution
 constexpr int kMaxBedR = 16;           // int(BedOuterBound) supported by the bed kernel
+constexpr int kTriPerThread = 4;       // rasteriser: triangles per thread per work-group
+constexpr int kTriPerGroup = kThreads * kTriPerThread;
+constexpr int kPairTile = 2048;        // pairing: touched-box slots per work-group
+constexpr int kBandRows = 4;           // bed: grid rows per work-group
 
-// Raw device record, one per orientation.  Totals are exact 64-bit integers; the host turns
-// them into the reference's float totals and masses.
+// Device record, one per orientation.  Totals are exact 64-bit integers accumulated with
+// atomics by the pairing and bed kernels; the host turns them into the reference's float
+// totals and masses.
 struct DevRecord {
   int64_t Va, Vb, Vo, Vss, Vnv, Vtc, Vbed;
   int64_t n_entries, n_slots, n_final;
@@ -26,11 +31,21 @@ struct DevRecord {
   int32_t pad;
 };
 
-// Per-work-group scratch, carved from one device allocation.  G2 = grid * grid.
+// Per in-flight orientation bookkeeping (reset by the AABB kernel).
+struct OrientHdr {
+  int32_t bbox[4];       // touched slots: x0, x1, y0, y1
+  uint32_t big_top;      // slots with more than kSmall entries
+  uint32_t pool_top;     // entries beyond kInline
+  int32_t flags;
+  float kx, ky, kz;
+  int32_t pad[6];
+};
+
+// Per in-flight orientation scratch, carved from one device allocation.  G2 = grid * grid.
 struct ScratchLayout {
-  int64_t count, keys, ranks, bedinfo, big, off, cursor, pool_slot, pool_key, pool_rank, cmp_key,
-      cmp_rank;  // byte offsets inside one work-group's region
-  int64_t bytes; // bytes per work-group region
+  int64_t count, ent, bedinfo, big, off, cursor, pool_slot, pool_key, pool_rank, cmp_key,
+      cmp_rank;  // byte offsets inside one orientation's region
+  int64_t bytes; // bytes per region
   int32_t pool;  // overflow pool capacity (entries)
 };
 
@@ -40,20 +55,25 @@ struct SweepParams {
   const float *vx, *vy, *vz, *nx, *ny, *nz;
   int32_t n_tri, n_vtx;
   float cx, cy, cz;        // raw AABB centre (STomoSh_Base.cpp:75-82)
-  // orientation set
-  const float *rot;        // [n_orient][9] row-major rotation matrices
-  int32_t n_orient;
+  // orientations of this batch: rot/rec already offset to the batch's first orientation
+  const float *rot;        // [n_batch][9] row-major rotation matrices
+  int32_t n_batch;
+  int32_t tri_groups;      // rasteriser work-groups per orientation
+  int32_t pair_tiles;      // pairing work-groups per orientation
+  int32_t bed_bands;       // bed work-groups per orientation
   // options
   int32_t grid;
   float bed_out, bed_in, h2, shell_off;
   int32_t do_shell;
+  int32_t bed_on;          // bed type != none and int(BedOuterBound) >= 1
   PairParams pp;
   const float *bed_dist;   // [2r][2r] float distances sqrtf(dx^2 + dy^2), dx,dy in [-r, r)
   // scratch and outputs
   uint8_t *scratch;
+  OrientHdr *hdr;          // [n_batch]
   ScratchLayout lay;
-  DevRecord *rec;
-  // optional per-slot dumps for one orientation (grid*grid each, zero-filled by the host)
+  DevRecord *rec;          // [n_batch]
+  // optional per-slot dumps for a one-orientation batch (grid*grid each, zero-filled by host)
   int32_t *dbg_pre, *dbg_final, *dbg_vo, *dbg_vss;
 };
 

@@ -1,15 +1,16 @@
 // tomo_kernels.hip -- the per-orientation support-tomography sweep for gfx950.
 //
-// One persistent work-group (4 waves of 64) owns one orientation at a time and walks the
-// orientation list with a grid stride.  Per orientation it
-//   (1) rotates every vertex and min-reduces the rotated AABB          (STomoSh_Base.cpp:45-109)
-//   (2) rasterises every triangle against pixel centres and appends     (STomoSh_INT3.cpp:1138-1209,
-//       (z, nZ) keys to its private slot table in HBM                    STomoVoxelSpaceInfo.cpp:175-230)
-//   (3) pairs each touched slot: <= 8 entries in registers, larger slots (STomoSh_INT3.cpp:1041-1062)
-//       one wave per slot with an LDS bitonic sort and ballot scans
-//   (4) lays the raft/brim/skirt bed with the reference's raster-order   (STomoSh_INT3.cpp:1331-1443)
-//       semantics and
-//   (5) reduces the totals with wave shuffles into one record            (STomoSh_INT3.cpp:1211-1255)
+// Orientations are processed in batches; each orientation of a batch owns a slot table in HBM.
+// Five kernels per batch, each with its own register budget and many short work-groups:
+//   aabb_kernel     rotate every vertex, min-reduce the rotated AABB     (STomoSh_Base.cpp:45-109)
+//   raster_kernel   rasterise triangles against pixel centres and append  (STomoSh_INT3.cpp:1138-1209,
+//                   (z, nZ) keys to the orientation's slot table            STomoVoxelSpaceInfo.cpp:175-230)
+//   compact_kernel  group entries beyond the inline layers per slot
+//   pair_kernel     pair each touched slot: <= 8 entries in registers,    (STomoSh_INT3.cpp:1041-1062,
+//                   9..63 with a wave-wide shuffle bitonic sort, more with  1211-1255)
+//                   an LDS sort and rank selection; totals by atomics
+//   bed_kernel      raft/brim/skirt bed with the reference's raster-order (STomoSh_INT3.cpp:1331-1443)
+//                   semantics, from LDS-staged neighbour flags
 // There is no MFMA: this is rasterisation, sorting and integer reduction.  All float arithmetic
 // follows the reference's operation order without contraction (built with -ffp-contract=off).
 #include <hip/hip_runtime.h>
@@ -23,23 +24,13 @@ namespace {
 constexpr float kMargin = 0.001f;           // g_fMARGIN (Tomo_types.h:40)
 constexpr float kNormFactor = 1000.0f;      // g_fNORMALFACTOR (Tomo_types.h:39)
 
-struct alignas(16) Shared {
-  uint64_t sortbuf[kWaves][kBigCap];
-  int64_t acc[kWaves][20];
-  float red[kWaves][3];
-  int32_t bbox[4];
-  uint32_t big_top, pool_top;
-  int32_t flags;
-  uint32_t scan[kWaves];
-};
+constexpr int kBedLdsBytes = 24 * 1024;  // bed band + halo flags ((4 + 2r) x (width + 2r))
 
-// Release own global writes, meet, then drop possibly stale L1 lines before reading what other
-// waves of the work-group wrote (slot tables live in global memory).
-__device__ __forceinline__ void wg_sync_global() {
-  __threadfence();
-  __syncthreads();
-  __threadfence();
-}
+// Work-group barrier that also orders the work-group's global-memory slot tables.  A slot
+// table is private to its work-group and all its waves share one CU (and its L1), so the
+// workgroup-scope release/acquire of __syncthreads suffices; an agent-scope fence would write
+// back and invalidate the whole L2 of the XCD.
+__device__ __forceinline__ void wg_sync_global() { __syncthreads(); }
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -66,7 +57,8 @@ __device__ __forceinline__ T wave_sum(T v) {
 __device__ __forceinline__ int msb64(uint64_t m) { return 63 - __clzll((long long)m); }
 
 struct Tables {
-  uint32_t *count, *keys, *ranks, *bedinfo, *off, *cursor, *pool_slot, *pool_key, *pool_rank,
+  uint2 *ent;  // [kInline][G2] {key, rank}, layer-major
+  uint32_t *count, *bedinfo, *off, *cursor, *pool_slot, *pool_key, *pool_rank,
       *cmp_key, *cmp_rank;
   int32_t *big;
   int32_t pool;
@@ -106,7 +98,7 @@ __device__ __forceinline__ int32_t to_i16(float f, int32_t &flags) {
 struct Emit {
   int32_t G, G2;
   Tables t;
-  Shared *sh;
+  uint32_t *pool_top, *big_top;  // this orientation's counters (global)
   int32_t bx0, bx1, by0, by1;
   int32_t flags;
 
@@ -126,10 +118,9 @@ struct Emit {
     by0 = min(by0, sy); by1 = max(by1, sy);
     uint32_t k = atomicAdd(&t.count[s], 1u);
     if (k < (uint32_t)kInline) {
-      t.keys[(int64_t)k * G2 + s] = key;
-      t.ranks[(int64_t)k * G2 + s] = rank;
+      t.ent[(int64_t)k * G2 + s] = make_uint2(key, rank);
     } else {
-      uint32_t j = atomicAdd(&sh->pool_top, 1u);
+      uint32_t j = atomicAdd(pool_top, 1u);
       if (j < (uint32_t)t.pool) {
         t.pool_slot[j] = (uint32_t)s;
         t.pool_key[j] = key;
@@ -137,7 +128,7 @@ struct Emit {
       }
     }
     if (k == (uint32_t)kSmall) {
-      uint32_t b = atomicAdd(&sh->big_top, 1u);
+      uint32_t b = atomicAdd(big_top, 1u);
       t.big[b] = s;
     }
     return s;
@@ -241,11 +232,12 @@ __device__ __forceinline__ int pow2_at_least(int n) {
   return P;
 }
 
-// Pair D (<= 63) distinct keys held one per lane, sorted descending (lanes >= D invalid).
-// Same semantics as pair_keys in tomo_slot.h, with the sequential state machines turned into
-// "last event before this lane" ballot scans.  The result is valid in every lane.
-__device__ void wave_pair(uint32_t key, int D, int lane, const PairParams &pp, SlotResult &r) {
-  const bool valid = lane < D;
+// Pair the distinct keys of the lanes set in V (at most 63), in descending key order from low
+// to high lane.  Same semantics as pair_keys in tomo_slot.h, with the sequential state machines
+// turned into "last event before this lane" ballot scans.  The result is valid in every lane.
+__device__ void wave_pair(uint32_t key, uint64_t V, int lane, const PairParams &pp, SlotResult &r) {
+  const bool valid = ((V >> lane) & 1ull) != 0;
+  const int D = __popcll(V);
   const int32_t z = key_z(key), nz = key_nz(key);
   const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
   const uint32_t k10 = make_key(1, 0);
@@ -256,7 +248,7 @@ __device__ void wave_pair(uint32_t key, int D, int lane, const PairParams &pp, S
   r.vo = r.vss = r.va = r.vb = r.vnv = r.vtc = 0;
   if (D <= 1) {
     r.n = D;
-    r.last_z = D ? __shfl(z, 0) : 0;
+    r.last_z = D ? __shfl(z, msb64(V)) : 0;
     r.last_type = 0;
     return;
   }
@@ -281,7 +273,8 @@ __device__ void wave_pair(uint32_t key, int D, int lane, const PairParams &pp, S
   const int32_t sSSB = wave_sum(ssb ? z : 0), sSSA = wave_sum(ssa ? z : 0);
   const int32_t sNVB = wave_sum(nvb ? z : 0), sNVA = wave_sum(nva ? z : 0);
   const int32_t sTC = wave_sum(tc ? z : 0);
-  int32_t last_z = __shfl(z, D - 1), last_type = __shfl(ty, D - 1);
+  const int lastl = msb64(V);
+  int32_t last_z = __shfl(z, lastl), last_type = __shfl(ty, lastl);
   int n = D;
   if (exOpen) {
     if (n < kMaxPix) { n++; last_z = 0; last_type = T_SSA; } else r.flags |= F_SENTINEL_DROP;
@@ -307,9 +300,8 @@ __device__ void wave_pair(uint32_t key, int D, int lane, const PairParams &pp, S
 // One slot with more than kSmall entries, handled by a whole wave.  Restores SetBit's exact
 // kept set: the first 63 distinct keys in insertion order, insertion order being (triangle,
 // main-before-sub) = ascending rank (STomoVoxelSpaceInfo.cpp:200-220).
-__device__ void big_slot(int32_t s, uint32_t c, const Tables &t, int32_t G2, Shared &sh, int wave,
-                         int lane, const PairParams &pp, SlotResult &r, int32_t &flags) {
-  uint64_t *buf = sh.sortbuf[wave];
+__device__ void big_slot(int32_t s, uint32_t c, const Tables &t, int32_t G2, uint64_t *buf,
+                         int lane, const PairParams &pp, SlotResult &r) {
   int32_t fl = 0;
   if (c > (uint32_t)kBigCap) {
     fl |= F_UNRESOLVED;
@@ -322,8 +314,9 @@ __device__ void big_slot(int32_t s, uint32_t c, const Tables &t, int32_t G2, Sha
     if (i < (int)c) {
       uint32_t key, rank;
       if (i < kInline) {
-        key = t.keys[(int64_t)i * G2 + s];
-        rank = t.ranks[(int64_t)i * G2 + s];
+        const uint2 e = t.ent[(int64_t)i * G2 + s];
+        key = e.x;
+        rank = e.y;
       } else {
         uint32_t j = off + (uint32_t)(i - kInline);
         if (j < (uint32_t)t.pool) {
@@ -385,22 +378,49 @@ __device__ void big_slot(int32_t s, uint32_t c, const Tables &t, int32_t G2, Sha
     mykey = (lane < D) ? (uint32_t)buf[lane] : 0u;
   }
   wave_sync();
-  wave_pair(mykey, D, lane, pp, r);
+  wave_pair(mykey, (D >= 64) ? ~0ull : ((1ull << D) - 1ull), lane, pp, r);
   r.flags |= fl;
-  flags |= fl;
 }
 
-}  // namespace
+// A slot with kSmall < c < 64 entries: one key per lane, register bitonic sort over the wave
+// with lane shuffles, duplicates masked out.  No ranks are needed: at most 63 distinct keys.
+__device__ void mid_slot(int32_t s, uint32_t c, const Tables &t, int32_t G2, int lane,
+                         const PairParams &pp, SlotResult &r) {
+  uint32_t key = 0u;
+  if (lane < (int)c) {
+    const uint32_t off = (c > (uint32_t)kInline) ? t.off[s] : 0u;
+    key = (lane < kInline) ? t.ent[(int64_t)lane * G2 + s].x : t.cmp_key[off + (uint32_t)(lane - kInline)];
+  }
+#pragma unroll
+  for (int size = 2; size <= 64; size <<= 1) {
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      const uint32_t other = __shfl_xor(key, stride);
+      const bool keep_max = ((lane & size) == 0) == ((lane & stride) == 0);
+      key = keep_max ? max(key, other) : min(key, other);
+    }
+  }
+  const uint32_t prev = __shfl_up(key, 1);
+  const bool first = lane < (int)c && (lane == 0 || key != prev);
+  wave_pair(key, __ballot(first), lane, pp, r);
+}
 
-__global__ __launch_bounds__(kThreads) void sweep_kernel(SweepParams p) {
-  __shared__ Shared sh;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int32_t G = p.grid, G2 = G * G;
-  uint8_t *base = p.scratch + (int64_t)blockIdx.x * p.lay.bytes;
+// Work-groups are dispatched to the 8 XCDs round-robin by block index.  Map block i to
+// orientation b = (i mod 8) + 8 * ((i / 8) / per) and part (i / 8) mod per, so every part of an
+// orientation runs on the same XCD (its slot table stays in that XCD's L2) and each XCD walks
+// its orientations one after another.  The grid is padded to a multiple of 8 orientations.
+__device__ __forceinline__ bool xcd_map(int32_t per, int32_t nb, int32_t &b, int32_t &part) {
+  const int32_t i = (int32_t)blockIdx.x, x = i & 7, j = i >> 3;
+  b = x + 8 * (j / per);
+  part = j % per;
+  return b < nb;
+}
+
+__device__ __forceinline__ Tables tables_for(const SweepParams &p, int32_t b) {
+  uint8_t *base = p.scratch + (int64_t)b * p.lay.bytes;
   Tables t;
   t.count = (uint32_t *)(base + p.lay.count);
-  t.keys = (uint32_t *)(base + p.lay.keys);
-  t.ranks = (uint32_t *)(base + p.lay.ranks);
+  t.ent = (uint2 *)(base + p.lay.ent);
   t.bedinfo = (uint32_t *)(base + p.lay.bedinfo);
   t.big = (int32_t *)(base + p.lay.big);
   t.off = (uint32_t *)(base + p.lay.off);
@@ -411,263 +431,418 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(SweepParams p) {
   t.cmp_key = (uint32_t *)(base + p.lay.cmp_key);
   t.cmp_rank = (uint32_t *)(base + p.lay.cmp_rank);
   t.pool = p.lay.pool;
-  // (double)v <= 1. + g_fMARGIN, as a float bound
-  const float vlim = __uint_as_float(0x3f8020c4u);
+  return t;
+}
 
-  for (int32_t o = blockIdx.x; o < p.n_orient; o += gridDim.x) {
-    const float *Rg = p.rot + (int64_t)o * 9;
-    const float r0 = Rg[0], r1 = Rg[1], r2 = Rg[2], r3 = Rg[3], r4 = Rg[4], r5 = Rg[5],
-                r6 = Rg[6], r7 = Rg[7], r8 = Rg[8];
-    // ---- (1) rotated AABB minimum (STomoSh_Base.cpp:77-109) --------------------------------
-    float mx = 1e5f, my = 1e5f, mz = 1e5f;
-    for (int32_t v = tid; v < p.n_vtx; v += kThreads) {
-      const float t0 = p.vx[v] - p.cx, t1 = p.vy[v] - p.cy, t2 = p.vz[v] - p.cz;
-      mx = fminf(mx, dot_row(r0, r1, r2, t0, t1, t2));
-      my = fminf(my, dot_row(r3, r4, r5, t0, t1, t2));
-      mz = fminf(mz, dot_row(r6, r7, r8, t0, t1, t2));
-    }
+// Sum N per-thread values over the work-group and add them to N global 64-bit counters.
+template <int N>
+__device__ __forceinline__ void group_add(int64_t (&v)[N], int64_t (*red)[N], int lane, int wave,
+                                          int tid, unsigned long long *const (&dst)[N]) {
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      mx = fminf(mx, __shfl_xor(mx, off));
-      my = fminf(my, __shfl_xor(my, off));
-      mz = fminf(mz, __shfl_xor(mz, off));
-    }
-    if (lane == 0) {
-      sh.red[wave][0] = mx;
-      sh.red[wave][1] = my;
-      sh.red[wave][2] = mz;
-    }
-    if (tid == 0) {
-      sh.bbox[0] = G; sh.bbox[1] = -1; sh.bbox[2] = G; sh.bbox[3] = -1;
-      sh.big_top = 0; sh.pool_top = 0; sh.flags = 0;
-    }
-    __syncthreads();
+  for (int q = 0; q < N; q++) v[q] = wave_sum(v[q]);
+  if (lane == 0) {
 #pragma unroll
-    for (int w = 0; w < kWaves; w++) {
-      mx = fminf(mx, sh.red[w][0]);
-      my = fminf(my, sh.red[w][1]);
-      mz = fminf(mz, sh.red[w][2]);
-    }
-    const float kx = mx - p.bed_out, ky = my - p.bed_out, kz = mz;
-
-    // ---- (2) rasterise (Pixelize, STomoSh_INT3.cpp:285-364) --------------------------------
-    Emit em;
-    em.G = G; em.G2 = G2; em.t = t; em.sh = &sh;
-    em.bx0 = G; em.bx1 = -1; em.by0 = G; em.by1 = -1; em.flags = 0;
-    for (int32_t tr = tid; tr < p.n_tri; tr += kThreads) {
-      const int32_t i0 = p.tri[3 * tr + 0], i1 = p.tri[3 * tr + 1], i2 = p.tri[3 * tr + 2];
-      float a[3][3], nzv[3];
-      const int32_t ids[3] = {i0, i1, i2};
+    for (int q = 0; q < N; q++) red[wave][q] = v[q];
+  }
+  __syncthreads();
+  if (tid < N) {
+    int64_t s = 0;
 #pragma unroll
-      for (int q = 0; q < 3; q++) {
-        const int32_t i = ids[q];
-        const float t0 = p.vx[i] - p.cx, t1 = p.vy[i] - p.cy, t2 = p.vz[i] - p.cz;
-        a[q][0] = dot_row(r0, r1, r2, t0, t1, t2) - kx;
-        a[q][1] = dot_row(r3, r4, r5, t0, t1, t2) - ky;
-        a[q][2] = dot_row(r6, r7, r8, t0, t1, t2) - kz;
-        nzv[q] = dot_row(r6, r7, r8, p.nx[i], p.ny[i], p.nz[i]);
-      }
-      raster_tri(em, p, tr, a[0][0], a[0][1], a[0][2], a[1][0], a[1][1], a[1][2], a[2][0],
-                 a[2][1], a[2][2], nzv[0], nzv[1], nzv[2], vlim);
-    }
-    // bounding box of touched slots and flags
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      em.bx0 = min(em.bx0, __shfl_xor(em.bx0, off));
-      em.bx1 = max(em.bx1, __shfl_xor(em.bx1, off));
-      em.by0 = min(em.by0, __shfl_xor(em.by0, off));
-      em.by1 = max(em.by1, __shfl_xor(em.by1, off));
-      em.flags |= __shfl_xor(em.flags, off);
-    }
-    if (lane == 0) {
-      atomicMin(&sh.bbox[0], em.bx0);
-      atomicMax(&sh.bbox[1], em.bx1);
-      atomicMin(&sh.bbox[2], em.by0);
-      atomicMax(&sh.bbox[3], em.by1);
-      atomicOr(&sh.flags, em.flags);
-    }
-    wg_sync_global();
-    const int32_t bx0 = sh.bbox[0], bx1 = sh.bbox[1], by0 = sh.bbox[2], by1 = sh.bbox[3];
-    const uint32_t nbig = sh.big_top;
-    const uint32_t npool = sh.pool_top;
-    Acc acc = {};
-    if (tid == 0 && npool > (uint32_t)t.pool) acc.flags |= F_UNRESOLVED;
-
-    // ---- overflow compaction: entries beyond kInline, grouped per slot ---------------------
-    if (npool > 0) {
-      uint32_t running = 0;
-      for (uint32_t b0 = 0; b0 < nbig; b0 += kThreads) {
-        const uint32_t b = b0 + tid;
-        int32_t s = -1;
-        uint32_t ex = 0;
-        if (b < nbig) {
-          s = t.big[b];
-          const uint32_t c = t.count[s];
-          ex = c > (uint32_t)kInline ? c - (uint32_t)kInline : 0u;
-        }
-        // work-group exclusive scan of ex
-        uint32_t incl = ex;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-          uint32_t u = __shfl_up(incl, off);
-          if (lane >= off) incl += u;
-        }
-        if (lane == 63) sh.scan[wave] = incl;
-        __syncthreads();
-        uint32_t wbase = 0, total = 0;
-#pragma unroll
-        for (int w = 0; w < kWaves; w++) {
-          if (w < wave) wbase += sh.scan[w];
-          total += sh.scan[w];
-        }
-        if (s >= 0 && ex > 0) {
-          t.off[s] = running + wbase + incl - ex;
-          t.cursor[s] = 0u;
-        }
-        running += total;
-        __syncthreads();
-      }
-      wg_sync_global();
-      const uint32_t np = npool < (uint32_t)t.pool ? npool : (uint32_t)t.pool;
-      for (uint32_t j = tid; j < np; j += kThreads) {
-        const uint32_t s = t.pool_slot[j];
-        const uint32_t q = atomicAdd(&t.cursor[s], 1u);
-        const uint32_t d = t.off[s] + q;
-        if (d < (uint32_t)t.pool) {
-          t.cmp_key[d] = t.pool_key[j];
-          t.cmp_rank[d] = t.pool_rank[j];
-        }
-      }
-      wg_sync_global();
-    }
-
-    // ---- (3a) pair small slots in registers ------------------------------------------------
-    const bool any = bx1 >= bx0;
-    const int32_t wy = by1 - by0 + 1;
-    const int32_t nbb = any ? (bx1 - bx0 + 1) * wy : 0;
-    for (int32_t i = tid; i < nbb; i += kThreads) {
-      const int32_t sx = bx0 + i / wy, sy = by0 + i % wy;
-      const int32_t s = sx * G + sy;
-      const uint32_t c = t.count[s];
-      if (c == 0u) {
-        t.bedinfo[s] = BI_POSS;
-        continue;
-      }
-      if (c > (uint32_t)kSmall) continue;  // handled by a wave below
-      t.count[s] = 0u;
-      acc.nslot++;
-      uint32_t k[kSmall];
-#pragma unroll
-      for (int q = 0; q < kSmall; q++) k[q] = (q < (int)c) ? t.keys[(int64_t)q * G2 + s] : 0u;
-      SlotResult r;
-      pair_keys<kSmall>(k, (int)c, p.pp, r);
-      account(acc, r, s, p);
-      t.bedinfo[s] = bed_info(r, p.pp);
-    }
-    // Large slots reset their counters below; every wave must be done reading counters above.
-    __syncthreads();
-    // ---- (3b) pair large slots, one wave each ----------------------------------------------
-    for (uint32_t b = wave; b < nbig; b += kWaves) {
-      const int32_t s = t.big[b];
-      const uint32_t c = t.count[s];
-      SlotResult r;
-      int32_t fl = 0;
-      big_slot(s, c, t, G2, sh, wave, lane, p.pp, r, fl);
-      if (lane == 0) {
-        t.count[s] = 0u;
-        acc.nslot++;
-        account(acc, r, s, p);
-        t.bedinfo[s] = bed_info(r, p.pp);
-      }
-    }
-    wg_sync_global();
-
-    // ---- (4) bed (GenerateBed, STomoSh_INT3.cpp:1375-1443) and final per-slot counts -------
-    const int32_t br = p.pp.bed_r;
-    const bool bed_on = any && p.pp.bed_type != 0 && br >= 1;
-    const int32_t m = bed_on ? br + 1 : 0;
-    const int32_t rx0 = any ? max(bx0 - m, 0) : 0, rx1 = any ? min(bx1 + m + 1, G) : 0;
-    const int32_t ry0 = any ? max(by0 - m, 0) : 0, ry1 = any ? min(by1 + m + 1, G) : 0;
-    const int32_t rwy = ry1 - ry0;
-    const int32_t nreg = (rx1 - rx0) * (rwy > 0 ? rwy : 0);
-    for (int32_t i = tid; i < nreg; i += kThreads) {
-      const int32_t X = rx0 + i / rwy, Y = ry0 + i % rwy;
-      const int32_t P = X * G + Y;
-      const bool inb = X >= bx0 && X <= bx1 && Y >= by0 && Y <= by1;
-      const uint32_t bi = inb ? t.bedinfo[P] : BI_POSS;
-      const int32_t n = (int32_t)(bi & 127u);
-      int32_t add = 0;
-      if (bed_on && (bi & BI_POSS)) {
-        float md = 1e5f;
-        const int32_t xa = max(X - br, 0), xb = min(X + br, G);
-        const int32_t ya = max(Y - br, 0), yb = min(Y + br, G);
-        for (int32_t sx = xa; sx < xb; sx++) {
-          if (sx < bx0 || sx > bx1) continue;
-          for (int32_t sy = ya; sy < yb; sy++) {
-            if (sy < by0 || sy > by1) continue;
-            const uint32_t bs = t.bedinfo[sx * G + sy];
-            if (!(bs & BI_SRC)) continue;
-            const bool before = (sx < X) || (sx == X && sy < Y);
-            if ((bs & BI_CONV) && before) continue;  // already covered by its own bed row
-            md = fminf(md, p.bed_dist[(sx - X + br) * (2 * br) + (sy - Y + br)]);
-          }
-        }
-        const bool bed = !(md > p.bed_out) &&
-                         (p.pp.bed_type == 1 || p.pp.bed_type == 2 || (p.pp.bed_type == 3 && md > p.bed_in));
-        if (bed) {
-          if (bi & BI_HAS10) {
-            acc.nbed++;
-          } else if (n < kMaxPix) {
-            acc.nbed++;
-            add = 1;
-          } else {
-            acc.flags |= F_CAP_DROP;
-          }
-        }
-      }
-      const int32_t nf = n + add;
-      acc.nfinal += nf;
-      acc.chkfinal += (uint64_t)(P + 1) * (uint64_t)nf;
-      if (p.dbg_final) p.dbg_final[P] = nf;
-    }
-
-    // ---- (5) reduce and write the record ---------------------------------------------------
-    int64_t v[19] = {acc.Va, acc.Vb, acc.Vo, acc.Vss, acc.Vnv, acc.Vtc, acc.nbed, acc.nent,
-                     acc.nslot, acc.nfinal, (int64_t)acc.chkpre, (int64_t)acc.chkfinal, acc.aVo,
-                     acc.aVss, acc.aVa, acc.aVb, acc.aVnv, acc.aVtc, (int64_t)acc.flags};
-#pragma unroll
-    for (int q = 0; q < 18; q++) v[q] = wave_sum(v[q]);
-    int32_t f = (int32_t)v[18];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) f |= __shfl_xor(f, off);
-    if (lane == 0) {
-#pragma unroll
-      for (int q = 0; q < 18; q++) sh.acc[wave][q] = v[q];
-      sh.acc[wave][18] = f;
-    }
-    __syncthreads();
-    if (tid == 0) {
-      int64_t s[19] = {};
-      for (int w = 0; w < kWaves; w++) {
-        for (int q = 0; q < 18; q++) s[q] += sh.acc[w][q];
-        s[18] |= sh.acc[w][18];
-      }
-      DevRecord &R = p.rec[o];
-      R.Va = s[0]; R.Vb = s[1]; R.Vo = s[2]; R.Vss = s[3]; R.Vnv = s[4]; R.Vtc = s[5];
-      R.Vbed = s[6]; R.n_entries = s[7]; R.n_slots = s[8]; R.n_final = s[9];
-      R.chk_pre = (uint64_t)s[10]; R.chk_final = (uint64_t)s[11];
-      R.aVo = s[12]; R.aVss = s[13]; R.aVa = s[14]; R.aVb = s[15]; R.aVnv = s[16]; R.aVtc = s[17];
-      R.flags = (int32_t)s[18] | sh.flags;
-      R.n_bed = (int32_t)s[6];
-      R.kx = kx; R.ky = ky; R.kz = kz;
-      R.pad = 0;
-    }
-    wg_sync_global();
+    for (int w = 0; w < kWaves; w++) s += red[w][tid];
+    if (s != 0) atomicAdd(dst[tid], (unsigned long long)s);
   }
 }
 
-hipError_t launch_sweep(const SweepParams &p, int n_groups, hipStream_t stream) {
-  hipLaunchKernelGGL(sweep_kernel, dim3(n_groups), dim3(kThreads), 0, stream, p);
+__device__ __forceinline__ int32_t group_or(int32_t f, int32_t *red, int lane, int wave, int tid) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) f |= __shfl_xor(f, off);
+  if (lane == 0) red[wave] = f;
+  __syncthreads();
+  int32_t r = 0;
+#pragma unroll
+  for (int w = 0; w < kWaves; w++) r |= red[w];
+  return r;
+}
+
+}  // namespace
+
+// ---- aabb_kernel: one work-group per orientation ------------------------------------------
+// Rotated AABB minimum (STomoSh_Base.cpp:77-109); resets the orientation's bookkeeping, and the
+// counters of the previous orientation's large slots that pair_kernel left set.
+__global__ __launch_bounds__(kThreads) void aabb_kernel(SweepParams p) {
+  __shared__ float red[kWaves][3];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int32_t b = blockIdx.x;
+  const float *Rg = p.rot + (int64_t)b * 9;
+  const float r0 = Rg[0], r1 = Rg[1], r2 = Rg[2], r3 = Rg[3], r4 = Rg[4], r5 = Rg[5], r6 = Rg[6],
+              r7 = Rg[7], r8 = Rg[8];
+  OrientHdr &h = p.hdr[b];
+  {
+    const Tables t = tables_for(p, b);
+    const uint32_t nbig_prev = h.big_top;
+    for (uint32_t j = tid; j < nbig_prev; j += kThreads) t.count[t.big[j]] = 0u;
+  }
+  float mx = 1e5f, my = 1e5f, mz = 1e5f;
+  for (int32_t v = tid; v < p.n_vtx; v += kThreads) {
+    const float t0 = p.vx[v] - p.cx, t1 = p.vy[v] - p.cy, t2 = p.vz[v] - p.cz;
+    mx = fminf(mx, dot_row(r0, r1, r2, t0, t1, t2));
+    my = fminf(my, dot_row(r3, r4, r5, t0, t1, t2));
+    mz = fminf(mz, dot_row(r6, r7, r8, t0, t1, t2));
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    mx = fminf(mx, __shfl_xor(mx, off));
+    my = fminf(my, __shfl_xor(my, off));
+    mz = fminf(mz, __shfl_xor(mz, off));
+  }
+  if (lane == 0) {
+    red[wave][0] = mx;
+    red[wave][1] = my;
+    red[wave][2] = mz;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    for (int w = 0; w < kWaves; w++) {
+      mx = fminf(mx, red[w][0]);
+      my = fminf(my, red[w][1]);
+      mz = fminf(mz, red[w][2]);
+    }
+    const float kx = mx - p.bed_out, ky = my - p.bed_out, kz = mz;
+    h.bbox[0] = p.grid; h.bbox[1] = -1; h.bbox[2] = p.grid; h.bbox[3] = -1;
+    h.big_top = 0u; h.pool_top = 0u; h.flags = 0;
+    h.kx = kx; h.ky = ky; h.kz = kz;
+    DevRecord &R = p.rec[b];
+    R = DevRecord{};
+    R.kx = kx; R.ky = ky; R.kz = kz;
+  }
+}
+
+// ---- raster_kernel: kTriPerGroup triangles of one orientation per work-group ----------------
+// Pixelize (STomoSh_INT3.cpp:285-364).  Work-groups are ordered triangle-chunk-major so that
+// concurrently running groups share the same triangles in cache.
+__global__ __launch_bounds__(kThreads) void raster_kernel(SweepParams p) {
+  __shared__ int32_t red[kWaves][5];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int32_t b, chunk;
+  if (!xcd_map(p.tri_groups, p.n_batch, b, chunk)) return;
+  const int32_t G = p.grid;
+  const float *Rg = p.rot + (int64_t)b * 9;
+  const float r0 = Rg[0], r1 = Rg[1], r2 = Rg[2], r3 = Rg[3], r4 = Rg[4], r5 = Rg[5], r6 = Rg[6],
+              r7 = Rg[7], r8 = Rg[8];
+  OrientHdr &h = p.hdr[b];
+  const float kx = h.kx, ky = h.ky, kz = h.kz;
+  // (double)v <= 1. + g_fMARGIN, as a float bound
+  const float vlim = __uint_as_float(0x3f8020c4u);
+  Emit em;
+  em.G = G; em.G2 = G * G; em.t = tables_for(p, b);
+  em.pool_top = &h.pool_top; em.big_top = &h.big_top;
+  em.bx0 = G; em.bx1 = -1; em.by0 = G; em.by1 = -1; em.flags = 0;
+  const int32_t first = chunk * kTriPerGroup;
+#pragma unroll 1
+  for (int k = 0; k < kTriPerThread; k++) {
+    const int32_t tr = first + k * kThreads + tid;
+    if (tr >= p.n_tri) break;
+    const int32_t ids[3] = {p.tri[3 * tr + 0], p.tri[3 * tr + 1], p.tri[3 * tr + 2]};
+    float a[3][3], nzv[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      const int32_t i = ids[q];
+      const float t0 = p.vx[i] - p.cx, t1 = p.vy[i] - p.cy, t2 = p.vz[i] - p.cz;
+      a[q][0] = dot_row(r0, r1, r2, t0, t1, t2) - kx;
+      a[q][1] = dot_row(r3, r4, r5, t0, t1, t2) - ky;
+      a[q][2] = dot_row(r6, r7, r8, t0, t1, t2) - kz;
+      nzv[q] = dot_row(r6, r7, r8, p.nx[i], p.ny[i], p.nz[i]);
+    }
+    raster_tri(em, p, tr, a[0][0], a[0][1], a[0][2], a[1][0], a[1][1], a[1][2], a[2][0], a[2][1],
+               a[2][2], nzv[0], nzv[1], nzv[2], vlim);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    em.bx0 = min(em.bx0, __shfl_xor(em.bx0, off));
+    em.bx1 = max(em.bx1, __shfl_xor(em.bx1, off));
+    em.by0 = min(em.by0, __shfl_xor(em.by0, off));
+    em.by1 = max(em.by1, __shfl_xor(em.by1, off));
+    em.flags |= __shfl_xor(em.flags, off);
+  }
+  if (lane == 0) {
+    red[wave][0] = em.bx0; red[wave][1] = em.bx1; red[wave][2] = em.by0; red[wave][3] = em.by1;
+    red[wave][4] = em.flags;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int32_t v0 = G, v1 = -1, v2 = G, v3 = -1, f = 0;
+    for (int w = 0; w < kWaves; w++) {
+      v0 = min(v0, red[w][0]); v1 = max(v1, red[w][1]);
+      v2 = min(v2, red[w][2]); v3 = max(v3, red[w][3]);
+      f |= red[w][4];
+    }
+    if (v1 >= v0) {
+      atomicMin(&h.bbox[0], v0); atomicMax(&h.bbox[1], v1);
+      atomicMin(&h.bbox[2], v2); atomicMax(&h.bbox[3], v3);
+    }
+    if (f) atomicOr(&h.flags, f);
+  }
+}
+
+// ---- compact_kernel: one work-group per orientation ----------------------------------------
+// Entries beyond the inline layers were appended to a pool in arrival order; group them per
+// slot (exclusive scan over the large-slot list, then scatter).
+__global__ __launch_bounds__(kThreads) void compact_kernel(SweepParams p) {
+  __shared__ uint32_t scan[kWaves];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int32_t b = blockIdx.x;
+  const OrientHdr &h = p.hdr[b];
+  const uint32_t npool = h.pool_top, nbig = h.big_top;
+  if (npool == 0u) return;
+  const Tables t = tables_for(p, b);
+  uint32_t running = 0;
+  for (uint32_t b0 = 0; b0 < nbig; b0 += kThreads) {
+    const uint32_t j = b0 + tid;
+    int32_t s = -1;
+    uint32_t ex = 0;
+    if (j < nbig) {
+      s = t.big[j];
+      const uint32_t c = t.count[s];
+      ex = c > (uint32_t)kInline ? c - (uint32_t)kInline : 0u;
+    }
+    uint32_t incl = ex;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t u = __shfl_up(incl, off);
+      if (lane >= off) incl += u;
+    }
+    if (lane == 63) scan[wave] = incl;
+    __syncthreads();
+    uint32_t wbase = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; w++) {
+      if (w < wave) wbase += scan[w];
+      total += scan[w];
+    }
+    if (s >= 0 && ex > 0) {
+      t.off[s] = running + wbase + incl - ex;
+      t.cursor[s] = 0u;
+    }
+    running += total;
+    __syncthreads();
+  }
+  const uint32_t np = npool < (uint32_t)t.pool ? npool : (uint32_t)t.pool;
+  for (uint32_t j = tid; j < np; j += kThreads) {
+    const uint32_t s = t.pool_slot[j];
+    const uint32_t q = atomicAdd(&t.cursor[s], 1u);
+    const uint32_t d = t.off[s] + q;
+    if (d < (uint32_t)t.pool) {
+      t.cmp_key[d] = t.pool_key[j];
+      t.cmp_rank[d] = t.pool_rank[j];
+    }
+  }
+}
+
+// ---- pair_kernel: kPairTile slots of the touched box per work-group ------------------------
+// Pairing (STomoSh_INT3.cpp:366-395) and the per-slot sums of Calculate (:1211-1255).
+__global__ __launch_bounds__(kThreads) void pair_kernel(SweepParams p) {
+  __shared__ uint64_t sortbuf[kWaves][kBigCap];
+  __shared__ int64_t red[kWaves][17];
+  __shared__ int32_t fred[kWaves];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int32_t b, tile;
+  if (!xcd_map(p.pair_tiles, p.n_batch, b, tile)) return;
+  const int32_t G = p.grid, G2 = G * G;
+  const OrientHdr &h = p.hdr[b];
+  const int32_t bx0 = h.bbox[0], bx1 = h.bbox[1], by0 = h.bbox[2], by1 = h.bbox[3];
+  const uint32_t nbig = h.big_top, npool = h.pool_top;
+  const bool any = bx1 >= bx0;
+  const int32_t wy = by1 - by0 + 1;
+  const int32_t nbb = any ? (bx1 - bx0 + 1) * wy : 0;
+  const int32_t i0 = tile * kPairTile;
+  const int32_t nwaves = p.pair_tiles * kWaves;
+  if (tile == 0 && tid == 0) {
+    // rasteriser flags, and pool overflow
+    const int32_t f0 = h.flags | (npool > (uint32_t)p.lay.pool ? F_UNRESOLVED : 0);
+    if (f0) atomicOr(&p.rec[b].flags, f0);
+  }
+  if (i0 >= nbb && (uint32_t)(tile * kWaves) >= nbig) return;  // nothing for this group
+  const Tables t = tables_for(p, b);
+  const bool final_here = !p.bed_on;  // without a bed the final row count is known here
+  Acc acc = {};
+  const int32_t i1 = min(i0 + kPairTile, nbb);
+  for (int32_t i = i0 + tid; i < i1; i += kThreads) {
+    const int32_t sx = bx0 + i / wy, sy = by0 + i % wy;
+    const int32_t s = sx * G + sy;
+    const uint32_t c = t.count[s];
+    if (c == 0u) {
+      t.bedinfo[s] = BI_POSS;
+      continue;
+    }
+    if (c > (uint32_t)kSmall) continue;  // a wave pairs it below
+    t.count[s] = 0u;
+    acc.nslot++;
+    uint32_t k[kSmall];
+#pragma unroll
+    for (int q = 0; q < kSmall; q++) k[q] = (q < (int)c) ? t.ent[(int64_t)q * G2 + s].x : 0u;
+    SlotResult r;
+    pair_keys<kSmall>(k, (int)c, p.pp, r);
+    account(acc, r, s, p);
+    t.bedinfo[s] = bed_info(r, p.pp);
+    if (final_here) {
+      acc.nfinal += r.n;
+      acc.chkfinal += (uint64_t)(s + 1) * (uint64_t)r.n;
+      if (p.dbg_final) p.dbg_final[s] = r.n;
+    }
+  }
+  // Large slots: one wave each.  Their counters stay set until the next aabb_kernel on this
+  // scratch region (resetting them here would race with the small-slot scan of other groups).
+  for (uint32_t j = (uint32_t)(tile * kWaves + wave); j < nbig; j += (uint32_t)nwaves) {
+    const int32_t s = t.big[j];
+    const uint32_t c = t.count[s];
+    SlotResult r;
+    if (c < 64u && (c <= (uint32_t)kInline || npool <= (uint32_t)t.pool))
+      mid_slot(s, c, t, G2, lane, p.pp, r);
+    else
+      big_slot(s, c, t, G2, sortbuf[wave], lane, p.pp, r);
+    if (lane == 0) {
+      acc.nslot++;
+      account(acc, r, s, p);
+      t.bedinfo[s] = bed_info(r, p.pp);
+      if (final_here) {
+        acc.nfinal += r.n;
+        acc.chkfinal += (uint64_t)(s + 1) * (uint64_t)r.n;
+        if (p.dbg_final) p.dbg_final[s] = r.n;
+      }
+    }
+  }
+  DevRecord &R = p.rec[b];
+  int64_t v[17] = {acc.Va, acc.Vb, acc.Vo, acc.Vss, acc.Vnv, acc.Vtc, acc.nent, acc.nslot,
+                   (int64_t)acc.chkpre, acc.aVo, acc.aVss, acc.aVa, acc.aVb, acc.aVnv, acc.aVtc,
+                   acc.nfinal, (int64_t)acc.chkfinal};
+  unsigned long long *const dst[17] = {
+      (unsigned long long *)&R.Va, (unsigned long long *)&R.Vb, (unsigned long long *)&R.Vo,
+      (unsigned long long *)&R.Vss, (unsigned long long *)&R.Vnv, (unsigned long long *)&R.Vtc,
+      (unsigned long long *)&R.n_entries, (unsigned long long *)&R.n_slots,
+      (unsigned long long *)&R.chk_pre, (unsigned long long *)&R.aVo,
+      (unsigned long long *)&R.aVss, (unsigned long long *)&R.aVa, (unsigned long long *)&R.aVb,
+      (unsigned long long *)&R.aVnv, (unsigned long long *)&R.aVtc,
+      (unsigned long long *)&R.n_final, (unsigned long long *)&R.chk_final};
+  group_add<17>(v, red, lane, wave, tid, dst);
+  const int32_t f = group_or(acc.flags, fred, lane, wave, tid);
+  if (tid == 0 && f) atomicOr(&R.flags, f);
+}
+
+// ---- bed_kernel: kBandRows grid rows of the bed scan region per work-group -----------------
+// GenerateBed / IsBedCandidate (STomoSh_INT3.cpp:1331-1443) with the raster-order effect: a raft
+// source slot that takes its own bed row stops being a source for slots scanned after it.
+__global__ __launch_bounds__(kThreads) void bed_kernel(SweepParams p) {
+  __shared__ uint8_t nb[kBedLdsBytes];
+  __shared__ int64_t red[kWaves][3];
+  __shared__ int32_t fred[kWaves];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int32_t b, band;
+  if (!xcd_map(p.bed_bands, p.n_batch, b, band)) return;
+  const int32_t G = p.grid;
+  const OrientHdr &h = p.hdr[b];
+  const int32_t bx0 = h.bbox[0], bx1 = h.bbox[1], by0 = h.bbox[2], by1 = h.bbox[3];
+  if (bx1 < bx0) return;
+  const int32_t br = p.pp.bed_r;
+  const int32_t m = br + 1;
+  const int32_t rx0 = max(bx0 - m, 0), rx1 = min(bx1 + m + 1, G);
+  const int32_t ry0 = max(by0 - m, 0), ry1 = min(by1 + m + 1, G);
+  const int32_t X0 = rx0 + band * kBandRows;
+  if (X0 >= rx1) return;
+  const int32_t X1 = min(X0 + kBandRows, rx1);
+  const Tables t = tables_for(p, b);
+  // neighbour flags of rows [X0 - r, X1 + r) x columns [ry0 - r, ry1 + r), clipped to the grid
+  const int32_t hx0 = max(X0 - br, 0), hx1 = min(X1 + br, G);
+  const int32_t hy0 = max(ry0 - br, 0), hy1 = min(ry1 + br, G);
+  const int32_t hw = hy1 - hy0;
+  const bool lds = (int64_t)(hx1 - hx0) * hw <= kBedLdsBytes;
+  if (lds) {
+    const int32_t nh = (hx1 - hx0) * hw;
+    for (int32_t i = tid; i < nh; i += kThreads) {
+      const int32_t x = hx0 + i / hw, y = hy0 + i % hw;
+      uint8_t f = 0;
+      if (x >= bx0 && x <= bx1 && y >= by0 && y <= by1) {
+        const uint32_t bi = t.bedinfo[x * G + y];
+        f = (uint8_t)(((bi & BI_SRC) ? 1u : 0u) | ((bi & BI_CONV) ? 2u : 0u));
+      }
+      nb[i] = f;
+    }
+    __syncthreads();
+  }
+  const int32_t rwy = ry1 - ry0;
+  const int32_t n = (X1 - X0) * rwy;
+  int64_t nbed = 0, nfinal = 0;
+  uint64_t chk = 0;
+  int32_t flags = 0;
+  for (int32_t i = tid; i < n; i += kThreads) {
+    const int32_t X = X0 + i / rwy, Y = ry0 + i % rwy;
+    const int32_t P = X * G + Y;
+    const bool inb = X >= bx0 && X <= bx1 && Y >= by0 && Y <= by1;
+    const uint32_t bi = inb ? t.bedinfo[P] : BI_POSS;
+    const int32_t nrow = (int32_t)(bi & 127u);
+    int32_t add = 0;
+    if (bi & BI_POSS) {
+      float md = 1e5f;
+      const int32_t xa = max(X - br, 0), xb = min(X + br, G);
+      const int32_t ya = max(Y - br, 0), yb = min(Y + br, G);
+      for (int32_t sx = xa; sx < xb; sx++) {
+        for (int32_t sy = ya; sy < yb; sy++) {
+          uint32_t f;
+          if (lds) {
+            f = nb[(sx - hx0) * hw + (sy - hy0)];
+          } else {
+            f = 0u;
+            if (sx >= bx0 && sx <= bx1 && sy >= by0 && sy <= by1) {
+              const uint32_t bs = t.bedinfo[sx * G + sy];
+              f = ((bs & BI_SRC) ? 1u : 0u) | ((bs & BI_CONV) ? 2u : 0u);
+            }
+          }
+          if (!(f & 1u)) continue;
+          const bool before = (sx < X) || (sx == X && sy < Y);
+          if ((f & 2u) && before) continue;  // already covered by its own bed row
+          md = fminf(md, p.bed_dist[(sx - X + br) * (2 * br) + (sy - Y + br)]);
+        }
+      }
+      const bool bed = !(md > p.bed_out) &&
+                       (p.pp.bed_type == 1 || p.pp.bed_type == 2 || (p.pp.bed_type == 3 && md > p.bed_in));
+      if (bed) {
+        if (bi & BI_HAS10) {
+          nbed++;
+        } else if (nrow < kMaxPix) {
+          nbed++;
+          add = 1;
+        } else {
+          flags |= F_CAP_DROP;
+        }
+      }
+    }
+    const int32_t nf = nrow + add;
+    nfinal += nf;
+    chk += (uint64_t)(P + 1) * (uint64_t)nf;
+    if (p.dbg_final) p.dbg_final[P] = nf;
+  }
+  DevRecord &R = p.rec[b];
+  int64_t v[3] = {nbed, nfinal, (int64_t)chk};
+  unsigned long long *const dst[3] = {(unsigned long long *)&R.Vbed, (unsigned long long *)&R.n_final,
+                                      (unsigned long long *)&R.chk_final};
+  group_add<3>(v, red, lane, wave, tid, dst);
+  const int32_t f = group_or(flags, fred, lane, wave, tid);
+  if (tid == 0 && f) atomicOr(&R.flags, f);
+}
+
+// Stage k of a batch: 0 aabb, 1 raster, 2 compact, 3 pair, 4 bed (skipped without a bed).
+hipError_t launch_stage(int stage, const SweepParams &p, hipStream_t stream) {
+  const int32_t nb = p.n_batch;
+  const int32_t nb8 = (nb + 7) / 8 * 8;  // xcd_map pads the orientation count to whole XCD rounds
+  switch (stage) {
+    case 0: hipLaunchKernelGGL(aabb_kernel, dim3(nb), dim3(kThreads), 0, stream, p); break;
+    case 1: hipLaunchKernelGGL(raster_kernel, dim3(nb8 * p.tri_groups), dim3(kThreads), 0, stream, p); break;
+    case 2: hipLaunchKernelGGL(compact_kernel, dim3(nb), dim3(kThreads), 0, stream, p); break;
+    case 3: hipLaunchKernelGGL(pair_kernel, dim3(nb8 * p.pair_tiles), dim3(kThreads), 0, stream, p); break;
+    case 4:
+      if (p.bed_on) hipLaunchKernelGGL(bed_kernel, dim3(nb8 * p.bed_bands), dim3(kThreads), 0, stream, p);
+      break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
