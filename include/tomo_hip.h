@@ -113,6 +113,13 @@ int tomo_debug_slots(tomo_handle *h, const float ypr[3], tomo_record *rec, int32
  * region of scratch memory).  Launch-shape knob for tuning. */
 int tomo_set_launch(tomo_handle *h, int32_t batch);
 
+/* Launch-shape knobs for tuning (values take effect at the next tomo_run):
+ * TOMO_TUNE_BATCH           orientations in flight (0 = automatic), same as tomo_set_launch;
+ * TOMO_TUNE_TRI_PER_THREAD  triangles per rasteriser thread (1..64, default 4). */
+#define TOMO_TUNE_BATCH 0
+#define TOMO_TUNE_TRI_PER_THREAD 1
+int tomo_set_tuning(tomo_handle *h, int32_t knob, int32_t value);
+
 /* Duration in milliseconds of the last tomo_run's kernel, from HIP events on the handle's
  * stream (negative if unavailable). */
 float tomo_last_kernel_ms(tomo_handle *h);

@@ -57,6 +57,8 @@ def lib():
         L.orc_pair_slot.argtypes = (ct.c_int32, P, P, ct.c_float, ct.c_int32, P, P, P, P, P)
         L.orc_pair_slot.restype = ct.c_int
         L.orc_vol_to_mass.argtypes = (P, P)
+        L.orc_bary_point.argtypes = (P, P, P, P, P)
+        L.orc_bary_point.restype = ct.c_int
         L.orc_find_first_optimal.argtypes = (ct.c_int32, P)
         L.orc_find_first_optimal.restype = ct.c_int32
         _lib = L
@@ -141,6 +143,14 @@ def pair_slot(z, nz, theta_c, explicit_ss=True):
     if m < 0:
         raise ValueError("bad slot")
     return int(vo[0]), int(vss[0]), zo[:m].copy(), nzo[:m].copy(), to[:m].copy()
+
+
+def bary_point(p, a, b, c):
+    """(accepted, u, v, w) of _getBaryCoord for one pixel centre."""
+    arr = [np.ascontiguousarray(x, np.float32) for x in (p, a, b, c)]
+    uvw = np.zeros(3, np.float32)
+    ok = lib().orc_bary_point(*[_ptr(x) for x in arr], _ptr(uvw))
+    return bool(ok), uvw
 
 
 def find_first_optimal(mss):

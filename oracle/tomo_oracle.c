@@ -521,7 +521,7 @@ static void orc_eval(orc_ws *w, const orc_ctx *c, float thr, int32_t nTri, const
     if (cnt_pre) cnt_pre[s] = n;
     if (n > 1) {
       int32_t vo = 0, vss = 0;
-      orc_vslot_pair(slot, n, thr, c->explicit_ss, &vo, &vss, &w->flags);
+      n = orc_vslot_pair(slot, n, thr, c->explicit_ss, &vo, &vss, &w->flags);
       fvo += (float)vo;
       fvss += (float)vss;
       r->Vo += vo;
@@ -531,6 +531,13 @@ static void orc_eval(orc_ws *w, const orc_ctx *c, float thr, int32_t nTri, const
       if (svo) svo[s] = vo;
       if (svss) svss[s] = vss;
     }
+    /* the pre-fix reference kept per-slot sums in int16: flag slots whose rows could wrap */
+    int32_t sabs = 0;
+    for (int p = 0; p < n; p++) {
+      int32_t z = slot[(p + 1) * 3 + 1];
+      sabs += z < 0 ? -z : z;
+    }
+    if (sabs > 32767) w->flags |= ORC_F_INT16_WRAP;
   }
   orc_bed(w, c);
 
@@ -543,11 +550,10 @@ static void orc_eval(orc_ws *w, const orc_ctx *c, float thr, int32_t nTri, const
     int32_t s = w->dirty[i];
     int16_t *slot = orc_slot(w, s);
     int n = slot[0];
-    int32_t sa = 0, sb = 0, snvb = 0, snva = 0, stc = 0, sbed = 0, sabs = 0;
+    int32_t sa = 0, sb = 0, snvb = 0, snva = 0, stc = 0, sbed = 0;
     for (int p = 0; p < n; p++) {
       int16_t ty = slot[(p + 1) * 3 + 0];
       int32_t z = slot[(p + 1) * 3 + 1];
-      sabs += z < 0 ? -z : z;
       if (ty & ORC_AL) sa += z;
       if (ty & ORC_BE) sb += z;
       if (ty & ORC_NVB) snvb += z;
@@ -555,7 +561,6 @@ static void orc_eval(orc_ws *w, const orc_ctx *c, float thr, int32_t nTri, const
       if (ty & ORC_TC) stc += z;
       if (ty & ORC_BED) { sbed += z; r->n_bed++; }
     }
-    if (sabs > 32767) w->flags |= ORC_F_INT16_WRAP;
     fva += (float)sa;
     fvb += (float)sb;
     fvnv += (float)(snvb - snva);
@@ -665,6 +670,14 @@ int orc_pair_slot(int32_t n, const int16_t *z, const int16_t *nz, float theta_c,
     if (type_out) type_out[p] = slot[(p + 1) * 3 + 0];
   }
   return n;
+}
+
+int orc_bary_point(const float p[2], const float a[2], const float b[2], const float c[2], float uvw[3]) {
+  float a3[3] = {a[0], a[1], 0.0f}, b3[3] = {b[0], b[1], 0.0f}, c3[3] = {c[0], c[1], 0.0f};
+  float u = -1.0f, v = -1.0f, w = -1.0f;
+  int ok = orc_bary(p[0], p[1], a3, b3, c3, &u, &v, &w);
+  uvw[0] = u; uvw[1] = v; uvw[2] = w;
+  return ok;
 }
 
 int32_t orc_find_first_optimal(int32_t n, const float *mss) {

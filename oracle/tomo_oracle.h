@@ -84,6 +84,9 @@ int orc_pair_slot(int32_t n, const int16_t *z, const int16_t *nz, float theta_c,
 /* VolToMass (STomoVolMassInfo.cpp:42-81) on the record's float totals. */
 void orc_vol_to_mass(const orc_opts *opt, orc_record *r);
 
+/* _getBaryCoord (Tomo_types.cpp:86-115) for one point; returns 1 and u, v, w when accepted. */
+int orc_bary_point(const float p[2], const float a[2], const float b[2], const float c[2], float uvw[3]);
+
 /* First strict minimum with initial value 1e5 (Tomo_Shell2026.cpp:89-97). */
 int32_t orc_find_first_optimal(int32_t n, const float *mss);
 

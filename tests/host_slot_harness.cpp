@@ -1,4 +1,5 @@
 // Host build of the product's per-slot pairing (tomo_slot.h) for CPU unit tests.
+#include "../tomo_shell2026_amd/csrc/tomo_raster.h"
 #include "../tomo_shell2026_amd/csrc/tomo_slot.h"
 
 using namespace tomo;
@@ -29,4 +30,21 @@ int harness_pair63(const uint32_t *keys, int c, int cs, int cn, int ex, int bt, 
   return run<63>(keys, c, cs, cn, ex, bt, br, out);
 }
 uint32_t harness_make_key(int z, int nz) { return make_key(z, nz); }
+
+// Coverage test of the rasteriser for n pixel centres against one margined triangle:
+// out[i*5] = prefilter candidate, out[i*5+1] = exact accept, out[i*5+2..4] = u, v, w bits.
+int harness_bary(const float *tri6, const float *pix, int n, uint32_t *out) {
+  TriSetup s = tri_setup(tri6[0], tri6[1], tri6[2], tri6[3], tri6[4], tri6[5]);
+  for (int i = 0; i < n; i++) {
+    float nv, nw, u = -1.f, v = -1.f, w = -1.f;
+    bary_num(s, pix[2 * i], pix[2 * i + 1], nv, nw);
+    const bool cand = bary_candidate(s.den, nv, nw);
+    bool acc = !tri_degenerate(s) && bary_exact(s.den, nv, nw, u, v, w);
+    if (!acc) { u = v = w = -1.f; }
+    uint32_t ub, vb, wb;
+    __builtin_memcpy(&ub, &u, 4); __builtin_memcpy(&vb, &v, 4); __builtin_memcpy(&wb, &w, 4);
+    out[5 * i] = cand; out[5 * i + 1] = acc; out[5 * i + 2] = ub; out[5 * i + 3] = vb; out[5 * i + 4] = wb;
+  }
+  return 0;
+}
 }

@@ -127,3 +127,59 @@ def test_mesh_front_end():
     w = synth.displaced_sphere_workload()
     assert w.tri.shape == (69120, 3) and w.vtx.shape == (34562, 3)
     assert is_watertight(w.tri) and len(w.ypr) == 21960
+
+
+def _bary_cases(rng, n_tri):
+    """Random triangles: ordinary, slivers, tiny, huge, near-degenerate; pixel centres on and
+    around them (integer + 0.001 + 0.5 like the rasteriser's loop)."""
+    out = []
+    for k in range(n_tri):
+        kind = k % 5
+        c = rng.uniform(5, 200, 2)
+        if kind == 0:
+            t = c + rng.normal(0, 1.2, (3, 2))
+        elif kind == 1:  # sliver
+            d = rng.normal(0, 1, 2)
+            d /= np.linalg.norm(d)
+            t = np.stack([c, c + d * rng.uniform(2, 40), c + d * rng.uniform(2, 40) + rng.normal(0, 0.02, 2)])
+        elif kind == 2:  # tiny
+            t = c + rng.normal(0, 0.05, (3, 2))
+        elif kind == 3:  # huge
+            t = c + rng.normal(0, 30, (3, 2))
+        else:  # near-degenerate area ~ threshold
+            t = np.stack([c, c + [1.0, 0.0], c + [rng.uniform(-1, 2), rng.uniform(0.01, 0.05)]])
+        t = (t.astype(np.float32) + np.float32(0.001)).astype(np.float32)
+        x0, y0 = np.floor(t.min(0)).astype(int) - 1
+        x1, y1 = np.floor(t.max(0)).astype(int) + 1
+        xs = np.arange(x0, x1 + 1, dtype=np.float32) + np.float32(0.001) + np.float32(0.5)
+        ys = np.arange(y0, y1 + 1, dtype=np.float32) + np.float32(0.001) + np.float32(0.5)
+        X, Y = np.meshgrid(xs, ys, indexing="ij")
+        pix = np.stack([X.ravel(), Y.ravel()], 1).astype(np.float32)
+        if len(pix) > 4000:
+            pix = pix[rng.choice(len(pix), 4000, replace=False)]
+        out.append((t, pix))
+    return out
+
+
+def test_raster_coverage_matches_oracle(harness, oracle_mod):
+    """The rasteriser's exact coverage test equals _getBaryCoord bit for bit, and the division-free
+    prefilter never rejects a pixel the exact test accepts."""
+    harness.harness_bary.argtypes = (ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p)
+    harness.harness_bary.restype = ct.c_int
+    rng = np.random.default_rng(5)
+    n_acc = n_cand = n_pix = 0
+    for t, pix in _bary_cases(rng, 400):
+        tri6 = np.ascontiguousarray(t.ravel(), np.float32)
+        out = np.zeros((len(pix), 5), np.uint32)
+        harness.harness_bary(tri6.ctypes.data_as(ct.c_void_p), np.ascontiguousarray(pix).ctypes.data_as(ct.c_void_p),
+                             len(pix), out.ctypes.data_as(ct.c_void_p))
+        for i in range(0, len(pix), 7):  # oracle check on a subsample (python loop)
+            ok, uvw = oracle_mod.bary_point(pix[i], t[0], t[1], t[2])
+            assert bool(out[i, 1]) == ok
+            if ok:
+                assert tuple(out[i, 2:5]) == tuple(uvw.view(np.uint32)), (t, pix[i])
+        assert np.all(out[out[:, 1] == 1, 0] == 1), "prefilter rejected an accepted pixel"
+        n_acc += int(out[:, 1].sum())
+        n_cand += int(out[:, 0].sum())
+        n_pix += len(pix)
+    assert n_acc > 1000 and n_cand < 0.8 * n_pix

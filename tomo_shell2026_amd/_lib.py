@@ -27,7 +27,7 @@ assert RECORD_DTYPE.itemsize == 208
 # Every symbol include/tomo_hip.h declares.
 EXPORTS = (
     "tomo_create", "tomo_set_mesh", "tomo_set_orientations", "tomo_run", "tomo_sync",
-    "tomo_get_records", "tomo_sweep", "tomo_debug_slots", "tomo_set_launch",
+    "tomo_get_records", "tomo_sweep", "tomo_debug_slots", "tomo_set_launch", "tomo_set_tuning",
     "tomo_last_kernel_ms", "tomo_get_info", "tomo_set_phase_profile", "tomo_phase_profile", "tomo_destroy", "tomo_last_error",
     "tomo_find_first_optimal", "TomoSh_HIP", "getMss", "getMo", "getVtc", "getVolMassInfo",
     "getMat4x4", "getnData2i", "getpData2i", "OnDestroy",
@@ -59,6 +59,7 @@ def load():
         "tomo_sweep": ((P, I32, P, P), ct.c_int),
         "tomo_debug_slots": ((P, P, P, P, P, P, P), ct.c_int),
         "tomo_set_launch": ((P, I32), ct.c_int),
+        "tomo_set_tuning": ((P, I32, I32), ct.c_int),
         "tomo_last_kernel_ms": ((P,), F),
         "tomo_get_info": ((P, P, P, P), ct.c_int),
         "tomo_set_phase_profile": ((P, I32), ct.c_int),
@@ -144,6 +145,11 @@ class Sweep:
 
     def set_launch(self, n_groups):
         check(load().tomo_set_launch(self._h, int(n_groups)), "tomo_set_launch")
+
+    TUNE_BATCH, TUNE_TRI_PER_THREAD = 0, 1
+
+    def set_tuning(self, knob, value):
+        check(load().tomo_set_tuning(self._h, int(knob), int(value)), "tomo_set_tuning")
 
     def run(self, begin=0, end=None):
         end = self.n if end is None else end

@@ -179,8 +179,11 @@ struct tomo_handle {
   ScratchLayout lay{};
   uint8_t *d_scratch = nullptr;
   OrientHdr *d_hdr = nullptr;
+  float4 *d_pv = nullptr;      // rotated vertices per in-flight orientation
+  size_t pv_cap = 0;           // float4 capacity of d_pv
   int32_t scratch_batch = 0;
   int32_t batch_req = 0;
+  int32_t tri_per_thread = kTriPerThread;
   int32_t last_batch = 0;
   int32_t last_begin = 0, last_end = 0;
   std::vector<float> vbed_seq;
@@ -198,7 +201,23 @@ void free_stage_events(tomo_handle *h) {
   h->stage_of.clear();
 }
 
+int ensure_pv(tomo_handle *h, int32_t batch) {
+  const size_t need = (size_t)batch * (size_t)(h->n_vtx > 0 ? h->n_vtx : 1);
+  if (need <= h->pv_cap) return TOMO_OK;
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  if (h->d_pv) { HIP_TRY(hipFree(h->d_pv)); h->d_pv = nullptr; }
+  h->pv_cap = 0;
+  if (hipMalloc(&h->d_pv, sizeof(float4) * need) != hipSuccess) {
+    h->d_pv = nullptr;
+    return fail(TOMO_E_OOM, "rotated-vertex buffer allocation failed");
+  }
+  h->pv_cap = need;
+  return TOMO_OK;
+}
+
 int ensure_scratch(tomo_handle *h, int32_t batch) {
+  int rc = ensure_pv(h, batch);
+  if (rc) return rc;
   if (batch <= h->scratch_batch) return TOMO_OK;
   HIP_TRY(hipStreamSynchronize(h->stream));
   if (h->d_scratch) { HIP_TRY(hipFree(h->d_scratch)); h->d_scratch = nullptr; }
@@ -301,9 +320,12 @@ SweepParams base_params(tomo_handle *h) {
   p.bed_on = (h->opt.bed_type != 0 && h->opt.bed_r >= 1) ? 1 : 0;
   p.lay = h->lay;
   p.hdr = h->d_hdr;
+  p.pv = h->d_pv;
   p.scratch = h->d_scratch;
   const int32_t g2 = h->opt.grid * h->opt.grid;
-  p.tri_groups = (h->n_tri + kTriPerGroup - 1) / kTriPerGroup;
+  p.tri_per_thread = h->tri_per_thread;
+  const int32_t per_group = kThreads * h->tri_per_thread;
+  p.tri_groups = (h->n_tri + per_group - 1) / per_group;
   p.pair_tiles = (g2 + kPairTile - 1) / kPairTile;
   p.bed_bands = (h->opt.grid + kBandRows - 1) / kBandRows;
   return p;
@@ -471,6 +493,22 @@ int tomo_set_launch(tomo_handle *h, int32_t batch) {
   return TOMO_OK;
 }
 
+int tomo_set_tuning(tomo_handle *h, int32_t knob, int32_t value) {
+  if (!h) return fail(TOMO_E_INVALID_ARG, "null handle");
+  switch (knob) {
+    case TOMO_TUNE_BATCH:
+      if (value < 0) return fail(TOMO_E_INVALID_ARG, "batch must be >= 0");
+      h->batch_req = value;
+      return TOMO_OK;
+    case TOMO_TUNE_TRI_PER_THREAD:
+      if (value < 1 || value > 64) return fail(TOMO_E_INVALID_ARG, "triangles per thread must be 1..64");
+      h->tri_per_thread = value;
+      return TOMO_OK;
+    default:
+      return fail(TOMO_E_INVALID_ARG, "unknown tuning knob");
+  }
+}
+
 int tomo_run(tomo_handle *h, int32_t begin, int32_t end) {
   if (!h) return fail(TOMO_E_INVALID_ARG, "null handle");
   if (begin < 0 || end > h->n_orient || begin > end) return fail(TOMO_E_INVALID_ARG, "bad orientation range");
@@ -594,6 +632,7 @@ int tomo_destroy(tomo_handle *h) {
   if (!h) return TOMO_OK;
   if (h->stream) hipStreamSynchronize(h->stream);
   free_stage_events(h);
+  if (h->d_pv) hipFree(h->d_pv);
   if (h->d_hdr) hipFree(h->d_hdr);
   if (h->d_scratch) hipFree(h->d_scratch);
   if (h->d_rec) hipFree(h->d_rec);

@@ -3,6 +3,8 @@
 #pragma once
 #include <stdint.h>
 
+#include <hip/hip_runtime.h>
+
 #include "tomo_slot.h"
 
 namespace tomo {
@@ -13,8 +15,7 @@ constexpr int kSmall = 8;              // slots with <= 8 entries are paired in 
 constexpr int kInline = 16;            // entries stored inline per slot (layer-major)
 constexpr int kBigCap = 512;           // entries a single slot may hold for exact resolution
 constexpr int kMaxBedR = 16;           // int(BedOuterBound) supported by the bed kernel
-constexpr int kTriPerThread = 4;       // rasteriser: triangles per thread per work-group
-constexpr int kTriPerGroup = kThreads * kTriPerThread;
+constexpr int kTriPerThread = 4;       // rasteriser: default triangles per thread per work-group
 constexpr int kPairTile = 2048;        // pairing: touched-box slots per work-group
 constexpr int kBandRows = 4;           // bed: grid rows per work-group
 
@@ -59,6 +60,7 @@ struct SweepParams {
   const float *rot;        // [n_batch][9] row-major rotation matrices
   int32_t n_batch;
   int32_t tri_groups;      // rasteriser work-groups per orientation
+  int32_t tri_per_thread;  // triangles per rasteriser thread
   int32_t pair_tiles;      // pairing work-groups per orientation
   int32_t bed_bands;       // bed work-groups per orientation
   // options
@@ -70,6 +72,7 @@ struct SweepParams {
   const float *bed_dist;   // [2r][2r] float distances sqrtf(dx^2 + dy^2), dx,dy in [-r, r)
   // scratch and outputs
   uint8_t *scratch;
+  float4 *pv;              // [n_batch][n_vtx] rotated vertex x, y, z and normal z
   OrientHdr *hdr;          // [n_batch]
   ScratchLayout lay;
   DevRecord *rec;          // [n_batch]

@@ -16,12 +16,12 @@
 #include <hip/hip_runtime.h>
 
 #include "tomo_device.h"
+#include "tomo_raster.h"
 
 namespace tomo {
 
 namespace {
 
-constexpr float kMargin = 0.001f;           // g_fMARGIN (Tomo_types.h:40)
 constexpr float kNormFactor = 1000.0f;      // g_fNORMALFACTOR (Tomo_types.h:39)
 
 constexpr int kBedLdsBytes = 24 * 1024;  // bed band + halo flags ((4 + 2r) x (width + 2r))
@@ -135,77 +135,6 @@ struct Emit {
   }
 };
 
-// _getBaryCoord (Tomo_types.cpp:86-115) split into a per-triangle and a per-pixel part, and
-// triVoxel (STomoSh_INT3.cpp:1138-1209).  a*: placed vertices; n*z: rotated normal z.
-__device__ __forceinline__ void raster_tri(Emit &em, const SweepParams &p, int32_t t, float a0x,
-                                           float a0y, float a0z, float a1x, float a1y, float a1z,
-                                           float a2x, float a2y, float a2z, float n0z, float n1z,
-                                           float n2z, float vlim) {
-  const float v0x = a0x + kMargin, v0y = a0y + kMargin, v0z = a0z + kMargin;
-  const float v1x = a1x + kMargin, v1y = a1y + kMargin, v1z = a1z + kMargin;
-  const float v2x = a2x + kMargin, v2y = a2y + kMargin, v2z = a2z + kMargin;
-  float mnx = (v0x <= v1x) ? v0x : v1x; mnx = (mnx <= v2x) ? mnx : v2x;
-  float mny = (v0y <= v1y) ? v0y : v1y; mny = (mny <= v2y) ? mny : v2y;
-  float mxx = (v0x >= v1x) ? v0x : v1x; mxx = (mxx >= v2x) ? mxx : v2x;
-  float mxy = (v0y >= v1y) ? v0y : v1y; mxy = (mxy >= v2y) ? mxy : v2y;
-  int32_t x0 = (int32_t)mnx, y0 = (int32_t)mny, x1 = (int32_t)mxx, y1 = (int32_t)mxy;
-  // Pixels far outside the grid can only yield dropped (out-of-grid) entries: bound the loop so
-  // a wild vertex cannot stall the work-group, and report the footprint overflow.
-  const int32_t lo = -64, hi = p.grid + 64;
-  if (x0 < lo || y0 < lo || x1 > hi || y1 > hi) {
-    em.flags |= F_FOOTPRINT_OOB;
-    x0 = max(x0, lo); y0 = max(y0, lo); x1 = min(x1, hi); y1 = min(y1, hi);
-  }
-  const float e0x = v1x - v0x, e0y = v1y - v0y;
-  const float e1x = v2x - v0x, e1y = v2y - v0y;
-  const float d00 = e0x * e0x + e0y * e0y + 0.0f;
-  const float d01 = e0x * e1x + e0y * e1y + 0.0f;
-  const float d11 = e1x * e1x + e1y * e1y + 0.0f;
-  const float den = d00 * d11 - d01 * d01;
-  const float aden = (den >= 0.0f) ? den : -den;
-  int32_t hits = 0;
-  const uint32_t rank = 2u * (uint32_t)t;
-  if (aden > kMargin) {
-    for (int32_t x = x0; x <= x1; x++) {
-      const float cx = (float)x + kMargin + p.h2;
-      const float e2x = cx - v0x;
-      for (int32_t y = y0; y <= y1; y++) {
-        const float cy = (float)y + kMargin + p.h2;
-        const float e2y = cy - v0y;
-        const float d20 = e2x * e0x + e2y * e0y + 0.0f;
-        const float d21 = e2x * e1x + e2y * e1y + 0.0f;
-        const float v = (d11 * d20 - d01 * d21) / den;
-        const float w = (d00 * d21 - d01 * d20) / den;
-        const float u = (float)((1.0 - (double)v) - (double)w);
-        // (double)v <= 1. + g_fMARGIN  <=>  v <= vlim (largest float not above 1 + (double)0.001f)
-        if (!(u >= -kMargin && v >= -kMargin && v <= vlim && (u + v) <= vlim)) continue;
-        const float px = v0x * u + v1x * v + v2x * w;
-        const float py = v0y * u + v1y * v + v2y * w;
-        float pz = v0z * u + v1z * v + v2z * w;
-        float nz = n0z * u + n1z * v + n2z * w;
-        int32_t s = em.put(px, py, pz, nz, rank);
-        if (s >= 0 && s != x * p.grid + y) em.flags |= F_RANK_TIE;
-        hits++;
-        if (p.do_shell) {
-          pz = pz + ((nz >= 0.0f) ? -p.shell_off : p.shell_off);
-          nz = -nz;
-          em.put(px, py, pz, nz, rank + 1u);
-          hits++;
-        }
-      }
-    }
-  }
-  if (hits == 0) {  // very small triangle: the un-margined centroid (:1190-1206)
-    const float tx = (a0x + a1x + a2x) * 0.333333f + kMargin;
-    const float ty = (a0y + a1y + a2y) * 0.333333f + kMargin;
-    const float tz = (a0z + a1z + a2z) * 0.333333f + kMargin;
-    em.put(tx, ty, tz, n0z, rank);
-    if (p.do_shell) {
-      const float sz = tz + ((n0z >= 0.0f) ? -p.shell_off : p.shell_off);
-      em.put(tx, ty, sz, -n0z, rank + 1u);
-    }
-  }
-}
 
 // Bitonic sort of P (power of two) 64-bit values in this wave's LDS buffer, descending.
 __device__ void wave_sort_desc(uint64_t *buf, int P, int lane) {
@@ -467,7 +396,8 @@ __device__ __forceinline__ int32_t group_or(int32_t f, int32_t *red, int lane, i
 }  // namespace
 
 // ---- aabb_kernel: one work-group per orientation ------------------------------------------
-// Rotated AABB minimum (STomoSh_Base.cpp:77-109); resets the orientation's bookkeeping, and the
+// Rotates every vertex (kept for the rasteriser) and reduces the rotated AABB minimum
+// (STomoSh_Base.cpp:77-109); resets the orientation's bookkeeping, and the
 // counters of the previous orientation's large slots that pair_kernel left set.
 __global__ __launch_bounds__(kThreads) void aabb_kernel(SweepParams p) {
   __shared__ float red[kWaves][3];
@@ -483,11 +413,16 @@ __global__ __launch_bounds__(kThreads) void aabb_kernel(SweepParams p) {
     for (uint32_t j = tid; j < nbig_prev; j += kThreads) t.count[t.big[j]] = 0u;
   }
   float mx = 1e5f, my = 1e5f, mz = 1e5f;
+  float4 *pv = p.pv + (int64_t)b * p.n_vtx;
   for (int32_t v = tid; v < p.n_vtx; v += kThreads) {
     const float t0 = p.vx[v] - p.cx, t1 = p.vy[v] - p.cy, t2 = p.vz[v] - p.cz;
-    mx = fminf(mx, dot_row(r0, r1, r2, t0, t1, t2));
-    my = fminf(my, dot_row(r3, r4, r5, t0, t1, t2));
-    mz = fminf(mz, dot_row(r6, r7, r8, t0, t1, t2));
+    const float x = dot_row(r0, r1, r2, t0, t1, t2);
+    const float y = dot_row(r3, r4, r5, t0, t1, t2);
+    const float z = dot_row(r6, r7, r8, t0, t1, t2);
+    pv[v] = make_float4(x, y, z, dot_row(r6, r7, r8, p.nx[v], p.ny[v], p.nz[v]));
+    mx = fminf(mx, x);
+    my = fminf(my, y);
+    mz = fminf(mz, z);
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -517,44 +452,166 @@ __global__ __launch_bounds__(kThreads) void aabb_kernel(SweepParams p) {
   }
 }
 
-// ---- raster_kernel: kTriPerGroup triangles of one orientation per work-group ----------------
-// Pixelize (STomoSh_INT3.cpp:285-364).  Work-groups are ordered triangle-chunk-major so that
-// concurrently running groups share the same triangles in cache.
+// ---- raster_kernel: tri_per_thread * 256 triangles of one orientation per work-group -------
+// Pixelize + triVoxel (STomoSh_INT3.cpp:285-364, 1138-1209).  Each lane owns one triangle and
+// walks its pixel box with the cheap prefilter (tomo_raster.h); surviving pixels are queued in
+// LDS and the exact coverage test, interpolation and slot append run on compacted lanes.
+// Work-groups follow xcd_map so an orientation's slot table stays in one XCD's L2.
+struct RasterLds {
+  float tri[kWaves][64][13];       // margined v0, v1, v2 (x, y, z), n0z, n1z, n2z, den
+  uint32_t qmeta[kWaves][128];     // owner lane | (x + 256) << 6 | (y + 256) << 19
+  float qnv[kWaves][128], qnw[kWaves][128];
+  int32_t hits[kWaves][64];
+  int32_t red[kWaves][5];
+};
+
 __global__ __launch_bounds__(kThreads) void raster_kernel(SweepParams p) {
-  __shared__ int32_t red[kWaves][5];
+  __shared__ RasterLds L;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   int32_t b, chunk;
   if (!xcd_map(p.tri_groups, p.n_batch, b, chunk)) return;
   const int32_t G = p.grid;
-  const float *Rg = p.rot + (int64_t)b * 9;
-  const float r0 = Rg[0], r1 = Rg[1], r2 = Rg[2], r3 = Rg[3], r4 = Rg[4], r5 = Rg[5], r6 = Rg[6],
-              r7 = Rg[7], r8 = Rg[8];
   OrientHdr &h = p.hdr[b];
   const float kx = h.kx, ky = h.ky, kz = h.kz;
-  // (double)v <= 1. + g_fMARGIN, as a float bound
-  const float vlim = __uint_as_float(0x3f8020c4u);
+  const float4 *pv = p.pv + (int64_t)b * p.n_vtx;
   Emit em;
   em.G = G; em.G2 = G * G; em.t = tables_for(p, b);
   em.pool_top = &h.pool_top; em.big_top = &h.big_top;
   em.bx0 = G; em.bx1 = -1; em.by0 = G; em.by1 = -1; em.flags = 0;
-  const int32_t first = chunk * kTriPerGroup;
+  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  const int32_t first = chunk * p.tri_per_thread * kThreads;
+  float (*tri)[13] = L.tri[wave];
 #pragma unroll 1
-  for (int k = 0; k < kTriPerThread; k++) {
+  for (int k = 0; k < p.tri_per_thread; k++) {
     const int32_t tr = first + k * kThreads + tid;
-    if (tr >= p.n_tri) break;
-    const int32_t ids[3] = {p.tri[3 * tr + 0], p.tri[3 * tr + 1], p.tri[3 * tr + 2]};
-    float a[3][3], nzv[3];
+    const bool active = tr < p.n_tri;
+    float a[3][3] = {}, nzv[3] = {};
+    int32_t x0 = 0, x1 = -1, y0 = 0, y1 = -1;
+    TriSetup ts = {};
+    if (active) {
+      const int32_t ids[3] = {p.tri[3 * tr + 0], p.tri[3 * tr + 1], p.tri[3 * tr + 2]};
 #pragma unroll
-    for (int q = 0; q < 3; q++) {
-      const int32_t i = ids[q];
-      const float t0 = p.vx[i] - p.cx, t1 = p.vy[i] - p.cy, t2 = p.vz[i] - p.cz;
-      a[q][0] = dot_row(r0, r1, r2, t0, t1, t2) - kx;
-      a[q][1] = dot_row(r3, r4, r5, t0, t1, t2) - ky;
-      a[q][2] = dot_row(r6, r7, r8, t0, t1, t2) - kz;
-      nzv[q] = dot_row(r6, r7, r8, p.nx[i], p.ny[i], p.nz[i]);
+      for (int q = 0; q < 3; q++) {
+        const float4 r = pv[ids[q]];
+        a[q][0] = r.x - kx;
+        a[q][1] = r.y - ky;
+        a[q][2] = r.z - kz;
+        nzv[q] = r.w;
+      }
+      const float v0x = a[0][0] + kEps, v0y = a[0][1] + kEps;
+      const float v1x = a[1][0] + kEps, v1y = a[1][1] + kEps;
+      const float v2x = a[2][0] + kEps, v2y = a[2][1] + kEps;
+      float mnx = (v0x <= v1x) ? v0x : v1x; mnx = (mnx <= v2x) ? mnx : v2x;
+      float mny = (v0y <= v1y) ? v0y : v1y; mny = (mny <= v2y) ? mny : v2y;
+      float mxx = (v0x >= v1x) ? v0x : v1x; mxx = (mxx >= v2x) ? mxx : v2x;
+      float mxy = (v0y >= v1y) ? v0y : v1y; mxy = (mxy >= v2y) ? mxy : v2y;
+      x0 = (int32_t)mnx; y0 = (int32_t)mny; x1 = (int32_t)mxx; y1 = (int32_t)mxy;
+      // Pixels far outside the grid only yield dropped entries: bound the loop so a wild vertex
+      // cannot stall the group, and report the footprint overflow.
+      const int32_t lo = -64, hi = G + 64;
+      if (x0 < lo || y0 < lo || x1 > hi || y1 > hi) {
+        em.flags |= F_FOOTPRINT_OOB;
+        x0 = max(x0, lo); y0 = max(y0, lo); x1 = min(x1, hi); y1 = min(y1, hi);
+      }
+      ts = tri_setup(v0x, v0y, v1x, v1y, v2x, v2y);
+      if (tri_degenerate(ts)) { x1 = x0 - 1; }  // no pixel can pass
+      float *rec = tri[lane];
+      rec[0] = v0x; rec[1] = v0y; rec[2] = a[0][2] + kEps;
+      rec[3] = v1x; rec[4] = v1y; rec[5] = a[1][2] + kEps;
+      rec[6] = v2x; rec[7] = v2y; rec[8] = a[2][2] + kEps;
+      rec[9] = nzv[0]; rec[10] = nzv[1]; rec[11] = nzv[2]; rec[12] = ts.den;
     }
-    raster_tri(em, p, tr, a[0][0], a[0][1], a[0][2], a[1][0], a[1][1], a[1][2], a[2][0], a[2][1],
-               a[2][2], nzv[0], nzv[1], nzv[2], vlim);
+    L.hits[wave][lane] = 0;
+    int32_t x = x0, y = y0;
+    bool done = !active || x1 < x0 || y1 < y0;
+    uint32_t qn = 0;
+    wave_sync();
+    for (;;) {
+      bool cand = false;
+      float nv = 0.0f, nw = 0.0f;
+      const int32_t cxi = x, cyi = y;
+      if (!done) {
+        const float cx = (float)x + kEps + p.h2;
+        const float cy = (float)y + kEps + p.h2;
+        bary_num(ts, cx, cy, nv, nw);
+        cand = bary_candidate(ts.den, nv, nw);
+        if (++y > y1) {
+          y = y0;
+          if (++x > x1) done = true;
+        }
+      }
+      const uint64_t m = __ballot(cand);
+      if (cand) {
+        const uint32_t pos = qn + (uint32_t)__popcll(m & lt);
+        L.qmeta[wave][pos] = (uint32_t)lane | ((uint32_t)(cxi + 256) << 6) | ((uint32_t)(cyi + 256) << 19);
+        L.qnv[wave][pos] = nv;
+        L.qnw[wave][pos] = nw;
+      }
+      qn += (uint32_t)__popcll(m);
+      const bool all_done = __ballot(!done) == 0ull;
+      if (qn >= 64u || (all_done && qn > 0u)) {
+        wave_sync();
+        const uint32_t np = qn < 64u ? qn : 64u;
+        if ((uint32_t)lane < np) {
+          const uint32_t meta = L.qmeta[wave][lane];
+          const int32_t owner = (int32_t)(meta & 63u);
+          const int32_t px_i = (int32_t)((meta >> 6) & 8191u) - 256;
+          const int32_t py_i = (int32_t)(meta >> 19) - 256;
+          const float *rec = tri[owner];
+          float u, v, w;
+          if (bary_exact(rec[12], L.qnv[wave][lane], L.qnw[wave][lane], u, v, w)) {
+            // _bary_product (Tomo_types.cpp:75-84) of the margined vertices and the normals
+            const float px = rec[0] * u + rec[3] * v + rec[6] * w;
+            const float py = rec[1] * u + rec[4] * v + rec[7] * w;
+            float pz = rec[2] * u + rec[5] * v + rec[8] * w;
+            float nz = rec[9] * u + rec[10] * v + rec[11] * w;
+            const uint32_t rank = 2u * (uint32_t)(first + k * kThreads + wave * 64 + owner);
+            const int32_t s = em.put(px, py, pz, nz, rank);
+            if (s >= 0 && s != px_i * G + py_i) em.flags |= F_RANK_TIE;
+            atomicAdd(&L.hits[wave][owner], 1);
+            if (p.do_shell) {  // shell sub-pixel (STomoSh_INT3.cpp:1179-1185)
+              pz = pz + ((nz >= 0.0f) ? -p.shell_off : p.shell_off);
+              nz = -nz;
+              em.put(px, py, pz, nz, rank + 1u);
+            }
+          }
+        }
+        wave_sync();
+        if (qn > 64u) {
+          const uint32_t rest = qn - 64u;
+          uint32_t mm = 0u;
+          float a1 = 0.0f, a2 = 0.0f;
+          if ((uint32_t)lane < rest) {
+            mm = L.qmeta[wave][64 + lane];
+            a1 = L.qnv[wave][64 + lane];
+            a2 = L.qnw[wave][64 + lane];
+          }
+          wave_sync();
+          if ((uint32_t)lane < rest) {
+            L.qmeta[wave][lane] = mm;
+            L.qnv[wave][lane] = a1;
+            L.qnw[wave][lane] = a2;
+          }
+          wave_sync();
+        }
+        qn -= np;
+      }
+      if (all_done && qn == 0u) break;
+    }
+    wave_sync();
+    if (active && L.hits[wave][lane] == 0) {
+      // very small triangle: one pixel at the un-margined centroid (STomoSh_INT3.cpp:1190-1206)
+      const float tx = (a[0][0] + a[1][0] + a[2][0]) * 0.333333f + kEps;
+      const float ty = (a[0][1] + a[1][1] + a[2][1]) * 0.333333f + kEps;
+      const float tz = (a[0][2] + a[1][2] + a[2][2]) * 0.333333f + kEps;
+      const uint32_t rank = 2u * (uint32_t)tr;
+      em.put(tx, ty, tz, nzv[0], rank);
+      if (p.do_shell) {
+        const float sz = tz + ((nzv[0] >= 0.0f) ? -p.shell_off : p.shell_off);
+        em.put(tx, ty, sz, -nzv[0], rank + 1u);
+      }
+    }
+    wave_sync();
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -565,16 +622,16 @@ __global__ __launch_bounds__(kThreads) void raster_kernel(SweepParams p) {
     em.flags |= __shfl_xor(em.flags, off);
   }
   if (lane == 0) {
-    red[wave][0] = em.bx0; red[wave][1] = em.bx1; red[wave][2] = em.by0; red[wave][3] = em.by1;
-    red[wave][4] = em.flags;
+    L.red[wave][0] = em.bx0; L.red[wave][1] = em.bx1; L.red[wave][2] = em.by0; L.red[wave][3] = em.by1;
+    L.red[wave][4] = em.flags;
   }
   __syncthreads();
   if (tid == 0) {
     int32_t v0 = G, v1 = -1, v2 = G, v3 = -1, f = 0;
     for (int w = 0; w < kWaves; w++) {
-      v0 = min(v0, red[w][0]); v1 = max(v1, red[w][1]);
-      v2 = min(v2, red[w][2]); v3 = max(v3, red[w][3]);
-      f |= red[w][4];
+      v0 = min(v0, L.red[w][0]); v1 = max(v1, L.red[w][1]);
+      v2 = min(v2, L.red[w][2]); v3 = max(v3, L.red[w][3]);
+      f |= L.red[w][4];
     }
     if (v1 >= v0) {
       atomicMin(&h.bbox[0], v0); atomicMax(&h.bbox[1], v1);
