@@ -109,6 +109,12 @@ int tomo_sweep(tomo_handle *h, int32_t n, const float *ypr, tomo_record *out);
 int tomo_debug_slots(tomo_handle *h, const float ypr[3], tomo_record *rec, int32_t *cnt_pre,
                      int32_t *cnt_final, int32_t *slot_vo, int32_t *slot_vss);
 
+/* Final rows of every slot of one orientation (the render export's input): rows[grid*grid*64]
+ * as (type << 32 | key) with key = ((z + 32768) << 16) | (nZ + 32768) and type the pixel-type
+ * bits (Tomo_types.h:72-82); nrows[grid*grid] rows per slot.  Replaces reading the slot buffer
+ * of the optimum (Tomo_Shell2026.cpp:219-236). */
+int tomo_debug_rows(tomo_handle *h, const float ypr[3], tomo_record *rec, uint64_t *rows, int32_t *nrows);
+
 /* Orientations in flight per batch for tomo_run (0 = automatic; each needs one slot-table
  * region of scratch memory).  Launch-shape knob for tuning. */
 int tomo_set_launch(tomo_handle *h, int32_t batch);

@@ -54,6 +54,8 @@ def lib():
         L.orc_sweep.restype = ct.c_int
         L.orc_one.argtypes = (P, ct.c_int32, P, ct.c_int32, P, P, P, P, P, P, P, P)
         L.orc_one.restype = ct.c_int
+        L.orc_one_rows.argtypes = (P, ct.c_int32, P, ct.c_int32, P, P, P, P, P, P, P, P, P)
+        L.orc_one_rows.restype = ct.c_int
         L.orc_pair_slot.argtypes = (ct.c_int32, P, P, ct.c_float, ct.c_int32, P, P, P, P, P)
         L.orc_pair_slot.restype = ct.c_int
         L.orc_vol_to_mass.argtypes = (P, P)
@@ -126,6 +128,21 @@ def one(f32, i32, grid, tri, vtx, vnrm, ypr3):
     if rc != 0:
         raise RuntimeError(f"orc_one failed: {rc}")
     return (out[0],) + tuple(arrs)
+
+
+def one_rows(f32, i32, grid, tri, vtx, vnrm, ypr3):
+    """One orientation: (record, cnt_final, rows[grid*grid*64] uint64)."""
+    tri, vtx, vnrm = _mesh_arrays(tri, vtx, vnrm)
+    y = np.ascontiguousarray(ypr3, dtype=np.float32).reshape(3)
+    out = np.zeros(1, dtype=RECORD_DTYPE)
+    fin = np.zeros(grid * grid, np.int32)
+    rows = np.zeros(grid * grid * 64, np.uint64)
+    o = make_opts(f32, i32, grid)
+    rc = lib().orc_one_rows(ct.byref(o), tri.shape[0], _ptr(tri), vtx.shape[0], _ptr(vtx), _ptr(vnrm),
+                            _ptr(y), _ptr(out), None, _ptr(fin), None, None, _ptr(rows))
+    if rc != 0:
+        raise RuntimeError(f"orc_one_rows failed: {rc}")
+    return out[0], fin, rows
 
 
 def pair_slot(z, nz, theta_c, explicit_ss=True):

@@ -587,6 +587,14 @@ static void orc_eval(orc_ws *w, const orc_ctx *c, float thr, int32_t nTri, const
 int orc_one(const orc_opts *opt, int32_t nTri, const int32_t *tri, int32_t nVtx,
             const float *vtx, const float *vnrm, const float ypr[3], orc_record *out,
             int32_t *cnt_pre, int32_t *cnt_final, int32_t *slot_vo, int32_t *slot_vss) {
+  return orc_one_rows(opt, nTri, tri, nVtx, vtx, vnrm, ypr, out, cnt_pre, cnt_final, slot_vo, slot_vss,
+                      NULL);
+}
+
+int orc_one_rows(const orc_opts *opt, int32_t nTri, const int32_t *tri, int32_t nVtx,
+                 const float *vtx, const float *vnrm, const float ypr[3], orc_record *out,
+                 int32_t *cnt_pre, int32_t *cnt_final, int32_t *slot_vo, int32_t *slot_vss,
+                 uint64_t *rows) {
   if (!opt || !out || nTri < 0 || nVtx < 0 || (nTri > 0 && (!tri || !vtx || !vnrm))) return -1;
   orc_ctx c;
   orc_ctx_init(&c, opt, nVtx, vtx);
@@ -594,6 +602,17 @@ int orc_one(const orc_opts *opt, int32_t nTri, const int32_t *tri, int32_t nVtx,
   if (orc_ws_init(&w, c.grid, nVtx)) { orc_ws_free(&w); return -2; }
   float thr = (float)(-1.0 * sin((double)c.theta_c));
   orc_eval(&w, &c, thr, nTri, tri, nVtx, vtx, vnrm, ypr, out, cnt_pre, cnt_final, slot_vo, slot_vss);
+  if (rows) { /* final slot contents: (type << 32) | ((z + 32768) << 16) | (nZ + 32768) */
+    int64_t g2 = (int64_t)c.grid * c.grid;
+    memset(rows, 0, sizeof(uint64_t) * 64 * (size_t)g2);
+    for (int64_t sl = 0; sl < g2; sl++) {
+      int16_t *slot = orc_slot(&w, sl);
+      for (int p = 0; p < slot[0]; p++) {
+        uint32_t key = ((uint32_t)(slot[(p + 1) * 3 + 1] + 32768) << 16) | (uint32_t)(slot[(p + 1) * 3 + 2] + 32768);
+        rows[sl * 64 + p] = ((uint64_t)(uint16_t)slot[(p + 1) * 3 + 0] << 32) | key;
+      }
+    }
+  }
   orc_ws_free(&w);
   return 0;
 }

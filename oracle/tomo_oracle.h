@@ -76,6 +76,13 @@ int orc_one(const orc_opts *opt, int32_t nTri, const int32_t *tri, int32_t nVtx,
             const float *vtx, const float *vnrm, const float ypr[3], orc_record *out,
             int32_t *cnt_pre, int32_t *cnt_final, int32_t *slot_vo, int32_t *slot_vss);
 
+/* orc_one plus the final rows of every slot (rows[grid*grid*64], may be NULL) as
+ * (type << 32) | ((z + 32768) << 16) | (nZ + 32768). */
+int orc_one_rows(const orc_opts *opt, int32_t nTri, const int32_t *tri, int32_t nVtx,
+                 const float *vtx, const float *vnrm, const float ypr[3], orc_record *out,
+                 int32_t *cnt_pre, int32_t *cnt_final, int32_t *slot_vo, int32_t *slot_vss,
+                 uint64_t *rows);
+
 /* Pair one hand-made slot (rows of z and nZ, any order, distinct keys) the way vslotPair does
  * (STomoSh_INT3.cpp:1041-1062).  Returns the new slot length; writes Vo/Vss and row types. */
 int orc_pair_slot(int32_t n, const int16_t *z, const int16_t *nz, float theta_c, int32_t explicit_ss,

@@ -141,3 +141,43 @@ def test_full_sweep_properties():
     vo = a["Vo"].astype(np.float64)
     assert vo.std() / vo.mean() < 0.01
     assert (a["flags"] & (4 | 64)).sum() == 0
+
+
+def _render_groups(rows, nrows, grid):
+    """Pixel groups as slotsToPxls / GetSSPixels form them (STomoSh_INT3.cpp:1103-1135,
+    1257-1318) from a slot-row dump: {type: [(x, y, z, 0, 0, nZ), ...]}."""
+    r = rows.reshape(grid * grid, 64)
+    groups = {t: [] for t in range(11)}
+    ss = []
+    for s_ in np.nonzero(nrows)[0]:
+        x, y = divmod(int(s_), grid)
+        start = end = -1
+        for q in range(nrows[s_]):
+            v = int(r[s_, q])
+            t, key = v >> 32, v & 0xffffffff
+            z, nz = (key >> 16) - 32768, (key & 0xffff) - 32768
+            for k in range(11):
+                if t & (1 << k):
+                    groups[k].append((x, y, z, 0, 0, nz))
+            if t & 4:
+                start = z
+            if t & 8:
+                end = z
+            if start > -1 and end > -1:
+                ss += [(x, y, zz, 0, 0, 0) for zz in range(start, end - 1, -1)]
+                start = end = -1
+    groups[4] = ss
+    return groups
+
+
+@pytest.mark.parametrize("which", ["icosphere", "dsphere"])
+def test_final_rows_match_oracle(oracle_mod, which):
+    w = synth.icosphere_workload() if which == "icosphere" else synth.displaced_sphere_workload()
+    announce(f"final slot rows {which}")
+    with _lib.Sweep(w.f32, w.i32, w.grid) as s:
+        s.set_mesh(w.tri, w.vtx, w.vnrm)
+        for i in (0, len(w.ypr) // 3, len(w.ypr) - 1):
+            rec, rows, nrows = s.debug_rows(w.ypr[i])
+            rrec, rfin, rrows = oracle_mod.one_rows(w.f32, w.i32, w.grid, w.tri, w.vtx, w.vnrm, w.ypr[i])
+            np.testing.assert_array_equal(nrows, rfin)
+            np.testing.assert_array_equal(rows, rrows)

@@ -27,7 +27,7 @@ assert RECORD_DTYPE.itemsize == 208
 # Every symbol include/tomo_hip.h declares.
 EXPORTS = (
     "tomo_create", "tomo_set_mesh", "tomo_set_orientations", "tomo_run", "tomo_sync",
-    "tomo_get_records", "tomo_sweep", "tomo_debug_slots", "tomo_set_launch", "tomo_set_tuning",
+    "tomo_get_records", "tomo_sweep", "tomo_debug_slots", "tomo_debug_rows", "tomo_set_launch", "tomo_set_tuning",
     "tomo_last_kernel_ms", "tomo_get_info", "tomo_set_phase_profile", "tomo_phase_profile", "tomo_destroy", "tomo_last_error",
     "tomo_find_first_optimal", "TomoSh_HIP", "getMss", "getMo", "getVtc", "getVolMassInfo",
     "getMat4x4", "getnData2i", "getpData2i", "OnDestroy",
@@ -58,6 +58,7 @@ def load():
         "tomo_get_records": ((P, I32, I32, P), ct.c_int),
         "tomo_sweep": ((P, I32, P, P), ct.c_int),
         "tomo_debug_slots": ((P, P, P, P, P, P, P), ct.c_int),
+        "tomo_debug_rows": ((P, P, P, P, P), ct.c_int),
         "tomo_set_launch": ((P, I32), ct.c_int),
         "tomo_set_tuning": ((P, I32, I32), ct.c_int),
         "tomo_last_kernel_ms": ((P,), F),
@@ -189,6 +190,15 @@ class Sweep:
         check(load().tomo_phase_profile(self._h, ptr(out)), "tomo_phase_profile")
         return {k: float(out[i]) for i, k in enumerate(self.PHASES)}
 
+    def debug_rows(self, ypr3):
+        y = np.ascontiguousarray(ypr3, np.float32).reshape(3)
+        g2 = self.info()["grid"] ** 2
+        rec = np.zeros(1, RECORD_DTYPE)
+        rows = np.zeros(g2 * 64, np.uint64)
+        nrows = np.zeros(g2, np.int32)
+        check(load().tomo_debug_rows(self._h, ptr(y), ptr(rec), ptr(rows), ptr(nrows)), "tomo_debug_rows")
+        return rec[0], rows, nrows
+
     def debug_slots(self, ypr3):
         y = np.ascontiguousarray(ypr3, np.float32).reshape(3)
         info = self.info()
@@ -198,6 +208,18 @@ class Sweep:
         check(load().tomo_debug_slots(self._h, ptr(y), ptr(rec), *[ptr(a) for a in arrs]),
               "tomo_debug_slots")
         return (rec[0],) + tuple(arrs)
+
+
+def decode_rows(rows, nrows):
+    """Split tomo_debug_rows output into per-row (slot, type, z, nZ) arrays."""
+    g2 = nrows.size
+    r = rows.reshape(g2, 64)
+    idx = np.arange(64)[None, :] < nrows[:, None]
+    slot = np.nonzero(idx)[0]
+    v = r[idx]
+    typ = (v >> np.uint64(32)).astype(np.int64)
+    key = (v & np.uint64(0xffffffff)).astype(np.int64)
+    return slot, typ, (key >> 16) - 32768, (key & 0xffff) - 32768
 
 
 def find_first_optimal(mss):

@@ -157,6 +157,13 @@ ScratchLayout make_layout(int32_t G) {
 
 }  // namespace
 
+// Legacy shim outputs (Tomo_Shell2026.cpp:19-35).
+static std::vector<float> g_mss, g_mo, g_vtc;
+static float g_vm[22];
+static float g_mat[16];
+static bool g_have = false;
+static std::vector<std::vector<int16_t>> g_pix(11);
+
 struct tomo_handle {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -332,7 +339,8 @@ SweepParams base_params(tomo_handle *h) {
 }
 
 // Enqueue the five stages for orientations [begin, end) in batches of `batch`.
-int enqueue(tomo_handle *h, int32_t begin, int32_t end, int32_t batch, int32_t *dbg4, size_t g2) {
+int enqueue(tomo_handle *h, int32_t begin, int32_t end, int32_t batch, int32_t *dbg4, size_t g2,
+            uint64_t *dbg_rows = nullptr) {
   for (int32_t b0 = begin; b0 < end; b0 += batch) {
     SweepParams p = base_params(h);
     p.n_batch = (end - b0) < batch ? (end - b0) : batch;
@@ -344,6 +352,7 @@ int enqueue(tomo_handle *h, int32_t begin, int32_t end, int32_t batch, int32_t *
       p.dbg_vo = dbg4 + 2 * g2;
       p.dbg_vss = dbg4 + 3 * g2;
     }
+    p.dbg_rows = dbg_rows;
     if (p.tri_groups == 0) p.tri_groups = 1;
     for (int stage = 0; stage < 5; stage++) {
       if (h->phases) {
@@ -574,8 +583,9 @@ int tomo_get_info(tomo_handle *h, int32_t *grid, int32_t *batch, int64_t *scratc
   return TOMO_OK;
 }
 
-int tomo_debug_slots(tomo_handle *h, const float ypr[3], tomo_record *rec, int32_t *cnt_pre,
-                     int32_t *cnt_final, int32_t *slot_vo, int32_t *slot_vss) {
+// One orientation with per-slot dumps and optionally the final rows of every slot.
+static int debug_run(tomo_handle *h, const float ypr[3], tomo_record *rec, int32_t *cnt_pre,
+                     int32_t *cnt_final, int32_t *slot_vo, int32_t *slot_vss, uint64_t *rows) {
   if (!h || !ypr || !rec) return fail(TOMO_E_INVALID_ARG, "null argument");
   if (!h->d_soa || !h->d_tri) return fail(TOMO_E_STATE, "no mesh");
   int rc = tomo_set_orientations(h, 1, ypr);
@@ -584,15 +594,21 @@ int tomo_debug_slots(tomo_handle *h, const float ypr[3], tomo_record *rec, int32
   if (rc) return rc;
   const size_t g2 = (size_t)h->opt.grid * h->opt.grid;
   int32_t *d = nullptr;
+  uint64_t *drows = nullptr;
   HIP_TRY(hipMalloc(&d, sizeof(int32_t) * 4 * g2));
   hipError_t e = hipMemsetAsync(d, 0, sizeof(int32_t) * 4 * g2, h->stream);
+  if (e == hipSuccess && rows) {
+    e = hipMalloc(&drows, sizeof(uint64_t) * 64 * g2);
+    if (e == hipSuccess) e = hipMemsetAsync(drows, 0, sizeof(uint64_t) * 64 * g2, h->stream);
+  }
   const bool was = h->phases;
   h->phases = false;
   if (e == hipSuccess) {
-    rc = enqueue(h, 0, 1, 1, d, g2);
+    rc = enqueue(h, 0, 1, 1, d, g2, drows);
     if (rc) {
       h->phases = was;
       hipFree(d);
+      if (drows) hipFree(drows);
       return rc;
     }
     e = hipStreamSynchronize(h->stream);
@@ -601,11 +617,23 @@ int tomo_debug_slots(tomo_handle *h, const float ypr[3], tomo_record *rec, int32
   int32_t *outs[4] = {cnt_pre, cnt_final, slot_vo, slot_vss};
   for (int k = 0; k < 4 && e == hipSuccess; k++)
     if (outs[k]) e = hipMemcpy(outs[k], d + k * g2, sizeof(int32_t) * g2, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && rows) e = hipMemcpy(rows, drows, sizeof(uint64_t) * 64 * g2, hipMemcpyDeviceToHost);
   hipFree(d);
+  if (drows) hipFree(drows);
   if (e != hipSuccess) return fail(TOMO_E_HIP, std::string("debug sweep: ") + hipGetErrorString(e));
   h->last_begin = 0;
   h->last_end = 1;
   return tomo_get_records(h, 0, 1, rec);
+}
+
+int tomo_debug_slots(tomo_handle *h, const float ypr[3], tomo_record *rec, int32_t *cnt_pre,
+                     int32_t *cnt_final, int32_t *slot_vo, int32_t *slot_vss) {
+  return debug_run(h, ypr, rec, cnt_pre, cnt_final, slot_vo, slot_vss, nullptr);
+}
+
+int tomo_debug_rows(tomo_handle *h, const float ypr[3], tomo_record *rec, uint64_t *rows, int32_t *nrows) {
+  if (!rows || !nrows) return fail(TOMO_E_INVALID_ARG, "null argument");
+  return debug_run(h, ypr, rec, nullptr, nrows, nullptr, nullptr, rows);
 }
 
 int tomo_set_phase_profile(tomo_handle *h, int32_t enable) {
@@ -649,11 +677,53 @@ int tomo_destroy(tomo_handle *h) {
 
 // ---------------------------------------------------------------------------------------------
 // Legacy shim: global outputs, freed on the next call or OnDestroy (Tomo_Shell2026.cpp:19-87).
-static std::vector<float> g_mss, g_mo, g_vtc;
-static float g_vm[22];
-static float g_mat[16];
-static bool g_have = false;
-static std::vector<std::vector<int16_t>> g_pix(11);
+
+// Pixel groups of one orientation for rendering: slotsToPxls for every pixel type, then the
+// support columns of GetSSPixels in place of the SS group (STomoSh_INT3.cpp:1103-1135,
+// 1257-1318; Tomo_Shell2026.cpp:221-236).  Rows {x, y, z, nx, ny, nz} as int16.
+static int render_export(tomo_handle *h, const float *ypr) {
+  const int32_t G = h->opt.grid;
+  const size_t g2 = (size_t)G * G;
+  std::vector<uint64_t> rows(g2 * 64);
+  std::vector<int32_t> nrows(g2);
+  tomo_record r;
+  int rc = tomo_debug_rows(h, ypr, &r, rows.data(), nrows.data());
+  if (rc) return rc;
+  for (auto &v : g_pix) v.clear();
+  auto push = [](std::vector<int16_t> &dst, int x, int y, int z, int nz) {
+    dst.push_back((int16_t)x); dst.push_back((int16_t)y); dst.push_back((int16_t)z);
+    dst.push_back(0); dst.push_back(0); dst.push_back((int16_t)nz);
+  };
+  for (int x = 0; x < G; x++)
+    for (int y = 0; y < G; y++) {
+      const size_t s = (size_t)x * G + y;
+      const uint64_t *rw = &rows[s * 64];
+      for (int q = 0; q < nrows[s] && q < 64; q++) {
+        const uint32_t type = (uint32_t)(rw[q] >> 32), key = (uint32_t)rw[q];
+        for (int t = 0; t < 11; t++)
+          if (type & (1u << t)) push(g_pix[(size_t)t], x, y, key_z(key), key_nz(key));
+      }
+    }
+  std::vector<int16_t> &ss = g_pix[4];
+  ss.clear();
+  for (int x = 0; x < G; x++)
+    for (int y = 0; y < G; y++) {
+      const size_t s = (size_t)x * G + y;
+      const uint64_t *rw = &rows[s * 64];
+      int start = -1, end = -1;
+      for (int q = 0; q < nrows[s] && q < 64; q++) {
+        const uint32_t type = (uint32_t)(rw[q] >> 32);
+        const int z = key_z((uint32_t)rw[q]);
+        if (type & T_SSB) start = z;
+        if (type & T_SSA) end = z;
+        if (start > -1 && end > -1) {
+          for (int zz = start; zz >= end; zz--) push(ss, x, y, zz, 0);
+          start = end = -1;
+        }
+      }
+    }
+  return TOMO_OK;
+}
 
 int32_t TomoSh_HIP(float *f32, int32_t *i32, float *YPR, int32_t *tri, float *vtx, float *vtx_nrm,
                    float *tri_nrm, int32_t *chull_tri, float *chull_vtx, float *chull_trinrm) {
@@ -684,6 +754,11 @@ int32_t TomoSh_HIP(float *f32, int32_t *i32, float *YPR, int32_t *tri, float *vt
   if (n_ypr > 0) {
     vm_from_record(h, rec[opt], g_vm);
     std::memcpy(g_mat, rec[opt].mat4x4, sizeof(g_mat));
+    rc = render_export(h, &YPR[(size_t)opt * 3]);
+    if (rc) {
+      tomo_destroy(h);
+      return rc;
+    }
   }
   g_have = true;
   tomo_destroy(h);

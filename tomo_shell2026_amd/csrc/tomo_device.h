@@ -78,6 +78,7 @@ struct SweepParams {
   DevRecord *rec;          // [n_batch]
   // optional per-slot dumps for a one-orientation batch (grid*grid each, zero-filled by host)
   int32_t *dbg_pre, *dbg_final, *dbg_vo, *dbg_vss;
+  uint64_t *dbg_rows;      // [grid*grid][64] final rows (type << 32 | key) for the render export
 };
 
 }  // namespace tomo

@@ -164,7 +164,8 @@ __device__ __forceinline__ int pow2_at_least(int n) {
 // Pair the distinct keys of the lanes set in V (at most 63), in descending key order from low
 // to high lane.  Same semantics as pair_keys in tomo_slot.h, with the sequential state machines
 // turned into "last event before this lane" ballot scans.  The result is valid in every lane.
-__device__ void wave_pair(uint32_t key, uint64_t V, int lane, const PairParams &pp, SlotResult &r) {
+__device__ void wave_pair(uint32_t key, uint64_t V, int lane, const PairParams &pp, SlotResult &r,
+                          uint64_t *rows) {
   const bool valid = ((V >> lane) & 1ull) != 0;
   const int D = __popcll(V);
   const int32_t z = key_z(key), nz = key_nz(key);
@@ -179,6 +180,7 @@ __device__ void wave_pair(uint32_t key, uint64_t V, int lane, const PairParams &
     r.n = D;
     r.last_z = D ? __shfl(z, msb64(V)) : 0;
     r.last_type = 0;
+    if (rows && valid) rows[0] = make_row(0, key);
     return;
   }
   const bool isA = valid && nz >= 1, isB = valid && nz <= -1;
@@ -204,12 +206,19 @@ __device__ void wave_pair(uint32_t key, uint64_t V, int lane, const PairParams &
   const int32_t sTC = wave_sum(tc ? z : 0);
   const int lastl = msb64(V);
   int32_t last_z = __shfl(z, lastl), last_type = __shfl(ty, lastl);
+  if (rows && valid) rows[__popcll(V & lt)] = make_row(ty, key);
   int n = D;
   if (exOpen) {
-    if (n < kMaxPix) { n++; last_z = 0; last_type = T_SSA; } else r.flags |= F_SENTINEL_DROP;
+    if (n < kMaxPix) {
+      if (rows && lane == 0) rows[n] = make_row(T_SSA, make_key(0, 0));
+      n++; last_z = 0; last_type = T_SSA;
+    } else r.flags |= F_SENTINEL_DROP;
   }
   if (imOpen) {
-    if (n < kMaxPix) { n++; last_z = 0; last_type = T_NVA; } else r.flags |= F_SENTINEL_DROP;
+    if (n < kMaxPix) {
+      if (rows && lane == 0) rows[n] = make_row(T_NVA, make_key(0, 0));
+      n++; last_z = 0; last_type = T_NVA;
+    } else r.flags |= F_SENTINEL_DROP;
   }
   r.n = n;
   r.last_z = last_z;
@@ -230,7 +239,7 @@ __device__ void wave_pair(uint32_t key, uint64_t V, int lane, const PairParams &
 // kept set: the first 63 distinct keys in insertion order, insertion order being (triangle,
 // main-before-sub) = ascending rank (STomoVoxelSpaceInfo.cpp:200-220).
 __device__ void big_slot(int32_t s, uint32_t c, const Tables &t, int32_t G2, uint64_t *buf,
-                         int lane, const PairParams &pp, SlotResult &r) {
+                         int lane, const PairParams &pp, SlotResult &r, uint64_t *rows) {
   int32_t fl = 0;
   if (c > (uint32_t)kBigCap) {
     fl |= F_UNRESOLVED;
@@ -307,14 +316,14 @@ __device__ void big_slot(int32_t s, uint32_t c, const Tables &t, int32_t G2, uin
     mykey = (lane < D) ? (uint32_t)buf[lane] : 0u;
   }
   wave_sync();
-  wave_pair(mykey, (D >= 64) ? ~0ull : ((1ull << D) - 1ull), lane, pp, r);
+  wave_pair(mykey, (D >= 64) ? ~0ull : ((1ull << D) - 1ull), lane, pp, r, rows);
   r.flags |= fl;
 }
 
 // A slot with kSmall < c < 64 entries: one key per lane, register bitonic sort over the wave
 // with lane shuffles, duplicates masked out.  No ranks are needed: at most 63 distinct keys.
 __device__ void mid_slot(int32_t s, uint32_t c, const Tables &t, int32_t G2, int lane,
-                         const PairParams &pp, SlotResult &r) {
+                         const PairParams &pp, SlotResult &r, uint64_t *rows) {
   uint32_t key = 0u;
   if (lane < (int)c) {
     const uint32_t off = (c > (uint32_t)kInline) ? t.off[s] : 0u;
@@ -331,7 +340,7 @@ __device__ void mid_slot(int32_t s, uint32_t c, const Tables &t, int32_t G2, int
   }
   const uint32_t prev = __shfl_up(key, 1);
   const bool first = lane < (int)c && (lane == 0 || key != prev);
-  wave_pair(key, __ballot(first), lane, pp, r);
+  wave_pair(key, __ballot(first), lane, pp, r, rows);
 }
 
 // Work-groups are dispatched to the 8 XCDs round-robin by block index.  Map block i to
@@ -738,7 +747,7 @@ __global__ __launch_bounds__(kThreads) void pair_kernel(SweepParams p) {
 #pragma unroll
     for (int q = 0; q < kSmall; q++) k[q] = (q < (int)c) ? t.ent[(int64_t)q * G2 + s].x : 0u;
     SlotResult r;
-    pair_keys<kSmall>(k, (int)c, p.pp, r);
+    pair_keys<kSmall>(k, (int)c, p.pp, r, p.dbg_rows ? p.dbg_rows + (int64_t)s * 64 : nullptr);
     account(acc, r, s, p);
     t.bedinfo[s] = bed_info(r, p.pp);
     if (final_here) {
@@ -754,9 +763,9 @@ __global__ __launch_bounds__(kThreads) void pair_kernel(SweepParams p) {
     const uint32_t c = t.count[s];
     SlotResult r;
     if (c < 64u && (c <= (uint32_t)kInline || npool <= (uint32_t)t.pool))
-      mid_slot(s, c, t, G2, lane, p.pp, r);
+      mid_slot(s, c, t, G2, lane, p.pp, r, p.dbg_rows ? p.dbg_rows + (int64_t)s * 64 : nullptr);
     else
-      big_slot(s, c, t, G2, sortbuf[wave], lane, p.pp, r);
+      big_slot(s, c, t, G2, sortbuf[wave], lane, p.pp, r, p.dbg_rows ? p.dbg_rows + (int64_t)s * 64 : nullptr);
     if (lane == 0) {
       acc.nslot++;
       account(acc, r, s, p);
@@ -862,11 +871,18 @@ __global__ __launch_bounds__(kThreads) void bed_kernel(SweepParams p) {
       const bool bed = !(md > p.bed_out) &&
                        (p.pp.bed_type == 1 || p.pp.bed_type == 2 || (p.pp.bed_type == 3 && md > p.bed_in));
       if (bed) {
+        const uint32_t k10 = make_key(1, 0);
         if (bi & BI_HAS10) {
           nbed++;
+          if (p.dbg_rows) {  // the bed bit merges into the existing (1, 0) row
+            uint64_t *rw = p.dbg_rows + (int64_t)P * 64;
+            for (int q = 0; q < nrow; q++)
+              if ((uint32_t)rw[q] == k10) rw[q] |= (uint64_t)T_BED << 32;
+          }
         } else if (nrow < kMaxPix) {
           nbed++;
           add = 1;
+          if (p.dbg_rows) p.dbg_rows[(int64_t)P * 64 + nrow] = make_row(T_BED, k10);
         } else {
           flags |= F_CAP_DROP;
         }

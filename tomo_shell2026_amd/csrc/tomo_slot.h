@@ -96,10 +96,15 @@ TOMO_HD uint32_t bed_info(const SlotResult &r, const PairParams &pp) {
   return b;
 }
 
+// A final slot row for the render export: (type << 32) | key.
+TOMO_HD uint64_t make_row(int32_t type, uint32_t key) { return ((uint64_t)(uint32_t)type << 32) | key; }
+
 // Pair N register-resident keys, the first c valid (c <= N <= 63), any order, duplicates
-// allowed.  Fully unrolled so the keys stay in registers.
+// allowed.  Fully unrolled so the keys stay in registers.  rows (optional, 64 entries) receives
+// the slot's final rows in slot order, floor rows included.
 template <int N>
-TOMO_HD void pair_keys(uint32_t (&k)[N], int c, const PairParams &pp, SlotResult &r) {
+TOMO_HD void pair_keys(uint32_t (&k)[N], int c, const PairParams &pp, SlotResult &r,
+                       uint64_t *rows = nullptr) {
   // Sort descending.  N == 8 uses a 19-comparator network; other N an odd-even transposition
   // network (used by the host unit test for larger N).
   if constexpr (N == 8) {
@@ -151,11 +156,13 @@ TOMO_HD void pair_keys(uint32_t (&k)[N], int c, const PairParams &pp, SlotResult
     r.n = npre;
     r.last_z = npre ? key_z(k[0]) : 0;
     r.last_type = 0;
+    if (rows && npre) rows[0] = make_row(0, k[0]);
     return;
   }
   bool started = false, ex = false, im = false, tc_done = false;
   int32_t al = 0, be = 0, ssb = 0, ssa = 0, nvb = 0, nva = 0, tc = 0;
   int32_t last_z = 0, last_type = 0;
+  int row = 0;
 #pragma unroll
   for (int i = 0; i < N; i++) {
     bool d = (i < c) && (i == 0 || k[i] != k[i - 1]);
@@ -178,14 +185,22 @@ TOMO_HD void pair_keys(uint32_t (&k)[N], int c, const PairParams &pp, SlotResult
       }
       last_z = z;
       last_type = ty;
+      if (rows) rows[row] = make_row(ty, k[i]);
+      row++;
     }
   }
   int n = npre;
   if (ex) {
-    if (n < kMaxPix) { n++; last_z = 0; last_type = T_SSA; } else r.flags |= F_SENTINEL_DROP;
+    if (n < kMaxPix) {
+      if (rows) rows[n] = make_row(T_SSA, make_key(0, 0));
+      n++; last_z = 0; last_type = T_SSA;
+    } else r.flags |= F_SENTINEL_DROP;
   }
   if (im) {
-    if (n < kMaxPix) { n++; last_z = 0; last_type = T_NVA; } else r.flags |= F_SENTINEL_DROP;
+    if (n < kMaxPix) {
+      if (rows) rows[n] = make_row(T_NVA, make_key(0, 0));
+      n++; last_z = 0; last_type = T_NVA;
+    } else r.flags |= F_SENTINEL_DROP;
   }
   r.n = n;
   r.last_z = last_z;
