@@ -23,12 +23,58 @@ HBM_PEAK_GBS = 8000.0
 
 
 def algorithmic_bytes(n_vtx, n_tri, rec):
-    """Bytes one orientation must move at minimum: SoA fp32 position + normal per vertex, int32
-    indices, a write + read of an 8-byte entry per rasterised row, a write + read of a 4-byte
-    counter per touched slot, and the 64-byte result record."""
+    """Bytes one orientation must move at minimum over the whole sweep: SoA fp32 position +
+    normal per vertex, int32 indices, a write + read of an 8-byte entry per rasterised row, a
+    write + read of a 4-byte counter per touched slot, and the 64-byte result record."""
     p_bar = float(rec["n_entries"].mean())
     s_bar = float(rec["n_slots"].mean())
     return 24.0 * n_vtx + 12.0 * n_tri + 16.0 * p_bar + 8.0 * s_bar + 64.0, p_bar, s_bar
+
+
+def raster_bytes(n_vtx, n_tri, p_bar, s_bar):
+    """Bytes the rasteriser kernel must move per orientation: int32 indices, a 16-byte rotated
+    vertex (x, y, z, normal z) per vertex, an 8-byte slot entry written per rasterised row and a
+    4-byte slot counter read and written per touched slot."""
+    return 12.0 * n_tri + 16.0 * n_vtx + 8.0 * p_bar + 8.0 * s_bar
+
+
+def copy_calibration(gib=1):
+    """A plain device-to-device copy of known size, for scaling FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    n = gib << 28
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_per_launch": n * 4, "launches": 3}), flush=True)
+
+
+def summarize_pmc(fetch_dir, write_dir, cfetch_dir, cwrite_dir, kernel="raster_kernel"):
+    """Per-launch FETCH_SIZE / WRITE_SIZE of `kernel` from two rocprofv3 --pmc runs, scaled by the
+    same counters on the copy calibration (bytes read / written by a copy of known size)."""
+    import csv
+    import glob
+
+    def per_launch(d, name, counter):
+        vals = []
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(f) as fh:
+                for row in csv.DictReader(fh):
+                    if name in row["Kernel_Name"] and row["Counter_Name"] == counter:
+                        vals.append(float(row["Counter_Value"]))
+        return sum(vals) / len(vals), len(vals)
+
+    fk, n1 = per_launch(fetch_dir, kernel, "FETCH_SIZE")
+    wk, n2 = per_launch(write_dir, kernel, "WRITE_SIZE")
+    cf, _ = per_launch(cfetch_dir, "copy", "FETCH_SIZE")
+    cw, _ = per_launch(cwrite_dir, "copy", "WRITE_SIZE")
+    copy_bytes = float(1 << 30)
+    f_scale = copy_bytes / (cf * 1024.0)
+    w_scale = copy_bytes / (cw * 1024.0)
+    return {"kernel": kernel, "launches": [n1, n2], "fetch_kib_per_launch": fk, "write_kib_per_launch": wk,
+            "copy_fetch_kib": cf, "copy_write_kib": cw, "fetch_scale": f_scale, "write_scale": w_scale,
+            "bytes_per_launch": fk * 1024.0 * f_scale + wk * 1024.0 * w_scale}
 
 
 def cpu_baseline(w, records, stride, threads):
@@ -53,7 +99,20 @@ def main():
     ap.add_argument("--batch", type=int, default=0, help="orientations in flight (0 = auto)")
     ap.add_argument("--cpu-stride", type=int, default=11)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a 1 GiB device copy (for scaling rocprof FETCH_SIZE / WRITE_SIZE) and exit")
+    ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
+                    help="rocprofv3 --pmc output folders: write the traffic summary under profiles/ and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
+    if args.summarize_pmc:
+        summ = summarize_pmc(*args.summarize_pmc)
+        with open(os.path.join(ROOT, "profiles", "raster_kernel_dsphere69120_traffic.json"), "w") as fh:
+            json.dump(summ, fh, indent=1)
+        print(json.dumps(summ))
+        return
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported; the sweep runs on exactly one GPU "
               f"(orientation sharding across GPUs is not built)", file=sys.stderr)
@@ -84,9 +143,19 @@ def main():
     value = n * args.steps / dt
     b_orient, p_bar, s_bar = algorithmic_bytes(len(w.vtx), len(w.tri), rec)
     k_ms = float(np.mean(kernel_ms))
-    achieved = b_orient * n / (k_ms * 1e-3) / 1e9
+    # per-stage times from HIP events between the kernels (an untimed pass)
+    s.set_phase_profile(True)
+    s.run()
+    s.sync()
+    stages = s.phase_profile()
+    s.set_phase_profile(False)
+    batch = info["batch"]
+    launches = (n + batch - 1) // batch
+    raster_ms_launch = stages["raster"] / launches
+    rb = raster_bytes(len(w.vtx), len(w.tri), p_bar, s_bar)
+    achieved = rb * batch / (raster_ms_launch * 1e-3) / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "sweep_kernel_dsphere69120_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "raster_kernel_dsphere69120_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
@@ -108,8 +177,15 @@ def main():
                    "support": "explicit", "note": w.scale_note, "batch": info["batch"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                     "kernel_ms": round(k_ms, 3), "algorithmic_bytes_per_orientation": round(b_orient),
-                     "mean_entries": round(p_bar, 1), "mean_slots": round(s_bar, 1)},
+                     "kernel": "raster_kernel", "launch_orientations": batch,
+                     "kernel_ms_per_launch": round(raster_ms_launch, 3),
+                     "algorithmic_bytes_per_launch": round(rb * batch),
+                     "algorithmic_bytes_per_orientation": round(rb),
+                     "mean_entries": round(p_bar, 1), "mean_slots": round(s_bar, 1),
+                     "stage_ms_per_sweep": {k: round(v, 3) for k, v in stages.items()},
+                     "sweep_kernels_ms": round(k_ms, 3),
+                     "sweep_bytes_per_orientation": round(b_orient),
+                     "sweep_achieved_gbs": round(b_orient * n / (k_ms * 1e-3) / 1e9, 1)},
         "flags": {"cap_drop_orientations": int(((rec["flags"] & 8) != 0).sum()),
                   "fp32_inexact": int(((rec["flags"] & 1) != 0).sum()),
                   "footprint_oob": int(((rec["flags"] & 4) != 0).sum()),
