@@ -121,9 +121,14 @@ int tomo_set_launch(tomo_handle *h, int32_t batch);
 
 /* Launch-shape knobs for tuning (values take effect at the next tomo_run):
  * TOMO_TUNE_BATCH           orientations in flight (0 = automatic), same as tomo_set_launch;
- * TOMO_TUNE_TRI_PER_THREAD  triangles per rasteriser thread (1..64, default 4). */
+ * TOMO_TUNE_TRI_PER_THREAD  triangles per rasteriser thread (1..64, default 4);
+ * TOMO_TUNE_EXACT_FSUM      1 (default): where a float total can round (TOMO_F_FP32_INEXACT),
+ *                           replay the reference's ascending-slot float accumulation so fVo,
+ *                           fVss, fVa, fVb, fVnv, fVtc and the masses equal the reference's;
+ *                           0: report float(integer total) instead (cheaper, may differ there). */
 #define TOMO_TUNE_BATCH 0
 #define TOMO_TUNE_TRI_PER_THREAD 1
+#define TOMO_TUNE_EXACT_FSUM 2
 int tomo_set_tuning(tomo_handle *h, int32_t knob, int32_t value);
 
 /* Duration in milliseconds of the last tomo_run's kernel, from HIP events on the handle's
@@ -132,7 +137,8 @@ float tomo_last_kernel_ms(tomo_handle *h);
 
 /* Enable (1) or disable (0) per-stage timing (HIP events between kernels) in later tomo_run
  * calls, and read the last run's milliseconds summed over batches for: [0] rotated-AABB
- * reduction, [1] rasterisation, [2] overflow compaction, [3] pairing, [4] bed; [5..7] = 0. */
+ * reduction, [1] rasterisation, [2] overflow compaction, [3] pairing, [4] bed, [5] ordered float
+ * totals; [6..7] = 0. */
 int tomo_set_phase_profile(tomo_handle *h, int32_t enable);
 int tomo_phase_profile(tomo_handle *h, double *ms8);
 

@@ -79,11 +79,10 @@ def compare_records(gpu, ref, label=""):
     if fl.size:
         i = fl[0]
         bad.append(f"{label}flags: {fl.size} differ, first #{i}: gpu {gpu['flags'][i]} ref {ref['flags'][i]}")
-    exact = (ref["flags"] & 1) == 0
     for k in FLOAT_FIELDS:
         g = gpu[k].view(np.uint32)
         r = ref[k].view(np.uint32)
-        d = np.nonzero((g != r) & exact)[0]
+        d = np.nonzero(g != r)[0]
         if d.size:
             i = d[0]
             bad.append(f"{label}{k}: {d.size} differ, first #{i}: gpu {gpu[k][i]!r} ref {ref[k][i]!r}")

@@ -1,5 +1,6 @@
 """GPU parity: libtomo_hip.so against the CPU oracle, bit-exact on integer counts and on the
-reference's float totals / masses wherever the float total is exact.
+reference's float totals and masses (including orientations whose float running totals round,
+which the library replays in the reference's slot order).
 
 Every case prints its name before touching the GPU so a failing run names it.
 """
@@ -67,6 +68,32 @@ def test_displaced_sphere_slice(oracle_mod):
     g, r = assert_parity(oracle_mod, workload_case(w, w.ypr[idx]), "dsphere69120 every 61st orientation")
     # saturated slots occur in this workload: the 63-row cap order is exercised
     assert ((r["flags"] & 8) != 0).any()
+
+
+def test_manikin_shell_sample(oracle_mod):
+    """Open thin-shell manikin (inner and outer sheets), shell mode, 512 x 512 grid: every
+    457th orientation of the 1 degree grid.  Float totals exceed 2^24 here, so the ordered float
+    replay is exercised on every orientation."""
+    w = synth.manikin_workload()
+    idx = np.arange(0, len(w.ypr), 457)
+    g, r = assert_parity(oracle_mod, workload_case(w, w.ypr[idx]), f"{w.name} every 457th orientation")
+    assert ((r["flags"] & 1) != 0).all()   # FP32_INEXACT: the float replay decides fVo/fVss/Mss
+    assert ((r["flags"] & 4) == 0).all()   # footprint fits the grid
+
+
+def test_manikin_per_slot(oracle_mod):
+    w = synth.manikin_workload()
+    announce("per-slot dumps manikin shell")
+    with _lib.Sweep(w.f32, w.i32, w.grid) as s:
+        s.set_mesh(w.tri, w.vtx, w.vnrm)
+        for i in (0, 12345):
+            rec, pre, fin, vo, vss = s.debug_slots(w.ypr[i])
+            rr, rpre, rfin, rvo, rvss = oracle_mod.one(w.f32, w.i32, w.grid, w.tri, w.vtx, w.vnrm, w.ypr[i])
+            np.testing.assert_array_equal(pre, rpre)
+            np.testing.assert_array_equal(fin, rfin)
+            np.testing.assert_array_equal(vo, rvo)
+            np.testing.assert_array_equal(vss, rvss)
+            assert rec["fVss"] == rr["fVss"] and rec["Mss"] == rr["Mss"]
 
 
 def test_per_slot_dumps(oracle_mod):

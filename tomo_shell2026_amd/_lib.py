@@ -147,7 +147,7 @@ class Sweep:
     def set_launch(self, n_groups):
         check(load().tomo_set_launch(self._h, int(n_groups)), "tomo_set_launch")
 
-    TUNE_BATCH, TUNE_TRI_PER_THREAD = 0, 1
+    TUNE_BATCH, TUNE_TRI_PER_THREAD, TUNE_EXACT_FSUM = 0, 1, 2
 
     def set_tuning(self, knob, value):
         check(load().tomo_set_tuning(self._h, int(knob), int(value)), "tomo_set_tuning")
@@ -180,7 +180,7 @@ class Sweep:
         check(load().tomo_get_info(self._h, ct.byref(g), ct.byref(n), ct.byref(b)), "tomo_get_info")
         return {"grid": g.value, "batch": n.value, "scratch_bytes": b.value}
 
-    PHASES = ("aabb", "raster", "compact", "pair", "bed")
+    PHASES = ("aabb", "raster", "compact", "pair", "bed", "fsum")
 
     def set_phase_profile(self, enable=True):
         check(load().tomo_set_phase_profile(self._h, int(bool(enable))), "tomo_set_phase_profile")

@@ -150,6 +150,7 @@ ScratchLayout make_layout(int32_t G) {
   L.pool_rank = take(4 * (int64_t)pool);
   L.cmp_key = take(4 * (int64_t)pool);
   L.cmp_rank = take(4 * (int64_t)pool);
+  L.sv = take(4 * 6 * G2);
   L.bytes = at;
   L.pool = pool;
   return L;
@@ -191,6 +192,7 @@ struct tomo_handle {
   int32_t scratch_batch = 0;
   int32_t batch_req = 0;
   int32_t tri_per_thread = kTriPerThread;
+  int32_t exact_fsum = 1;
   int32_t last_batch = 0;
   int32_t last_begin = 0, last_end = 0;
   std::vector<float> vbed_seq;
@@ -274,6 +276,10 @@ void finish_record(const tomo_handle *h, const DevRecord &d, const float *R, tom
   // equal the integers (Pairing :386-392 and Calculate :1233-1240 accumulate ints into floats).
   r.fVa = (float)d.Va; r.fVb = (float)d.Vb; r.fVtc = (float)d.Vtc; r.fVnv = (float)d.Vnv;
   r.fVss = (float)d.Vss; r.fVo = (float)d.Vo;
+  if (d.fsum_done) {  // replayed on the device in the reference's accumulation order
+    r.fVo = d.fsum[0]; r.fVss = d.fsum[1]; r.fVa = d.fsum[2]; r.fVb = d.fsum[3];
+    r.fVnv = d.fsum[4]; r.fVtc = d.fsum[5];
+  }
   const size_t nb = (size_t)(d.Vbed < 0 ? 0 : d.Vbed);
   r.fVbed = nb < h->vbed_seq.size() ? h->vbed_seq[nb] : (float)nb * h->opt.bed_thick;
   float vm[22] = {};
@@ -335,6 +341,7 @@ SweepParams base_params(tomo_handle *h) {
   p.tri_groups = (h->n_tri + per_group - 1) / per_group;
   p.pair_tiles = (g2 + kPairTile - 1) / kPairTile;
   p.bed_bands = (h->opt.grid + kBandRows - 1) / kBandRows;
+  p.exact_fsum = h->exact_fsum;
   return p;
 }
 
@@ -354,7 +361,7 @@ int enqueue(tomo_handle *h, int32_t begin, int32_t end, int32_t batch, int32_t *
     }
     p.dbg_rows = dbg_rows;
     if (p.tri_groups == 0) p.tri_groups = 1;
-    for (int stage = 0; stage < 5; stage++) {
+    for (int stage = 0; stage < 6; stage++) {
       if (h->phases) {
         hipEvent_t e;
         HIP_TRY(hipEventCreate(&e));
@@ -512,6 +519,10 @@ int tomo_set_tuning(tomo_handle *h, int32_t knob, int32_t value) {
     case TOMO_TUNE_TRI_PER_THREAD:
       if (value < 1 || value > 64) return fail(TOMO_E_INVALID_ARG, "triangles per thread must be 1..64");
       h->tri_per_thread = value;
+      return TOMO_OK;
+    case TOMO_TUNE_EXACT_FSUM:
+      if (value != 0 && value != 1) return fail(TOMO_E_INVALID_ARG, "exact float totals must be 0 or 1");
+      h->exact_fsum = value;
       return TOMO_OK;
     default:
       return fail(TOMO_E_INVALID_ARG, "unknown tuning knob");

@@ -29,7 +29,9 @@ struct DevRecord {
   int64_t aVo, aVss, aVa, aVb, aVnv, aVtc;  // sums of |per-slot value| (float exactness test)
   int32_t flags, n_bed;
   float kx, ky, kz;                          // placement offsets (rotated AABB min - Bo, - Bo, 0)
-  int32_t pad;
+  int32_t fsum_done;                         // fsum[] holds the reference's float running totals
+  float fsum[6];                             // Vo, Vss, Va, Vb, Vnv, Vtc summed in slot order
+  int32_t pad[2];
 };
 
 // Per in-flight orientation bookkeeping (reset by the AABB kernel).
@@ -39,13 +41,14 @@ struct OrientHdr {
   uint32_t pool_top;     // entries beyond kInline
   int32_t flags;
   float kx, ky, kz;
-  int32_t pad[6];
+  unsigned long long zabs;  // sum of |z| over emitted entries: bounds every sum of |slot value|
+  int32_t pad[4];
 };
 
 // Per in-flight orientation scratch, carved from one device allocation.  G2 = grid * grid.
 struct ScratchLayout {
   int64_t count, ent, bedinfo, big, off, cursor, pool_slot, pool_key, pool_rank, cmp_key,
-      cmp_rank;  // byte offsets inside one orientation's region
+      cmp_rank, sv;  // byte offsets inside one orientation's region
   int64_t bytes; // bytes per region
   int32_t pool;  // overflow pool capacity (entries)
 };
@@ -79,6 +82,7 @@ struct SweepParams {
   // optional per-slot dumps for a one-orientation batch (grid*grid each, zero-filled by host)
   int32_t *dbg_pre, *dbg_final, *dbg_vo, *dbg_vss;
   uint64_t *dbg_rows;      // [grid*grid][64] final rows (type << 32 | key) for the render export
+  int32_t exact_fsum;      // keep per-slot values and rebuild float totals when they round
 };
 
 }  // namespace tomo

@@ -61,6 +61,7 @@ struct Tables {
   uint32_t *count, *bedinfo, *off, *cursor, *pool_slot, *pool_key, *pool_rank,
       *cmp_key, *cmp_rank;
   int32_t *big;
+  int32_t *sv;  // [6][G2] per-slot Vo, Vss, Va, Vb, Vnv, Vtc (kept only when totals may round)
   int32_t pool;
 };
 
@@ -72,6 +73,23 @@ struct Acc {
 };
 
 __device__ __forceinline__ int64_t iabs64(int32_t v) { return v < 0 ? -(int64_t)v : (int64_t)v; }
+
+// The six per-slot totals of one slot, kept for the ordered float re-summation.
+__device__ __forceinline__ void put_sv(int32_t *sv, int32_t G2, int32_t s, const SlotResult &r) {
+  sv[s] = r.vo;
+  sv[G2 + s] = r.vss;
+  sv[2 * G2 + s] = r.va;
+  sv[3 * G2 + s] = r.vb;
+  sv[4 * G2 + s] = r.vnv;
+  sv[5 * G2 + s] = r.vtc;
+}
+
+// Every float total of an orientation is exact when the sum of |per-slot value| stays within
+// 2^24.  zabs (sum of |z| over all emitted entries, a superset of the distinct rows) bounds
+// those sums; the implicit Vss (-alpha + beta + TC - NVB + NVA) can count a row twice.
+__device__ __forceinline__ bool totals_may_round(const OrientHdr &h, const PairParams &pp) {
+  return (pp.explicit_ss ? h.zabs : 2ull * h.zabs) > (1ull << 24);
+}
 
 __device__ __forceinline__ void account(Acc &a, const SlotResult &r, int32_t s, const SweepParams &p) {
   a.Va += r.va; a.Vb += r.vb; a.Vo += r.vo; a.Vss += r.vss; a.Vnv += r.vnv; a.Vtc += r.vtc;
@@ -101,6 +119,7 @@ struct Emit {
   uint32_t *pool_top, *big_top;  // this orientation's counters (global)
   int32_t bx0, bx1, by0, by1;
   int32_t flags;
+  uint64_t zabs;  // sum of |z| of this lane's entries
 
   // SetBit's slot addressing (coord2vID, STomoVoxelSpaceInfo.cpp:97-105) + append to the slot.
   __device__ __forceinline__ int32_t put(float px, float py, float pz, float nz, uint32_t rank) {
@@ -112,6 +131,7 @@ struct Emit {
     }
     int32_t z16 = to_i16(pz, flags);
     int32_t nz16 = to_i16(nz * kNormFactor, flags);
+    zabs += (uint64_t)(z16 < 0 ? -z16 : z16);
     uint32_t key = make_key(z16, nz16);
     int32_t s = sx * G + sy;
     bx0 = min(bx0, sx); bx1 = max(bx1, sx);
@@ -368,6 +388,7 @@ __device__ __forceinline__ Tables tables_for(const SweepParams &p, int32_t b) {
   t.pool_rank = (uint32_t *)(base + p.lay.pool_rank);
   t.cmp_key = (uint32_t *)(base + p.lay.cmp_key);
   t.cmp_rank = (uint32_t *)(base + p.lay.cmp_rank);
+  t.sv = (int32_t *)(base + p.lay.sv);
   t.pool = p.lay.pool;
   return t;
 }
@@ -455,6 +476,7 @@ __global__ __launch_bounds__(kThreads) void aabb_kernel(SweepParams p) {
     h.bbox[0] = p.grid; h.bbox[1] = -1; h.bbox[2] = p.grid; h.bbox[3] = -1;
     h.big_top = 0u; h.pool_top = 0u; h.flags = 0;
     h.kx = kx; h.ky = ky; h.kz = kz;
+    h.zabs = 0ull;
     DevRecord &R = p.rec[b];
     R = DevRecord{};
     R.kx = kx; R.ky = ky; R.kz = kz;
@@ -472,6 +494,7 @@ struct RasterLds {
   float qnv[kWaves][128], qnw[kWaves][128];
   int32_t hits[kWaves][64];
   int32_t red[kWaves][5];
+  uint64_t zred[kWaves];
 };
 
 __global__ __launch_bounds__(kThreads) void raster_kernel(SweepParams p) {
@@ -486,7 +509,7 @@ __global__ __launch_bounds__(kThreads) void raster_kernel(SweepParams p) {
   Emit em;
   em.G = G; em.G2 = G * G; em.t = tables_for(p, b);
   em.pool_top = &h.pool_top; em.big_top = &h.big_top;
-  em.bx0 = G; em.bx1 = -1; em.by0 = G; em.by1 = -1; em.flags = 0;
+  em.bx0 = G; em.bx1 = -1; em.by0 = G; em.by1 = -1; em.flags = 0; em.zabs = 0ull;
   const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
   const int32_t first = chunk * p.tri_per_thread * kThreads;
   float (*tri)[13] = L.tri[wave];
@@ -630,9 +653,11 @@ __global__ __launch_bounds__(kThreads) void raster_kernel(SweepParams p) {
     em.by1 = max(em.by1, __shfl_xor(em.by1, off));
     em.flags |= __shfl_xor(em.flags, off);
   }
+  const uint64_t zw = wave_sum(em.zabs);
   if (lane == 0) {
     L.red[wave][0] = em.bx0; L.red[wave][1] = em.bx1; L.red[wave][2] = em.by0; L.red[wave][3] = em.by1;
     L.red[wave][4] = em.flags;
+    L.zred[wave] = zw;
   }
   __syncthreads();
   if (tid == 0) {
@@ -647,6 +672,9 @@ __global__ __launch_bounds__(kThreads) void raster_kernel(SweepParams p) {
       atomicMin(&h.bbox[2], v2); atomicMax(&h.bbox[3], v3);
     }
     if (f) atomicOr(&h.flags, f);
+    uint64_t z = 0;
+    for (int w = 0; w < kWaves; w++) z += L.zred[w];
+    if (z) atomicAdd(&h.zabs, (unsigned long long)z);
   }
 }
 
@@ -730,6 +758,7 @@ __global__ __launch_bounds__(kThreads) void pair_kernel(SweepParams p) {
   if (i0 >= nbb && (uint32_t)(tile * kWaves) >= nbig) return;  // nothing for this group
   const Tables t = tables_for(p, b);
   const bool final_here = !p.bed_on;  // without a bed the final row count is known here
+  int32_t *const sv = (p.exact_fsum && totals_may_round(h, p.pp)) ? t.sv : nullptr;
   Acc acc = {};
   const int32_t i1 = min(i0 + kPairTile, nbb);
   for (int32_t i = i0 + tid; i < i1; i += kThreads) {
@@ -738,6 +767,10 @@ __global__ __launch_bounds__(kThreads) void pair_kernel(SweepParams p) {
     const uint32_t c = t.count[s];
     if (c == 0u) {
       t.bedinfo[s] = BI_POSS;
+      if (sv) {
+        const SlotResult z = {};
+        put_sv(sv, G2, s, z);
+      }
       continue;
     }
     if (c > (uint32_t)kSmall) continue;  // a wave pairs it below
@@ -750,6 +783,7 @@ __global__ __launch_bounds__(kThreads) void pair_kernel(SweepParams p) {
     pair_keys<kSmall>(k, (int)c, p.pp, r, p.dbg_rows ? p.dbg_rows + (int64_t)s * 64 : nullptr);
     account(acc, r, s, p);
     t.bedinfo[s] = bed_info(r, p.pp);
+    if (sv) put_sv(sv, G2, s, r);
     if (final_here) {
       acc.nfinal += r.n;
       acc.chkfinal += (uint64_t)(s + 1) * (uint64_t)r.n;
@@ -770,6 +804,7 @@ __global__ __launch_bounds__(kThreads) void pair_kernel(SweepParams p) {
       acc.nslot++;
       account(acc, r, s, p);
       t.bedinfo[s] = bed_info(r, p.pp);
+      if (sv) put_sv(sv, G2, s, r);
       if (final_here) {
         acc.nfinal += r.n;
         acc.chkfinal += (uint64_t)(s + 1) * (uint64_t)r.n;
@@ -903,6 +938,42 @@ __global__ __launch_bounds__(kThreads) void bed_kernel(SweepParams p) {
 }
 
 // Stage k of a batch: 0 aabb, 1 raster, 2 compact, 3 pair, 4 bed (skipped without a bed).
+// ---- fsum_kernel: one 64-lane work-group per orientation ----------------------------------
+// The reference accumulates per-slot integer totals into float running sums in ascending slot
+// order: Vo and Vss in Pairing (STomoSh_INT3.cpp:386-392), Va, Vb, Vnv and Vtc in Calculate
+// (:1233-1240).  Whenever the sum of |slot value| stays within 2^24 every partial sum is an
+// exactly representable integer and float(total) is that result; otherwise this kernel replays
+// the sums in the same order from the per-slot values pair_kernel kept.  Lane q < 6 owns total q.
+__global__ __launch_bounds__(64) void fsum_kernel(SweepParams p) {
+  __shared__ int32_t vals[6][64];
+  const int lane = threadIdx.x;
+  const int32_t b = blockIdx.x;
+  const OrientHdr &h = p.hdr[b];
+  DevRecord &R = p.rec[b];
+  const int64_t lim = (int64_t)1 << 24;
+  if (!totals_may_round(h, p.pp)) return;
+  if (!(R.aVo > lim || R.aVss > lim || R.aVa > lim || R.aVb > lim || R.aVnv > lim || R.aVtc > lim)) return;
+  const Tables t = tables_for(p, b);
+  const int32_t G = p.grid, G2 = G * G;
+  const int32_t bx0 = h.bbox[0], bx1 = h.bbox[1], by0 = h.bbox[2], by1 = h.bbox[3];
+  const int q = lane < 6 ? lane : 0;
+  float acc = 0.0f;
+  for (int32_t sx = bx0; sx <= bx1; sx++) {
+    for (int32_t y0 = by0; y0 <= by1; y0 += 64) {
+      const int32_t n = min(64, by1 - y0 + 1);
+      const int64_t s0 = (int64_t)sx * G + y0;
+#pragma unroll
+      for (int k = 0; k < 6; k++) vals[k][lane] = lane < n ? t.sv[(int64_t)k * G2 + s0 + lane] : 0;
+      __syncthreads();
+      if (lane < 6)
+        for (int i = 0; i < n; i++) acc = acc + (float)vals[q][i];
+      __syncthreads();
+    }
+  }
+  if (lane < 6) R.fsum[lane] = acc;
+  if (lane == 0) R.fsum_done = 1;
+}
+
 hipError_t launch_stage(int stage, const SweepParams &p, hipStream_t stream) {
   const int32_t nb = p.n_batch;
   const int32_t nb8 = (nb + 7) / 8 * 8;  // xcd_map pads the orientation count to whole XCD rounds
@@ -913,6 +984,9 @@ hipError_t launch_stage(int stage, const SweepParams &p, hipStream_t stream) {
     case 3: hipLaunchKernelGGL(pair_kernel, dim3(nb8 * p.pair_tiles), dim3(kThreads), 0, stream, p); break;
     case 4:
       if (p.bed_on) hipLaunchKernelGGL(bed_kernel, dim3(nb8 * p.bed_bands), dim3(kThreads), 0, stream, p);
+      break;
+    case 5:
+      if (p.exact_fsum) hipLaunchKernelGGL(fsum_kernel, dim3(nb), dim3(64), 0, stream, p);
       break;
     default: return hipErrorInvalidValue;
   }

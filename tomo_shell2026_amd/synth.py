@@ -179,6 +179,17 @@ def icosphere_workload(bed_type=BED_RAFT):
     return Workload("icosphere20480_r50_h1_g128_10deg", v, f, 128, ypr, 1.0, bed_type=bed_type)
 
 
+def manikin_workload(bed_type=BED_RAFT, seed=7, yaw_step=1, pitch_max=60):
+    """~87k-triangle open thin-shell manikin (inner and outer sheets), 1 mm voxels, 512^2 grid,
+    shell mode, yaw 0..359 x pitch 0..60 in 1 degree steps."""
+    v, f = manikin_shell(seed=seed)
+    v, s = fit_to_grid(v, 512)
+    note = "" if s == 1.0 else f"scaled by {s:.4f} to fit the 512 grid"
+    ypr = ypr_grid(np.arange(0, 360, yaw_step), np.arange(0, pitch_max + 1, 1))
+    return Workload(f"manikin{len(f)}_shell_h1_g512_1deg", v, f, 512, ypr, 1.0, note, bed_type=bed_type,
+                    shell=True, shell_um=500)
+
+
 def displaced_sphere_workload(bed_type=BED_RAFT, seed=1, yaw_step=1, pitch_max=60):
     """69,120-triangle displaced sphere, 0.5 mm voxels, 256^2 grid, yaw 0..359 x pitch 0..60."""
     v, f = displaced_latlong_sphere(seed=seed)
@@ -187,3 +198,169 @@ def displaced_sphere_workload(bed_type=BED_RAFT, seed=1, yaw_step=1, pitch_max=6
     note = "" if s == 1.0 else f"scaled by {s:.4f} to fit the 256 grid"
     ypr = ypr_grid(np.arange(0, 360, yaw_step), np.arange(0, pitch_max + 1, 1))
     return Workload("dsphere69120_h0.5_g256_1deg", v, f, 256, ypr, 0.5, note, bed_type=bed_type)
+
+
+# ---------------------------------------------------------------- thin-shell manikin
+def _sd_ellipsoid(p, c, semi):
+    """Approximate signed distance to an axis-aligned ellipsoid (gradient-normalised bound)."""
+    q = (p - c) / semi
+    k0 = np.linalg.norm(q, axis=-1)
+    k1 = np.linalg.norm(q / semi, axis=-1)
+    return k0 * (k0 - 1.0) / np.maximum(k1, 1e-12)
+
+
+def _sd_capsule(p, a, b, r):
+    pa, ba = p - a, b - a
+    h = np.clip((pa @ ba) / (ba @ ba), 0.0, 1.0)
+    return np.linalg.norm(pa - h[..., None] * ba, axis=-1) - r
+
+
+def _sd_sphere(p, c, r):
+    return np.linalg.norm(p - c, axis=-1) - r
+
+
+def manikin_parts(seed=7):
+    """Seeded manikin primitives (mm): torso ellipsoid with full axes 170 x 120 x 300, head sphere
+    r = 90, two arm and two leg capsules with radii drawn uniformly in [35, 50]."""
+    rng = np.random.default_rng(seed)
+    r = rng.uniform(35.0, 50.0, size=4)
+    j = rng.normal(0.0, 4.0, size=(4, 2))
+    torso = ("ellipsoid", np.array([0.0, 0.0, 0.0]), np.array([85.0, 60.0, 150.0]))
+    head = ("sphere", np.array([0.0, 0.0, 210.0]), 90.0)
+    arms = [("capsule", np.array([s * 95.0, 0.0, 95.0]), np.array([s * (112.0 + j[i, 0]), j[i, 1], -55.0]), r[i])
+            for i, s in enumerate((-1.0, 1.0))]
+    legs = [("capsule", np.array([s * 42.0, 0.0, -110.0]), np.array([s * (50.0 + j[2 + i, 0]), j[2 + i, 1], -140.0]),
+             r[2 + i]) for i, s in enumerate((-1.0, 1.0))]
+    return [torso, head] + arms + legs
+
+
+def manikin_sdf(p, parts):
+    d = None
+    for part in parts:
+        if part[0] == "ellipsoid":
+            e = _sd_ellipsoid(p, part[1], part[2])
+        elif part[0] == "sphere":
+            e = _sd_sphere(p, part[1], part[2])
+        else:
+            e = _sd_capsule(p, part[1], part[2], part[3])
+        d = e if d is None else np.minimum(d, e)
+    return d
+
+
+# Freudenthal split of a cube into 6 tetrahedra sharing the diagonal (0,0,0)-(1,1,1); neighbouring
+# cubes split their shared faces along the same diagonal, so the extracted surface is closed.
+_CUBE = np.array([[i, j, k] for i in (0, 1) for j in (0, 1) for k in (0, 1)])  # corner c = 4i + 2j + k
+_TETS = []
+for _perm in ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)):
+    _a = np.zeros(3, int)
+    _t = [0]
+    for _ax in _perm[:2]:
+        _a[_ax] = 1
+        _t.append(int(4 * _a[0] + 2 * _a[1] + _a[2]))
+    _t.append(7)
+    _TETS.append(_t)
+_TETS = np.array(_TETS)
+
+
+def marching_tetrahedra(F, origin, h, level=0.0):
+    """Closed triangle mesh of the surface F = level over a regular grid (F sampled at
+    origin + h * (i, j, k)).  Faces are oriented outward, towards increasing F."""
+    nx, ny, nz = F.shape
+    gid = lambda i, j, k: (i * ny + j) * nz + k  # noqa: E731
+    ii, jj, kk = np.meshgrid(np.arange(nx - 1), np.arange(ny - 1), np.arange(nz - 1), indexing="ij")
+    ii, jj, kk = ii.ravel(), jj.ravel(), kk.ravel()
+    Fc = np.stack([F[ii + c[0], jj + c[1], kk + c[2]] for c in _CUBE], axis=1) - level
+    Fc = np.where(Fc == 0.0, 1e-6 * h, Fc)  # no vertex exactly on a grid point: keeps faces non-degenerate
+    keep = (Fc.min(1) < 0) & (Fc.max(1) > 0)
+    ii, jj, kk, Fc = ii[keep], jj[keep], kk[keep], Fc[keep]
+    corner_id = np.stack([gid(ii + c[0], jj + c[1], kk + c[2]) for c in _CUBE], axis=1)
+    corner_pos = np.stack([np.stack([ii + c[0], jj + c[1], kk + c[2]], 1) for c in _CUBE], axis=1)
+    tris, keys, ts, pa, pb, ins = [], [], [], [], [], []
+    for tet in _TETS:
+        v = Fc[:, tet]                      # (n, 4)
+        ids = corner_id[:, tet]
+        pos = corner_pos[:, tet].astype(np.float64)
+        inside = v < 0
+        cnt = inside.sum(1)
+        for want in (1, 2, 3):
+            sel = np.nonzero(cnt == want)[0]
+            if sel.size == 0:
+                continue
+            vin = inside[sel]
+            # order corners: inside ones first (stable), so the crossing edges are known
+            order = np.argsort(~vin, axis=1, kind="stable")
+            vv = np.take_along_axis(v[sel], order, 1)
+            ii4 = np.take_along_axis(ids[sel], order, 1)
+            pp = np.take_along_axis(pos[sel], order[:, :, None], 1)
+            if want == 1:
+                edges = [[(0, 1), (0, 2), (0, 3)]]
+            elif want == 3:
+                edges = [[(0, 3), (1, 3), (2, 3)]]
+            else:
+                edges = [[(0, 2), (0, 3), (1, 3)], [(0, 2), (1, 3), (1, 2)]]
+            for tri_edges in edges:
+                corners = []
+                for (a, b) in tri_edges:
+                    fa, fb = vv[:, a], vv[:, b]
+                    t = fa / (fa - fb)
+                    ka = np.minimum(ii4[:, a], ii4[:, b])
+                    kb = np.maximum(ii4[:, a], ii4[:, b])
+                    corners.append((ka, kb, pp[:, a] + t[:, None] * (pp[:, b] - pp[:, a])))
+                keys.append(np.stack([np.stack([c[0], c[1]], 1) for c in corners], 1))  # (m, 3, 2)
+                ts.append(np.stack([c[2] for c in corners], 1))                             # (m, 3, 3)
+                ins.append(pp[:, 0])                                                        # an inside corner
+    keys = np.concatenate(keys)
+    pts = np.concatenate(ts)
+    pin = np.concatenate(ins)
+    flat = keys.reshape(-1, 2)
+    uniq, inv = np.unique(flat, axis=0, return_inverse=True)
+    inv = inv.reshape(-1)
+    verts = np.zeros((len(uniq), 3))
+    verts[inv] = pts.reshape(-1, 3)
+    faces = inv.reshape(-1, 3)
+    # orient outward: the normal points away from the tetrahedron's inside corner
+    a, b, c = verts[faces[:, 0]], verts[faces[:, 1]], verts[faces[:, 2]]
+    n = np.cross(b - a, c - a)
+    flip = np.einsum("ij,ij->i", n, (a + b + c) / 3.0 - pin) < 0
+    faces[flip] = faces[flip][:, [0, 2, 1]]
+    return origin + h * verts, faces
+
+
+def _compact(v, f):
+    used = np.unique(f)
+    remap = -np.ones(len(v), np.int64)
+    remap[used] = np.arange(len(used))
+    return v[used], remap[f]
+
+
+def manikin_shell(seed=7, spacing=None, wall=2.0, open_mm=15.0, target_tris=87000):
+    """Open thin-shell manikin (mm): the outer surface of the primitive union (marching
+    tetrahedra), plus the same surface offset inward by `wall` with reversed faces, both opened
+    by removing every face below `open_mm` above the lowest point (open leg ends).  The grid
+    spacing is the given one, else the one whose triangle count is closest to target_tris."""
+    parts = manikin_parts(seed)
+
+    def build(hh):
+        lo = np.array([-175.0, -100.0, -200.0]) - 2 * hh
+        hi = np.array([175.0, 100.0, 305.0]) + 2 * hh
+        n = np.ceil((hi - lo) / hh).astype(int) + 1
+        g = [lo[d] + hh * np.arange(n[d]) for d in range(3)]
+        P = np.stack(np.meshgrid(*g, indexing="ij"), -1)
+        F = manikin_sdf(P, parts)
+        vo, fo = marching_tetrahedra(F, lo, hh, 0.0)
+        vi, fi = marching_tetrahedra(F, lo, hh, -wall)
+        fi = fi[:, [0, 2, 1]] + len(vo)
+        v = np.concatenate([vo, vi])
+        f = np.concatenate([fo, fi])
+        zc = v[f].mean(1)[:, 2]
+        f = f[zc > v[:, 2].min() + open_mm]
+        return _compact(v, f)
+
+    if spacing is not None:
+        return build(spacing)
+    best = None
+    for hh in (9.0, 8.5, 8.0, 7.6, 7.3, 7.0):
+        v, f = build(hh)
+        if best is None or abs(len(f) - target_tris) < abs(len(best[1]) - target_tris):
+            best = (v, f, hh)
+    return best[0], best[1]
