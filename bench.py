@@ -91,6 +91,22 @@ def cpu_baseline(w, records, stride, threads):
             "matches_gpu_on_sample": same}
 
 
+def refine_comparison(w):
+    """Least voxel Vss on the dense 0.1 degree yaw x pitch grid (0..359.9 x 0..60) by the plain
+    sweep, against the coarse 5 degree sweep + batched gradient refinement of the continuous
+    support volume + voxel judging on the 0.1 degree grid."""
+    from tomo_shell2026_amd.refine import OrientationSearch
+    srch = OrientationSearch(w.f32, w.i32, w.grid, w.tri, w.vtx, w.vnrm)
+    srch.refined()  # warm-up (kernel loads, torch initialisation)
+    ref = srch.refined()
+    dense = srch.dense(0.1)
+    srch.close()
+    return {"dense_grid_deg": 0.1, "dense": dense, "refined": ref,
+            "vss_gap_rel": round((ref["vss"] - dense["vss"]) / dense["vss"], 6),
+            "wall_speedup": round(dense["wall_s"] / ref["wall_s"], 1),
+            "kernel_speedup": round(dense["kernel_ms"] / ref["kernel_ms"], 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,6 +115,11 @@ def main():
     ap.add_argument("--batch", type=int, default=0, help="orientations in flight (0 = auto)")
     ap.add_argument("--cpu-stride", type=int, default=11)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=("dsphere", "manikin"), default="dsphere",
+                    help="dsphere: 69,120-triangle displaced sphere, 256 grid (the headline workload); "
+                         "manikin: ~87k-triangle thin-shell manikin, 512 grid, shell mode")
+    ap.add_argument("--no-refine", action="store_true",
+                    help="skip the dense 0.1 degree grid vs coarse + gradient-refined search comparison")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 1 GiB device copy (for scaling rocprof FETCH_SIZE / WRITE_SIZE) and exit")
     ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
@@ -119,7 +140,7 @@ def main():
         sys.exit(2)
 
     from tomo_shell2026_amd import _lib, synth
-    w = synth.displaced_sphere_workload()
+    w = synth.manikin_workload() if args.workload == "manikin" else synth.displaced_sphere_workload()
     n = len(w.ypr)
     s = _lib.Sweep(w.f32, w.i32, w.grid)
     s.set_mesh(w.tri, w.vtx, w.vnrm)
@@ -156,7 +177,7 @@ def main():
     achieved = rb * batch / (raster_ms_launch * 1e-3) / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "raster_kernel_dsphere69120_traffic.json")
-    if os.path.exists(tpath):
+    if args.workload == "dsphere" and os.path.exists(tpath):
         with open(tpath) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
     out = {
@@ -174,7 +195,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": w.name, "triangles": int(len(w.tri)), "vertices": int(len(w.vtx)),
                    "orientations": n, "grid": w.grid, "voxel_mm": w.voxel_mm, "bed": "raft",
-                   "support": "explicit", "note": w.scale_note, "batch": info["batch"]},
+                   "support": "explicit", "shell": bool(w.shell), "note": w.scale_note, "batch": info["batch"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "kernel": "raster_kernel", "launch_orientations": batch,
@@ -193,6 +214,8 @@ def main():
         "argmin": int(_lib.find_first_optimal(rec["Mss"])),
     }
     s.close()
+    if not args.no_refine:
+        out["refine"] = refine_comparison(w)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(w, rec, args.cpu_stride, threads)

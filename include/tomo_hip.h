@@ -154,6 +154,34 @@ const char *tomo_last_error(void);
  * Tomo_Shell2026.cpp:89-97). */
 int32_t tomo_find_first_optimal(int32_t n, const float *mss);
 
+/* ---- continuous support volume (differentiable form) -------------------------------------
+ * Vss_c(yaw, pitch) = sum over triangles of sigma(-r.n - sin(theta_c)) A (-r.n) (r.c - zlow),
+ * r the build direction (third row of the reference's Rz(roll) Ry(pitch) Rx(yaw),
+ * SMatrix4f.cpp:213-240), n / A / c the unit face normal, area and centroid, sigma a C1
+ * smoothstep of width tau around the critical angle (tau = 0: the hard step) and zlow the soft
+ * minimum -temp log sum_v exp(-r.v / temp) of the vertex heights (temp = 0: the lowest vertex).
+ * The per-triangle precedent is getCHVss (STomoNV_CvxH.cpp:97-174); there is no reference entry
+ * point for it.  Roll does not enter (d/d roll = 0).  Device pointers and the stream (a
+ * hipStream_t, NULL for the null stream) are the caller's; calls are asynchronous on it. */
+typedef struct tomo_cvss tomo_cvss;
+
+/* Mesh as for tomo_set_mesh (host arrays; vertex normals are not used), theta_c from
+ * f32x12[1]; tau in [0, 2] (units of r.n), temp >= 0 (mesh units). */
+int tomo_cvss_create(const float *f32x12, int32_t n_tri, const int32_t *tri, int32_t n_vtx,
+                     const float *vtx, float tau, float temp, tomo_cvss **out);
+
+/* n orientations ypr_dev[n][3] (radians) -> vss_dev[n].  aux_dev: double[n][12] workspace that
+ * keeps each orientation's lowest point and sums for the backward pass. */
+int tomo_cvss_forward(tomo_cvss *c, int32_t n, const float *ypr_dev, float *vss_dev, double *aux_dev,
+                      void *stream);
+
+/* grad_ypr_dev[n][3] = grad_vss_dev[n] * d Vss_c / d (yaw, pitch, roll), using the aux_dev the
+ * forward call filled for the same ypr_dev (its slots 6..8 are overwritten). */
+int tomo_cvss_backward(tomo_cvss *c, int32_t n, const float *ypr_dev, double *aux_dev,
+                       const float *grad_vss_dev, float *grad_ypr_dev, void *stream);
+
+int tomo_cvss_destroy(tomo_cvss *c);
+
 /* ---- legacy drop-in shim (Tomo_Shell2026.h:19-35) ----------------------------------------- */
 
 /* Same arguments and return (optimal orientation index) as TomoSh_INT3 / TomoSh_CUDA

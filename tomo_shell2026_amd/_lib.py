@@ -31,6 +31,7 @@ EXPORTS = (
     "tomo_last_kernel_ms", "tomo_get_info", "tomo_set_phase_profile", "tomo_phase_profile", "tomo_destroy", "tomo_last_error",
     "tomo_find_first_optimal", "TomoSh_HIP", "getMss", "getMo", "getVtc", "getVolMassInfo",
     "getMat4x4", "getnData2i", "getpData2i", "OnDestroy",
+    "tomo_cvss_create", "tomo_cvss_forward", "tomo_cvss_backward", "tomo_cvss_destroy",
 )
 
 _lib = None
@@ -47,6 +48,13 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise TomoError(f"{LIB_PATH} is not built; run python -m tomo_shell2026_amd.build")
+    # PyTorch-ROCm ships its own HIP and HSA runtimes.  A process that uses both must load
+    # torch's first: the library's libhsa-runtime64.so.1 then binds to the one already loaded,
+    # while the other order leaves torch's HIP on a foreign HSA runtime that finds no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ct.CDLL(LIB_PATH)
     P, I32, F = ct.c_void_p, ct.c_int32, ct.c_float
     sig = {
@@ -77,6 +85,10 @@ def load():
         "getnData2i": ((ct.c_int16,), I32),
         "getpData2i": ((ct.c_int16,), ct.POINTER(ct.c_int16)),
         "OnDestroy": ((), None),
+        "tomo_cvss_create": ((P, I32, P, I32, P, F, F, P), ct.c_int),
+        "tomo_cvss_forward": ((P, I32, P, P, P, P), ct.c_int),
+        "tomo_cvss_backward": ((P, I32, P, P, P, P, P), ct.c_int),
+        "tomo_cvss_destroy": ((P,), ct.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtomo_hip.so")
 ARCH = os.environ.get("TOMO_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["tomo_kernels.hip", "tomo_api.cpp"]
+SOURCES = ["tomo_kernels.hip", "tomo_cvss.hip", "tomo_api.cpp"]
 HEADERS = ["tomo_device.h", "tomo_slot.h"]
 
 FLAGS = [

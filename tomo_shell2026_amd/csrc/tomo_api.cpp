@@ -2,6 +2,7 @@
 // record finishing (float totals, VolToMass, transform) and the legacy drop-in shim.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -14,6 +15,15 @@
 
 namespace tomo {
 hipError_t launch_stage(int stage, const SweepParams &p, hipStream_t stream);
+struct CvssMeshDev {
+  const float *nx, *ny, *nz, *area, *cx, *cy, *cz, *vx, *vy, *vz;
+  int32_t n_tri, n_vtx;
+  float s, tau, temp;
+};
+hipError_t cvss_forward(const CvssMeshDev &m, int32_t n, const float *ypr, float *vss, double *aux,
+                        hipStream_t st);
+hipError_t cvss_backward(const CvssMeshDev &m, int32_t n, const float *ypr, double *aux,
+                         const float *grad_vss, float *grad_ypr, hipStream_t st);
 }
 
 using namespace tomo;
@@ -790,6 +800,104 @@ int32_t getnData2i(int16_t t) {
 int16_t *getpData2i(int16_t t) {
   if (t < 0 || t >= 11 || g_pix[(size_t)t].empty()) return nullptr;
   return g_pix[(size_t)t].data();
+}
+
+// ---- continuous support volume ----------------------------------------------------------
+
+struct tomo_cvss {
+  int device = 0;
+  float *d = nullptr;  // 7 * n_tri + 3 * n_vtx floats
+  CvssMeshDev m{};
+};
+
+int tomo_cvss_create(const float *f32, int32_t n_tri, const int32_t *tri, int32_t n_vtx, const float *vtx,
+                     float tau, float temp, tomo_cvss **out) {
+  if (!out) return fail(TOMO_E_INVALID_ARG, "null output handle");
+  *out = nullptr;
+  if (!f32 || n_tri < 0 || n_vtx < 0 || (n_tri > 0 && (!tri || !vtx)) || (n_vtx > 0 && !vtx))
+    return fail(TOMO_E_INVALID_ARG, "bad mesh arguments");
+  if (n_tri > 0 && n_vtx == 0) return fail(TOMO_E_INVALID_ARG, "triangles without vertices");
+  for (int64_t i = 0; i < (int64_t)n_tri * 3; i++)
+    if (tri[i] < 0 || tri[i] >= n_vtx) return fail(TOMO_E_INVALID_ARG, "triangle index out of range");
+  const float theta_c = f32[1];
+  if (!(theta_c >= 0.0f && theta_c <= 3.14159265f)) return fail(TOMO_E_INVALID_ARG, "theta_c out of [0, pi]");
+  if (!(tau >= 0.0f && tau <= 2.0f) || !(temp >= 0.0f && temp < 1e6f))
+    return fail(TOMO_E_INVALID_ARG, "smoothing width / temperature out of range");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(TOMO_E_NO_DEVICE, "no HIP device");
+  // Geometry in double on the host, relative to the AABB centre, stored as float.
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  for (int32_t v = 0; v < n_vtx; v++)
+    for (int k = 0; k < 3; k++) {
+      const double x = vtx[(size_t)v * 3 + k];
+      if (v == 0 || x < lo[k]) lo[k] = x;
+      if (v == 0 || x > hi[k]) hi[k] = x;
+    }
+  const double c0[3] = {0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1]), 0.5 * (lo[2] + hi[2])};
+  const size_t T = (size_t)n_tri, V = (size_t)n_vtx;
+  std::vector<float> h(7 * T + 3 * V + 1, 0.0f);
+  for (size_t t = 0; t < T; t++) {
+    double p[3][3];
+    for (int j = 0; j < 3; j++)
+      for (int k = 0; k < 3; k++) p[j][k] = (double)vtx[(size_t)tri[t * 3 + j] * 3 + k] - c0[k];
+    const double e1[3] = {p[1][0] - p[0][0], p[1][1] - p[0][1], p[1][2] - p[0][2]};
+    const double e2[3] = {p[2][0] - p[0][0], p[2][1] - p[0][1], p[2][2] - p[0][2]};
+    const double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    const double len = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    for (int k = 0; k < 3; k++) h[k * T + t] = len > 0.0 ? (float)(n[k] / len) : 0.0f;
+    h[3 * T + t] = (float)(0.5 * len);
+    for (int k = 0; k < 3; k++) h[(4 + k) * T + t] = (float)((p[0][k] + p[1][k] + p[2][k]) / 3.0);
+  }
+  for (size_t v = 0; v < V; v++)
+    for (int k = 0; k < 3; k++) h[7 * T + k * V + v] = (float)((double)vtx[v * 3 + k] - c0[k]);
+  tomo_cvss *c = new (std::nothrow) tomo_cvss();
+  if (!c) return fail(TOMO_E_OOM, "handle allocation failed");
+  if (hipGetDevice(&c->device) != hipSuccess || hipMalloc(&c->d, sizeof(float) * h.size()) != hipSuccess) {
+    delete c;
+    return fail(TOMO_E_OOM, "device mesh allocation failed");
+  }
+  if (hipMemcpy(c->d, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    hipFree(c->d);
+    delete c;
+    return fail(TOMO_E_HIP, "device mesh upload failed");
+  }
+  float *d = c->d;
+  c->m = CvssMeshDev{d, d + T, d + 2 * T, d + 3 * T, d + 4 * T, d + 5 * T, d + 6 * T,
+                     d + 7 * T, d + 7 * T + V, d + 7 * T + 2 * V, n_tri, n_vtx, sinf(theta_c), tau, temp};
+  *out = c;
+  return TOMO_OK;
+}
+
+static int cvss_check(const tomo_cvss *c, int32_t n, const void *a, const void *b) {
+  if (!c || n < 0 || (n > 0 && (!a || !b))) return fail(TOMO_E_INVALID_ARG, "bad arguments");
+  const int64_t chunk = 256 * 8;
+  const int64_t blocks = (int64_t)n * ((std::max<int64_t>(c->m.n_tri, c->m.n_vtx) + chunk - 1) / chunk);
+  if (blocks >= ((int64_t)1 << 31)) return fail(TOMO_E_INVALID_ARG, "batch too large for one launch");
+  return TOMO_OK;
+}
+
+int tomo_cvss_forward(tomo_cvss *c, int32_t n, const float *ypr, float *vss, double *aux, void *stream) {
+  int rc = cvss_check(c, n, ypr, vss);
+  if (rc) return rc;
+  if (n > 0 && !aux) return fail(TOMO_E_INVALID_ARG, "null workspace");
+  HIP_TRY(cvss_forward(c->m, n, ypr, vss, aux, (hipStream_t)stream));
+  return TOMO_OK;
+}
+
+int tomo_cvss_backward(tomo_cvss *c, int32_t n, const float *ypr, double *aux, const float *grad_vss,
+                       float *grad_ypr, void *stream) {
+  int rc = cvss_check(c, n, ypr, grad_ypr);
+  if (rc) return rc;
+  if (n > 0 && (!aux || !grad_vss)) return fail(TOMO_E_INVALID_ARG, "null workspace or gradient");
+  HIP_TRY(cvss_backward(c->m, n, ypr, aux, grad_vss, grad_ypr, (hipStream_t)stream));
+  return TOMO_OK;
+}
+
+int tomo_cvss_destroy(tomo_cvss *c) {
+  if (!c) return TOMO_OK;
+  if (c->d) hipFree(c->d);
+  delete c;
+  return TOMO_OK;
 }
 
 void OnDestroy(void) {
