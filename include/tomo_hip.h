@@ -6,7 +6,7 @@
  *  1. The drop-in legacy shim.  Same ten-pointer signature, header-array layout, return value
  *     and getter set as the reference DLL that cfms_tomo binds with ctypes
  *     (cfms_tomo/Tomo_Shell_vc_src/Tomo_Shell2026.h:19-35; bound at
- *     cfms_tomo/tomo_dll_interface.py:66-77).  Outputs are owned by the library and stay valid
+ *     cfms_tomo/tomo_dll_interface.py:24-35, called at :85-97).  Outputs are owned by the library and stay valid
  *     until the next TomoSh_HIP call or OnDestroy, as in the reference (Tomo_Shell2026.cpp:19-35,
  *     73-87).  Not re-entrant (global state), like the reference.
  *
@@ -15,7 +15,7 @@
  *     (_TomoSh_Function_Call, Tomo_Shell2026.cpp:119-249) split into set-up, sweep and readback.
  *
  * Units and options follow the reference: Float32Info[12] and Int32Info[11] in the order packed
- * by tomo_dll_interface.py:115-124 and decoded by S3DPrinterInfo::Set (S3DPrinterInfo.cpp:124-145);
+ * by tomo_dll_interface.py:73-82 and decoded by S3DPrinterInfo::Set (S3DPrinterInfo.cpp:124-145);
  * vertex coordinates are voxel indices (STomoVoxelSpaceInfo.cpp:97-105).
  */
 #ifndef TOMO_HIP_H

@@ -35,7 +35,7 @@ enum {
 };
 
 /* Options: Float32Info[12] and Int32Info[11] in the reference's order
- * (tomo_dll_interface.py:115-124; S3DPrinterInfo.cpp:124-145), plus the grid edge. */
+ * (tomo_dll_interface.py:73-82; S3DPrinterInfo.cpp:124-145), plus the grid edge. */
 typedef struct {
   float f32[12];   /* dVoxel, theta_c, area, wall, Fcore, Fclad, Fss, Css, rho, BedInner, BedOuter, BedThick */
   int32_t i32[11]; /* verbose, explicitSS, shell, nVoxel, nTri, nVtx, nYPR, nCHtri, nCHvtx, BedType, shell_um */

@@ -1,5 +1,5 @@
 """The cfms_tomo drop-in API: option packing and orientation grid as the reference builds them
-(cfms_tomo/tomo_dll_interface.py:78-124), optimum search (tomo_io.py:50-80), and on the GPU a
+(cfms_tomo/tomo_dll_interface.py:37-43, 73-82), optimum search (tomo_io.py:50-80), and on the GPU a
 run_dll round trip checked against the oracle."""
 import ctypes as ct
 

@@ -1,6 +1,6 @@
 """TomoDLLInterface: the drop-in replacement of cfms_tomo/tomo_dll_interface.py.
 
-Same constructor and run_dll contract as the reference (tomo_dll_interface.py:54-135): the mesh
+Same constructor and run_dll contract as the reference (tomo_dll_interface.py:55-135): the mesh
 and options are packed into Float32Info[12] / Int32Info[11] and the ten-pointer entry point is
 called; per-orientation Mss, the optimum and the placement matrix are read back through the
 same getters.  The entry point is TomoSh_HIP of libtomo_hip.so (MI355X), which reproduces the
@@ -69,7 +69,7 @@ class TomoDLLInterface:
         return cutobject.tmesh if hasattr(cutobject, "tmesh") else cutobject
 
     def pack(self, T, bVerbose):
-        """Float32Info / Int32Info in the reference's order (tomo_dll_interface.py:114-124)."""
+        """Float32Info / Int32Info in the reference's order (tomo_dll_interface.py:73-82)."""
         S_O = self.slicing_option
         vtx = np.asarray(T.vertices).astype(np.float32, copy=True)
         tri = np.asarray(T.faces).astype(np.int32, copy=True)

@@ -2,7 +2,7 @@
 
 The reference's run_dll reads vertices, vertex normals, faces, face normals, area,
 watertightness and the convex hull from a trimesh object
-(cfms_tomo/tomo_dll_interface.py:59-70, 101-112).  trimesh is not a dependency here: this
+(cfms_tomo/tomo_dll_interface.py:59-70).  trimesh is not a dependency here: this
 module computes the same quantities with numpy so the drop-in path runs without it.
 Vertex normals are the angle-weighted sum of incident face normals, normalised (trimesh's
 default weighting); results are not pinned against trimesh itself.

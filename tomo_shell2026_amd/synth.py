@@ -20,7 +20,7 @@ from .mesh import TomoMesh, mesh_area, vertex_normals
 
 # ---------------------------------------------------------------- options (Float32Info / Int32Info)
 def default_f32(area, theta_deg=60.0, bed_inner=0.0, bed_outer=2.0, bed_thick=0.3 + 0.27 + 2 * 0.2):
-    """Float32Info[12] in the order of tomo_dll_interface.py:115-118 with SlicingOptions defaults
+    """Float32Info[12] in the order of tomo_dll_interface.py:73-76 with SlicingOptions defaults
     (tomo_slicingOptions.py:41-58)."""
     return np.array([1.0, theta_deg * np.pi / 180.0, area, 0.8, 0.15, 1.0, 0.2, 1.0, 0.001121,
                      bed_inner, bed_outer, bed_thick], dtype=np.float32)
@@ -31,14 +31,14 @@ BED_NONE, BED_BRIM, BED_RAFT, BED_SKIRT = 0, 1, 2, 3
 
 def default_i32(n_tri, n_vtx, n_ypr, grid=256, explicit_ss=True, shell=False, bed_type=BED_RAFT,
                 shell_um=500):
-    """Int32Info[11] in the order of tomo_dll_interface.py:120-124."""
+    """Int32Info[11] in the order of tomo_dll_interface.py:78-82."""
     return np.array([0, int(explicit_ss), int(shell), grid, n_tri, n_vtx, n_ypr, 0, 0, bed_type,
                      shell_um], dtype=np.int32)
 
 
 # ---------------------------------------------------------------- orientation grids
 def ypr_grid(yaw_deg, pitch_deg):
-    """Rows pitch-major, yaw fastest, roll 0 (tomo_dll_interface.py:80-85 ordering).
+    """Rows pitch-major, yaw fastest, roll 0 (tomo_dll_interface.py:38-43 ordering).
     Angles are float32(deg * pi / 180) computed in float64."""
     yaw = np.asarray(yaw_deg, np.float64) * np.pi / 180.0
     pit = np.asarray(pitch_deg, np.float64) * np.pi / 180.0
@@ -47,7 +47,7 @@ def ypr_grid(yaw_deg, pitch_deg):
 
 
 def reference_ypr_grid(theta_step_deg):
-    """The drop-in grid of TomoDLLInterface.__init__ (tomo_dll_interface.py:80-85): yaw and pitch
+    """The drop-in grid of TomoDLLInterface.__init__ (tomo_dll_interface.py:37-43): yaw and pitch
     both linspace(0, 2 pi, 360/step + 1, float32), roll 0."""
     n = int(360 / theta_step_deg) + 1
     r = np.linspace(0.0, 2 * np.pi, num=n, endpoint=True, dtype=np.float32)
