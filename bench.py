@@ -91,6 +91,71 @@ def cpu_baseline(w, records, stride, threads):
             "matches_gpu_on_sample": same}
 
 
+def bench_batch70(args):
+    """70 meshes (6 thin-shell manikin parts, 64 displaced spheres of 1k-100k triangles), each on
+    its own 1 degree x 0..60 grid: autotune, capture one graph, time `steps` launches; compare
+    with the same sweeps launched mesh by mesh with the default shape and no graph."""
+    from tomo_shell2026_amd import _lib, synth
+    ws = synth.batch70_workloads()
+    total = sum(len(w.ypr) for w in ws)
+    B = _lib.Batch()
+    for w in ws:
+        B.add(w.f32, w.i32, w.grid, w.tri, w.vtx, w.vnrm, w.ypr)
+    t0 = time.perf_counter()
+    B.tune(4096)
+    tune_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    B.capture()
+    capture_s = time.perf_counter() - t0
+    for _ in range(max(args.warmup, 0)):
+        B.launch()
+        B.sync()
+    ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        B.launch()
+        B.sync()
+        ms.append(B.last_ms())
+    dt = time.perf_counter() - t0
+    shapes = {}
+    for m in range(len(ws)):
+        t = B.tuning(m)
+        k = (t["batch"], t["tri_per_thread"], t["inline"])
+        shapes[str(k)] = shapes.get(str(k), 0) + 1
+    flags = {"cap_drop": 0, "fp32_inexact": 0, "footprint_oob": 0, "unresolved": 0}
+    for m in range(len(ws)):
+        r = B.records(m)
+        flags["cap_drop"] += int(((r["flags"] & 8) != 0).sum())
+        flags["fp32_inexact"] += int(((r["flags"] & 1) != 0).sum())
+        flags["footprint_oob"] += int(((r["flags"] & 4) != 0).sum())
+        flags["unresolved"] += int(((r["flags"] & 64) != 0).sum())
+    B.close()
+    # the same sweeps, mesh by mesh, default launch shape, no graph
+    plain_ms = 0.0
+    for w in ws:
+        with _lib.Sweep(w.f32, w.i32, w.grid) as s:
+            s.set_mesh(w.tri, w.vtx, w.vnrm)
+            s.set_orientations(w.ypr)
+            s.run()
+            s.sync()
+            s.run()
+            s.sync()
+            plain_ms += s.kernel_ms()
+    graph_ms = float(np.mean(ms))
+    return {
+        "metric": "orientation evaluations/s, 70 uneven meshes in one graph launch",
+        "value": round(total * args.steps / dt, 1), "unit": "orientations/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+i32",
+        "data": "synthetic",
+        "config": {"workload": "batch70", "meshes": len(ws), "triangles_total": int(sum(len(w.tri) for w in ws)),
+                   "orientations_total": total, "grids": sorted({w.grid for w in ws})},
+        "graph_ms": round(graph_ms, 2), "plain_per_mesh_ms": round(plain_ms, 2),
+        "graph_vs_plain_speedup": round(plain_ms / graph_ms, 3),
+        "tune_s": round(tune_s, 2), "capture_s": round(capture_s, 2), "tuned_shapes": shapes, "flags": flags,
+    }
+
+
 def refine_comparison(w):
     """Least voxel Vss on the dense 0.1 degree yaw x pitch grid (0..359.9 x 0..60) by the plain
     sweep, against the coarse 5 degree sweep + batched gradient refinement of the continuous
@@ -115,9 +180,10 @@ def main():
     ap.add_argument("--batch", type=int, default=0, help="orientations in flight (0 = auto)")
     ap.add_argument("--cpu-stride", type=int, default=11)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("dsphere", "manikin"), default="dsphere",
+    ap.add_argument("--workload", choices=("dsphere", "manikin", "batch70"), default="dsphere",
                     help="dsphere: 69,120-triangle displaced sphere, 256 grid (the headline workload); "
-                         "manikin: ~87k-triangle thin-shell manikin, 512 grid, shell mode")
+                         "manikin: ~87k-triangle thin-shell manikin, 512 grid, shell mode; "
+                         "batch70: 70 uneven meshes in one tuned, captured graph launch")
     ap.add_argument("--no-refine", action="store_true",
                     help="skip the dense 0.1 degree grid vs coarse + gradient-refined search comparison")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -140,6 +206,9 @@ def main():
         sys.exit(2)
 
     from tomo_shell2026_amd import _lib, synth
+    if args.workload == "batch70":
+        print(json.dumps(bench_batch70(args)), flush=True)
+        return
     w = synth.manikin_workload() if args.workload == "manikin" else synth.displaced_sphere_workload()
     n = len(w.ypr)
     s = _lib.Sweep(w.f32, w.i32, w.grid)

@@ -122,6 +122,8 @@ int tomo_set_launch(tomo_handle *h, int32_t batch);
 /* Launch-shape knobs for tuning (values take effect at the next tomo_run):
  * TOMO_TUNE_BATCH           orientations in flight (0 = automatic), same as tomo_set_launch;
  * TOMO_TUNE_TRI_PER_THREAD  triangles per rasteriser thread (1..64, default 4);
+ * TOMO_TUNE_INLINE          slot entries stored inline per slot (8..32, default 16); the rest go
+ *                           to an overflow pool (more inline = fewer pool passes, more memory);
  * TOMO_TUNE_EXACT_FSUM      1 (default): where a float total can round (TOMO_F_FP32_INEXACT),
  *                           replay the reference's ascending-slot float accumulation so fVo,
  *                           fVss, fVa, fVb, fVnv, fVtc and the masses equal the reference's;
@@ -129,6 +131,7 @@ int tomo_set_launch(tomo_handle *h, int32_t batch);
 #define TOMO_TUNE_BATCH 0
 #define TOMO_TUNE_TRI_PER_THREAD 1
 #define TOMO_TUNE_EXACT_FSUM 2
+#define TOMO_TUNE_INLINE 3
 int tomo_set_tuning(tomo_handle *h, int32_t knob, int32_t value);
 
 /* Duration in milliseconds of the last tomo_run's kernel, from HIP events on the handle's
@@ -153,6 +156,48 @@ const char *tomo_last_error(void);
 /* First strict minimum, initial value 1e5; 0 for n <= 1 (_find1stOptimal,
  * Tomo_Shell2026.cpp:89-97). */
 int32_t tomo_find_first_optimal(int32_t n, const float *mss);
+
+/* ---- batched sweeps: many meshes in one graph launch --------------------------------------
+ * Replaces calling the reference entry point once per cut part (TomoManager.get_best_orientation,
+ * cfms_tomo/__init__.py:26-30, one TomoSh_INT3 per part): every member mesh keeps its own options,
+ * orientations and records; all members' sweeps are captured into one hipGraph on one stream
+ * and run by one launch.  Members share the slot-table scratch, sized for the largest. */
+typedef struct tomo_batch tomo_batch;
+
+int tomo_batch_create(tomo_batch **out);
+
+/* Add a member (host arrays, copied): options and grid as tomo_create, mesh as tomo_set_mesh,
+ * orientations as tomo_set_orientations.  *member receives its index.  Invalidates a capture. */
+int tomo_batch_add(tomo_batch *b, const float *float32_info_x12, const int32_t *int32_info_x11,
+                   int32_t grid, int32_t n_tri, const int32_t *tri, int32_t n_vtx, const float *vtx,
+                   const float *vtx_nrm, int32_t n_ypr, const float *ypr, int32_t *member);
+
+/* Autotuner: per group of members with the same grid and ceil(log2(nTri)), time `sample`
+ * orientations of the group's largest member over triangles per rasteriser thread, inline slot
+ * capacity and orientations in flight, and keep the fastest shape that resolves every slot. */
+int tomo_batch_tune(tomo_batch *b, int32_t sample);
+
+/* The launch shape chosen for a member (defaults before tomo_batch_tune) and its measured
+ * microseconds per orientation (negative if not tuned). */
+int tomo_batch_get_tuning(tomo_batch *b, int32_t member, int32_t *batch, int32_t *tri_per_thread,
+                          int32_t *inline_capacity, float *us_per_orientation);
+
+/* Allocate the shared scratch and capture every member's sweep into one graph (no launch). */
+int tomo_batch_capture(tomo_batch *b);
+
+/* Launch the graph (capturing first if needed); asynchronous on the batch's stream. */
+int tomo_batch_launch(tomo_batch *b);
+int tomo_batch_sync(tomo_batch *b);
+
+/* Milliseconds of the last graph launch (HIP events on the batch's stream; negative if none). */
+float tomo_batch_last_ms(tomo_batch *b);
+
+int32_t tomo_batch_size(tomo_batch *b);
+
+/* Records of one member's orientations (out[n_ypr of that member]); synchronises. */
+int tomo_batch_records(tomo_batch *b, int32_t member, tomo_record *out);
+
+int tomo_batch_destroy(tomo_batch *b);
 
 /* ---- continuous support volume (differentiable form) -------------------------------------
  * Vss_c(yaw, pitch) = sum over triangles of sigma(-r.n - sin(theta_c)) A (-r.n) (r.c - zlow),

@@ -75,7 +75,7 @@ def compare_records(gpu, ref, label=""):
         if d.size:
             i = d[0]
             bad.append(f"{label}{k}: {d.size} differ, first #{i}: gpu {gpu[k][i]} ref {ref[k][i]}")
-    fl = np.nonzero((gpu["flags"] & 31) != ref["flags"])[0]
+    fl = np.nonzero((gpu["flags"] & 31) != (ref["flags"] & 31))[0]
     if fl.size:
         i = fl[0]
         bad.append(f"{label}flags: {fl.size} differ, first #{i}: gpu {gpu['flags'][i]} ref {ref['flags'][i]}")

@@ -32,6 +32,9 @@ EXPORTS = (
     "tomo_find_first_optimal", "TomoSh_HIP", "getMss", "getMo", "getVtc", "getVolMassInfo",
     "getMat4x4", "getnData2i", "getpData2i", "OnDestroy",
     "tomo_cvss_create", "tomo_cvss_forward", "tomo_cvss_backward", "tomo_cvss_destroy",
+    "tomo_batch_create", "tomo_batch_add", "tomo_batch_tune", "tomo_batch_get_tuning", "tomo_batch_capture",
+    "tomo_batch_launch", "tomo_batch_sync", "tomo_batch_last_ms", "tomo_batch_size", "tomo_batch_records",
+    "tomo_batch_destroy",
 )
 
 _lib = None
@@ -89,6 +92,17 @@ def load():
         "tomo_cvss_forward": ((P, I32, P, P, P, P), ct.c_int),
         "tomo_cvss_backward": ((P, I32, P, P, P, P, P), ct.c_int),
         "tomo_cvss_destroy": ((P,), ct.c_int),
+        "tomo_batch_create": ((P,), ct.c_int),
+        "tomo_batch_add": ((P, P, P, I32, I32, P, I32, P, P, I32, P, P), ct.c_int),
+        "tomo_batch_tune": ((P, I32), ct.c_int),
+        "tomo_batch_get_tuning": ((P, I32, P, P, P, P), ct.c_int),
+        "tomo_batch_capture": ((P,), ct.c_int),
+        "tomo_batch_launch": ((P,), ct.c_int),
+        "tomo_batch_sync": ((P,), ct.c_int),
+        "tomo_batch_last_ms": ((P,), F),
+        "tomo_batch_size": ((P,), I32),
+        "tomo_batch_records": ((P, I32, P), ct.c_int),
+        "tomo_batch_destroy": ((P,), ct.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -159,7 +173,7 @@ class Sweep:
     def set_launch(self, n_groups):
         check(load().tomo_set_launch(self._h, int(n_groups)), "tomo_set_launch")
 
-    TUNE_BATCH, TUNE_TRI_PER_THREAD, TUNE_EXACT_FSUM = 0, 1, 2
+    TUNE_BATCH, TUNE_TRI_PER_THREAD, TUNE_EXACT_FSUM, TUNE_INLINE = 0, 1, 2, 3
 
     def set_tuning(self, knob, value):
         check(load().tomo_set_tuning(self._h, int(knob), int(value)), "tomo_set_tuning")
@@ -237,3 +251,74 @@ def decode_rows(rows, nrows):
 def find_first_optimal(mss):
     m = np.ascontiguousarray(mss, np.float32)
     return int(load().tomo_find_first_optimal(m.size, ptr(m)))
+
+
+class Batch:
+    """Many meshes swept by one captured graph launch (tomo_batch_*)."""
+
+    def __init__(self):
+        h = ct.c_void_p()
+        check(load().tomo_batch_create(ct.byref(h)), "tomo_batch_create")
+        self._h = h
+        self._n = []
+
+    def add(self, f32, i32, grid, tri, vtx, vnrm, ypr):
+        f = np.ascontiguousarray(f32, np.float32)
+        i = np.ascontiguousarray(i32, np.int32)
+        t = np.ascontiguousarray(tri, np.int32).reshape(-1, 3)
+        v = np.ascontiguousarray(vtx, np.float32).reshape(-1, 3)
+        n = np.ascontiguousarray(vnrm, np.float32).reshape(-1, 3)
+        y = np.ascontiguousarray(ypr, np.float32).reshape(-1, 3)
+        m = ct.c_int32()
+        check(load().tomo_batch_add(self._h, ptr(f), ptr(i), int(grid), t.shape[0], ptr(t), v.shape[0], ptr(v),
+                                    ptr(n), y.shape[0], ptr(y), ct.byref(m)), "tomo_batch_add")
+        self._n.append(y.shape[0])
+        return m.value
+
+    def tune(self, sample=4096):
+        check(load().tomo_batch_tune(self._h, int(sample)), "tomo_batch_tune")
+
+    def tuning(self, member):
+        b, t, i = ct.c_int32(), ct.c_int32(), ct.c_int32()
+        us = ct.c_float()
+        check(load().tomo_batch_get_tuning(self._h, int(member), ct.byref(b), ct.byref(t), ct.byref(i),
+                                           ct.byref(us)), "tomo_batch_get_tuning")
+        return {"batch": b.value, "tri_per_thread": t.value, "inline": i.value,
+                "us_per_orientation": round(us.value, 4)}
+
+    def capture(self):
+        check(load().tomo_batch_capture(self._h), "tomo_batch_capture")
+
+    def launch(self):
+        check(load().tomo_batch_launch(self._h), "tomo_batch_launch")
+
+    def sync(self):
+        check(load().tomo_batch_sync(self._h), "tomo_batch_sync")
+
+    def last_ms(self):
+        return float(load().tomo_batch_last_ms(self._h))
+
+    def records(self, member):
+        out = np.zeros(self._n[member], RECORD_DTYPE)
+        check(load().tomo_batch_records(self._h, int(member), ptr(out)), "tomo_batch_records")
+        return out
+
+    def __len__(self):
+        return len(self._n)
+
+    def close(self):
+        if self._h:
+            load().tomo_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -139,10 +139,13 @@ void vol_to_mass(const Options &o, float *vm) {
   Mtotal = Mo + Mss;
 }
 
-ScratchLayout make_layout(int32_t G) {
+// One orientation's slot-table region.  inl entries per slot are stored inline; the overflow
+// pool grows as inl shrinks (2 * max(G^2, 65536) entries at inl = 8).
+ScratchLayout make_layout(int32_t G, int32_t inl) {
   ScratchLayout L{};
   const int64_t G2 = (int64_t)G * G;
-  const int32_t pool = (int32_t)(G2 > 65536 ? G2 : 65536);
+  const int64_t base = G2 > 65536 ? G2 : 65536;
+  const int32_t pool = (int32_t)(base * kInlineDefault / (inl < kInlineDefault ? inl : kInlineDefault));
   int64_t at = 0;
   auto take = [&](int64_t bytes) {
     const int64_t here = at;
@@ -150,7 +153,7 @@ ScratchLayout make_layout(int32_t G) {
     return here;
   };
   L.count = take(4 * G2);
-  L.ent = take(8 * G2 * kInline);
+  L.ent = take(8 * G2 * inl);
   L.bedinfo = take(4 * G2);
   L.big = take(4 * G2);
   L.off = take(4 * G2);
@@ -163,6 +166,7 @@ ScratchLayout make_layout(int32_t G) {
   L.sv = take(4 * 6 * G2);
   L.bytes = at;
   L.pool = pool;
+  L.inl = inl;
   return L;
 }
 
@@ -200,6 +204,11 @@ struct tomo_handle {
   float4 *d_pv = nullptr;      // rotated vertices per in-flight orientation
   size_t pv_cap = 0;           // float4 capacity of d_pv
   int32_t scratch_batch = 0;
+  int64_t scratch_region = 0;  // bytes per orientation region of the current allocation
+  // Shared mode (a tomo_batch member): stream, scratch, headers and rotated-vertex buffer
+  // belong to the batch; own_stream is the handle's original stream, kept for destruction.
+  bool shared = false;
+  hipStream_t own_stream = nullptr;
   int32_t batch_req = 0;
   int32_t tri_per_thread = kTriPerThread;
   int32_t exact_fsum = 1;
@@ -223,6 +232,7 @@ void free_stage_events(tomo_handle *h) {
 int ensure_pv(tomo_handle *h, int32_t batch) {
   const size_t need = (size_t)batch * (size_t)(h->n_vtx > 0 ? h->n_vtx : 1);
   if (need <= h->pv_cap) return TOMO_OK;
+  if (h->shared) return fail(TOMO_E_STATE, "shared rotated-vertex buffer too small");
   HIP_TRY(hipStreamSynchronize(h->stream));
   if (h->d_pv) { HIP_TRY(hipFree(h->d_pv)); h->d_pv = nullptr; }
   h->pv_cap = 0;
@@ -237,7 +247,8 @@ int ensure_pv(tomo_handle *h, int32_t batch) {
 int ensure_scratch(tomo_handle *h, int32_t batch) {
   int rc = ensure_pv(h, batch);
   if (rc) return rc;
-  if (batch <= h->scratch_batch) return TOMO_OK;
+  if (batch <= h->scratch_batch && h->lay.bytes == h->scratch_region) return TOMO_OK;
+  if (h->shared) return fail(TOMO_E_STATE, "shared scratch does not fit this launch shape");
   HIP_TRY(hipStreamSynchronize(h->stream));
   if (h->d_scratch) { HIP_TRY(hipFree(h->d_scratch)); h->d_scratch = nullptr; }
   if (h->d_hdr) { HIP_TRY(hipFree(h->d_hdr)); h->d_hdr = nullptr; }
@@ -256,6 +267,7 @@ int ensure_scratch(tomo_handle *h, int32_t batch) {
   HIP_TRY(hipMemsetAsync(h->d_hdr, 0, sizeof(OrientHdr) * (size_t)batch, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
   h->scratch_batch = batch;
+  h->scratch_region = h->lay.bytes;
   return TOMO_OK;
 }
 
@@ -427,7 +439,7 @@ int tomo_create(const float *f32, const int32_t *i32, int32_t grid, tomo_handle 
     tomo_destroy(h);
     return fail(TOMO_E_HIP, "stream/event creation failed");
   }
-  h->lay = make_layout(o.grid);
+  h->lay = make_layout(o.grid, kInlineDefault);
   // IsBedCandidate distances dist2D (STomoSh_INT3.cpp:1326-1329) for dx, dy in [-r, r).
   const int r = o.bed_r > 0 ? o.bed_r : 1;
   std::vector<float> dist((size_t)(2 * r) * (2 * r));
@@ -529,6 +541,13 @@ int tomo_set_tuning(tomo_handle *h, int32_t knob, int32_t value) {
     case TOMO_TUNE_TRI_PER_THREAD:
       if (value < 1 || value > 64) return fail(TOMO_E_INVALID_ARG, "triangles per thread must be 1..64");
       h->tri_per_thread = value;
+      return TOMO_OK;
+    case TOMO_TUNE_INLINE:
+      if (value < kSmall || value > kInlineMax) return fail(TOMO_E_INVALID_ARG, "inline slot capacity must be 8..32");
+      if (value != h->lay.inl) {
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        h->lay = make_layout(h->opt.grid, value);
+      }
       return TOMO_OK;
     case TOMO_TUNE_EXACT_FSUM:
       if (value != 0 && value != 1) return fail(TOMO_E_INVALID_ARG, "exact float totals must be 0 or 1");
@@ -681,6 +700,12 @@ int tomo_destroy(tomo_handle *h) {
   if (!h) return TOMO_OK;
   if (h->stream) hipStreamSynchronize(h->stream);
   free_stage_events(h);
+  if (h->shared) {  // buffers and stream belong to the batch
+    h->d_pv = nullptr;
+    h->d_hdr = nullptr;
+    h->d_scratch = nullptr;
+    h->stream = h->own_stream;
+  }
   if (h->d_pv) hipFree(h->d_pv);
   if (h->d_hdr) hipFree(h->d_hdr);
   if (h->d_scratch) hipFree(h->d_scratch);
@@ -693,6 +718,332 @@ int tomo_destroy(tomo_handle *h) {
   if (h->ev1) hipEventDestroy(h->ev1);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
+  return TOMO_OK;
+}
+
+
+// ---------------------------------------------------------------------------------------------
+// Batched sweeps: many meshes, each with its own orientations and options, in one captured
+// hipGraph launch on one stream.  Members keep their own mesh, rotations and records; they share
+// the batch's stream, slot-table scratch, headers and rotated-vertex buffer, sized for the
+// largest member.  Before each member an acquire pass zeroes the slot counters of the regions it
+// will use, laid out for its grid and inline capacity (the memory held another member's layout).
+
+struct tomo_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<tomo_handle *> members;
+  struct Tune {
+    int32_t batch = 0, tri_per_thread = kTriPerThread, inl = kInlineDefault;
+    float us_per_orientation = -1.0f;
+    int32_t tuned = 0;
+  };
+  std::vector<Tune> tune;
+  uint8_t *d_scratch = nullptr;
+  OrientHdr *d_hdr = nullptr;
+  float4 *d_pv = nullptr;
+  size_t scratch_bytes = 0, hdr_count = 0, pv_count = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool timed = false;
+};
+
+static void batch_drop_graph(tomo_batch *b) {
+  if (b->exec) { hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+  if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
+}
+
+static void release_private_scratch(tomo_handle *h) {
+  hipStreamSynchronize(h->stream);
+  if (h->d_scratch) { hipFree(h->d_scratch); h->d_scratch = nullptr; }
+  if (h->d_hdr) { hipFree(h->d_hdr); h->d_hdr = nullptr; }
+  if (h->d_pv) { hipFree(h->d_pv); h->d_pv = nullptr; }
+  h->scratch_batch = 0;
+  h->scratch_region = 0;
+  h->pv_cap = 0;
+}
+
+// Detach members from the shared buffers (before re-tuning or destruction).
+static void batch_unshare(tomo_batch *b) {
+  if (b->stream) hipStreamSynchronize(b->stream);
+  batch_drop_graph(b);
+  for (tomo_handle *h : b->members)
+    if (h->shared) {
+      h->shared = false;
+      h->stream = h->own_stream;
+      h->d_scratch = nullptr;
+      h->d_hdr = nullptr;
+      h->d_pv = nullptr;
+      h->scratch_batch = 0;
+      h->scratch_region = 0;
+      h->pv_cap = 0;
+    }
+  if (b->d_scratch) { hipFree(b->d_scratch); b->d_scratch = nullptr; }
+  if (b->d_hdr) { hipFree(b->d_hdr); b->d_hdr = nullptr; }
+  if (b->d_pv) { hipFree(b->d_pv); b->d_pv = nullptr; }
+  b->scratch_bytes = b->hdr_count = b->pv_count = 0;
+}
+
+int tomo_batch_create(tomo_batch **out) {
+  if (!out) return fail(TOMO_E_INVALID_ARG, "null output handle");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(TOMO_E_NO_DEVICE, "no HIP device");
+  tomo_batch *b = new (std::nothrow) tomo_batch();
+  if (!b) return fail(TOMO_E_OOM, "batch allocation failed");
+  if (hipGetDevice(&b->device) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+    tomo_batch_destroy(b);
+    return fail(TOMO_E_HIP, "stream/event creation failed");
+  }
+  *out = b;
+  return TOMO_OK;
+}
+
+int tomo_batch_add(tomo_batch *b, const float *f32, const int32_t *i32, int32_t grid, int32_t n_tri,
+                   const int32_t *tri, int32_t n_vtx, const float *vtx, const float *vtx_nrm, int32_t n_ypr,
+                   const float *ypr, int32_t *member) {
+  if (!b) return fail(TOMO_E_INVALID_ARG, "null batch");
+  if (n_ypr < 1) return fail(TOMO_E_INVALID_ARG, "a member needs at least one orientation");
+  batch_unshare(b);
+  tomo_handle *h = nullptr;
+  int rc = tomo_create(f32, i32, grid, &h);
+  if (rc) return rc;
+  rc = tomo_set_mesh(h, n_tri, tri, n_vtx, vtx, vtx_nrm);
+  if (!rc) rc = tomo_set_orientations(h, n_ypr, ypr);
+  if (rc) {
+    tomo_destroy(h);
+    return rc;
+  }
+  h->own_stream = h->stream;
+  b->members.push_back(h);
+  tomo_batch::Tune t;
+  t.batch = n_ypr < 2048 ? n_ypr : 2048;
+  b->tune.push_back(t);
+  if (member) *member = (int32_t)b->members.size() - 1;
+  return TOMO_OK;
+}
+
+static void apply_tune(tomo_handle *h, const tomo_batch::Tune &t) {
+  h->tri_per_thread = t.tri_per_thread;
+  h->batch_req = t.batch;
+  if (h->lay.inl != t.inl) h->lay = make_layout(h->opt.grid, t.inl);
+}
+
+// Time one launch shape on member h over orientations [b0, b0 + n): microseconds per
+// orientation, or a negative value if the shape cannot run or leaves a slot unresolved.
+static float time_shape(tomo_handle *h, const tomo_batch::Tune &t, int32_t b0, int32_t n) {
+  apply_tune(h, t);
+  if (tomo_run(h, b0, b0 + n) != TOMO_OK || tomo_sync(h) != TOMO_OK) return -1.0f;  // warm-up
+  if (tomo_run(h, b0, b0 + n) != TOMO_OK || tomo_sync(h) != TOMO_OK) return -1.0f;
+  const float ms = tomo_last_kernel_ms(h);
+  std::vector<DevRecord> d((size_t)n);
+  if (hipMemcpy(d.data(), h->d_rec + b0, sizeof(DevRecord) * (size_t)n, hipMemcpyDeviceToHost) != hipSuccess)
+    return -1.0f;
+  for (const DevRecord &r : d)
+    if (r.flags & TOMO_F_UNRESOLVED) return -1.0f;
+  return ms < 0.0f ? -1.0f : ms * 1000.0f / (float)n;
+}
+
+// Autotuner.  Members are grouped by (grid, ceil(log2(nTri))); each group's largest member is
+// timed on `sample` orientations from the middle of its list with a coordinate descent over
+// triangles per rasteriser thread {1, 2, 4, 8}, inline slot capacity {8, 16, 24} and
+// orientations in flight {1024, 2048, 4096} (within a quarter of free memory); shapes that leave
+// a slot unresolved are rejected.  The group's winner applies to all its members.
+int tomo_batch_tune(tomo_batch *b, int32_t sample) {
+  if (!b) return fail(TOMO_E_INVALID_ARG, "null batch");
+  if (sample < 256) sample = 256;
+  batch_unshare(b);
+  const size_t M = b->members.size();
+  std::vector<int64_t> key(M);
+  for (size_t m = 0; m < M; m++) {
+    const tomo_handle *h = b->members[m];
+    int lg = 0;
+    while ((1 << lg) < (h->n_tri > 1 ? h->n_tri : 1)) lg++;
+    key[m] = (int64_t)h->opt.grid * 64 + lg;
+  }
+  std::vector<bool> done(M, false);
+  for (size_t m0 = 0; m0 < M; m0++) {
+    if (done[m0]) continue;
+    size_t rep = m0;
+    for (size_t m = m0; m < M; m++)
+      if (key[m] == key[m0] && b->members[m]->n_tri > b->members[rep]->n_tri) rep = m;
+    tomo_handle *h = b->members[rep];
+    const int32_t n = h->n_orient;
+    const int32_t ns = n < sample ? n : sample;
+    const int32_t b0 = (n - ns) / 2;
+    tomo_batch::Tune best;
+    best.batch = ns < 2048 ? ns : 2048;
+    float best_us = time_shape(h, best, b0, ns);
+    auto consider = [&](tomo_batch::Tune t) {
+      const float us = time_shape(h, t, b0, ns);
+      if (us > 0.0f && (best_us < 0.0f || us < best_us)) {
+        best_us = us;
+        best = t;
+      }
+    };
+    for (int32_t tpt : {1, 2, 8}) {
+      tomo_batch::Tune t = best;
+      t.tri_per_thread = tpt;
+      consider(t);
+    }
+    for (int32_t inl : {8, 24}) {
+      tomo_batch::Tune t = best;
+      t.inl = inl;
+      consider(t);
+    }
+    size_t freeb = 0, total = 0;
+    hipMemGetInfo(&freeb, &total);
+    for (int32_t bs : {1024, 4096}) {
+      tomo_batch::Tune t = best;
+      t.batch = bs < ns ? bs : ns;
+      const size_t need = (size_t)make_layout(h->opt.grid, t.inl).bytes * (size_t)t.batch;
+      if (t.batch != best.batch && need < freeb / 4) consider(t);
+    }
+    release_private_scratch(h);
+    if (best_us < 0.0f) return fail(TOMO_E_STATE, "no launch shape ran for a batch member");
+    best.us_per_orientation = best_us;
+    best.tuned = 1;
+    for (size_t m = m0; m < M; m++)
+      if (key[m] == key[m0]) {
+        tomo_batch::Tune t = best;
+        const int32_t nm = b->members[m]->n_orient;
+        if (t.batch > nm) t.batch = nm;
+        b->tune[m] = t;
+        done[m] = true;
+      }
+  }
+  return TOMO_OK;
+}
+
+int tomo_batch_get_tuning(tomo_batch *b, int32_t member, int32_t *batch, int32_t *tri_per_thread, int32_t *inl,
+                          float *us_per_orientation) {
+  if (!b || member < 0 || member >= (int32_t)b->members.size()) return fail(TOMO_E_INVALID_ARG, "bad member");
+  const tomo_batch::Tune &t = b->tune[(size_t)member];
+  if (batch) *batch = t.batch;
+  if (tri_per_thread) *tri_per_thread = t.tri_per_thread;
+  if (inl) *inl = t.inl;
+  if (us_per_orientation) *us_per_orientation = t.us_per_orientation;
+  return TOMO_OK;
+}
+
+// Record every member's sweep (launch shapes from the tuner, or the defaults) into one graph.
+int tomo_batch_capture(tomo_batch *b) {
+  if (!b) return fail(TOMO_E_INVALID_ARG, "null batch");
+  if (b->members.empty()) return fail(TOMO_E_STATE, "empty batch");
+  batch_unshare(b);
+  size_t sbytes = 0, hcount = 0, pcount = 0;
+  for (size_t m = 0; m < b->members.size(); m++) {
+    tomo_handle *h = b->members[m];
+    release_private_scratch(h);
+    apply_tune(h, b->tune[m]);
+    const size_t bs = (size_t)b->tune[m].batch;
+    sbytes = std::max(sbytes, (size_t)h->lay.bytes * bs);
+    hcount = std::max(hcount, bs);
+    pcount = std::max(pcount, bs * (size_t)(h->n_vtx > 0 ? h->n_vtx : 1));
+  }
+  if (hipMalloc(&b->d_scratch, sbytes) != hipSuccess) {
+    b->d_scratch = nullptr;
+    return fail(TOMO_E_OOM, "shared scratch allocation failed");
+  }
+  if (hipMalloc(&b->d_hdr, sizeof(OrientHdr) * hcount) != hipSuccess ||
+      hipMalloc(&b->d_pv, sizeof(float4) * pcount) != hipSuccess) {
+    batch_unshare(b);
+    return fail(TOMO_E_OOM, "shared buffer allocation failed");
+  }
+  b->scratch_bytes = sbytes;
+  b->hdr_count = hcount;
+  b->pv_count = pcount;
+  HIP_TRY(hipMemsetAsync(b->d_scratch, 0, sbytes, b->stream));
+  HIP_TRY(hipMemsetAsync(b->d_hdr, 0, sizeof(OrientHdr) * hcount, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  for (size_t m = 0; m < b->members.size(); m++) {
+    tomo_handle *h = b->members[m];
+    h->shared = true;
+    h->stream = b->stream;
+    h->d_scratch = b->d_scratch;
+    h->d_hdr = b->d_hdr;
+    h->d_pv = b->d_pv;
+    h->scratch_batch = b->tune[m].batch;
+    h->scratch_region = h->lay.bytes;
+    h->pv_cap = pcount;
+    h->phases = false;
+    free_stage_events(h);
+  }
+  HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  int rc = TOMO_OK;
+  for (size_t m = 0; m < b->members.size() && rc == TOMO_OK; m++) {
+    tomo_handle *h = b->members[m];
+    const int32_t bs = b->tune[m].batch;
+    SweepParams p = base_params(h);
+    p.n_batch = std::min(bs, h->n_orient);
+    const hipError_t e = launch_stage(6, p, b->stream);
+    if (e != hipSuccess) rc = fail(TOMO_E_HIP, std::string("acquire launch: ") + hipGetErrorString(e));
+    if (rc == TOMO_OK) rc = enqueue(h, 0, h->n_orient, bs, nullptr, 0, nullptr);
+  }
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(b->stream, &g);
+  if (rc) {
+    if (g) hipGraphDestroy(g);
+    return rc;
+  }
+  if (ec != hipSuccess) return fail(TOMO_E_HIP, std::string("graph capture: ") + hipGetErrorString(ec));
+  b->graph = g;
+  HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  for (tomo_handle *h : b->members) {
+    h->last_begin = 0;
+    h->last_end = h->n_orient;
+    h->timed = false;
+  }
+  return TOMO_OK;
+}
+
+int tomo_batch_launch(tomo_batch *b) {
+  if (!b) return fail(TOMO_E_INVALID_ARG, "null batch");
+  if (!b->exec) {
+    int rc = tomo_batch_capture(b);
+    if (rc) return rc;
+  }
+  HIP_TRY(hipEventRecord(b->ev0, b->stream));
+  HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+  HIP_TRY(hipEventRecord(b->ev1, b->stream));
+  b->timed = true;
+  return TOMO_OK;
+}
+
+int tomo_batch_sync(tomo_batch *b) {
+  if (!b) return fail(TOMO_E_INVALID_ARG, "null batch");
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return TOMO_OK;
+}
+
+float tomo_batch_last_ms(tomo_batch *b) {
+  if (!b || !b->timed) return -1.0f;
+  float ms = -1.0f;
+  if (hipEventSynchronize(b->ev1) != hipSuccess) return -1.0f;
+  if (hipEventElapsedTime(&ms, b->ev0, b->ev1) != hipSuccess) return -1.0f;
+  return ms;
+}
+
+int32_t tomo_batch_size(tomo_batch *b) { return b ? (int32_t)b->members.size() : 0; }
+
+int tomo_batch_records(tomo_batch *b, int32_t member, tomo_record *out) {
+  if (!b || member < 0 || member >= (int32_t)b->members.size()) return fail(TOMO_E_INVALID_ARG, "bad member");
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  tomo_handle *h = b->members[(size_t)member];
+  return tomo_get_records(h, 0, h->n_orient, out);
+}
+
+int tomo_batch_destroy(tomo_batch *b) {
+  if (!b) return TOMO_OK;
+  batch_unshare(b);
+  for (tomo_handle *h : b->members) tomo_destroy(h);
+  if (b->ev0) hipEventDestroy(b->ev0);
+  if (b->ev1) hipEventDestroy(b->ev1);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
   return TOMO_OK;
 }
 

@@ -12,7 +12,8 @@ namespace tomo {
 constexpr int kThreads = 256;          // work-group size: 4 waves of 64
 constexpr int kWaves = kThreads / 64;
 constexpr int kSmall = 8;              // slots with <= 8 entries are paired in registers
-constexpr int kInline = 16;            // entries stored inline per slot (layer-major)
+constexpr int kInlineDefault = 16;     // entries stored inline per slot (layer-major); tunable
+constexpr int kInlineMax = 32;         // in [kSmall, kInlineMax]
 constexpr int kBigCap = 512;           // entries a single slot may hold for exact resolution
 constexpr int kMaxBedR = 16;           // int(BedOuterBound) supported by the bed kernel
 constexpr int kTriPerThread = 4;       // rasteriser: default triangles per thread per work-group
@@ -38,7 +39,7 @@ struct DevRecord {
 struct OrientHdr {
   int32_t bbox[4];       // touched slots: x0, x1, y0, y1
   uint32_t big_top;      // slots with more than kSmall entries
-  uint32_t pool_top;     // entries beyond kInline
+  uint32_t pool_top;     // entries beyond the inline layers
   int32_t flags;
   float kx, ky, kz;
   unsigned long long zabs;  // sum of |z| over emitted entries: bounds every sum of |slot value|
@@ -51,6 +52,7 @@ struct ScratchLayout {
       cmp_rank, sv;  // byte offsets inside one orientation's region
   int64_t bytes; // bytes per region
   int32_t pool;  // overflow pool capacity (entries)
+  int32_t inl;   // inline entries per slot
 };
 
 struct SweepParams {

@@ -57,12 +57,13 @@ __device__ __forceinline__ T wave_sum(T v) {
 __device__ __forceinline__ int msb64(uint64_t m) { return 63 - __clzll((long long)m); }
 
 struct Tables {
-  uint2 *ent;  // [kInline][G2] {key, rank}, layer-major
+  uint2 *ent;  // [inl][G2] {key, rank}, layer-major
   uint32_t *count, *bedinfo, *off, *cursor, *pool_slot, *pool_key, *pool_rank,
       *cmp_key, *cmp_rank;
   int32_t *big;
   int32_t *sv;  // [6][G2] per-slot Vo, Vss, Va, Vb, Vnv, Vtc (kept only when totals may round)
   int32_t pool;
+  int32_t inl;  // inline entries per slot (>= kSmall)
 };
 
 struct Acc {
@@ -137,7 +138,7 @@ struct Emit {
     bx0 = min(bx0, sx); bx1 = max(bx1, sx);
     by0 = min(by0, sy); by1 = max(by1, sy);
     uint32_t k = atomicAdd(&t.count[s], 1u);
-    if (k < (uint32_t)kInline) {
+    if (k < (uint32_t)t.inl) {
       t.ent[(int64_t)k * G2 + s] = make_uint2(key, rank);
     } else {
       uint32_t j = atomicAdd(pool_top, 1u);
@@ -265,18 +266,18 @@ __device__ void big_slot(int32_t s, uint32_t c, const Tables &t, int32_t G2, uin
     fl |= F_UNRESOLVED;
     c = kBigCap;
   }
-  const uint32_t off = (c > (uint32_t)kInline) ? t.off[s] : 0u;
+  const uint32_t off = (c > (uint32_t)t.inl) ? t.off[s] : 0u;
   const int P = pow2_at_least((int)c);
   for (int i = lane; i < P; i += 64) {
     uint64_t v = 0;
     if (i < (int)c) {
       uint32_t key, rank;
-      if (i < kInline) {
+      if (i < t.inl) {
         const uint2 e = t.ent[(int64_t)i * G2 + s];
         key = e.x;
         rank = e.y;
       } else {
-        uint32_t j = off + (uint32_t)(i - kInline);
+        uint32_t j = off + (uint32_t)(i - t.inl);
         if (j < (uint32_t)t.pool) {
           key = t.cmp_key[j];
           rank = t.cmp_rank[j];
@@ -346,8 +347,8 @@ __device__ void mid_slot(int32_t s, uint32_t c, const Tables &t, int32_t G2, int
                          const PairParams &pp, SlotResult &r, uint64_t *rows) {
   uint32_t key = 0u;
   if (lane < (int)c) {
-    const uint32_t off = (c > (uint32_t)kInline) ? t.off[s] : 0u;
-    key = (lane < kInline) ? t.ent[(int64_t)lane * G2 + s].x : t.cmp_key[off + (uint32_t)(lane - kInline)];
+    const uint32_t off = (c > (uint32_t)t.inl) ? t.off[s] : 0u;
+    key = (lane < t.inl) ? t.ent[(int64_t)lane * G2 + s].x : t.cmp_key[off + (uint32_t)(lane - t.inl)];
   }
 #pragma unroll
   for (int size = 2; size <= 64; size <<= 1) {
@@ -390,6 +391,7 @@ __device__ __forceinline__ Tables tables_for(const SweepParams &p, int32_t b) {
   t.cmp_rank = (uint32_t *)(base + p.lay.cmp_rank);
   t.sv = (int32_t *)(base + p.lay.sv);
   t.pool = p.lay.pool;
+  t.inl = p.lay.inl;
   return t;
 }
 
@@ -697,7 +699,7 @@ __global__ __launch_bounds__(kThreads) void compact_kernel(SweepParams p) {
     if (j < nbig) {
       s = t.big[j];
       const uint32_t c = t.count[s];
-      ex = c > (uint32_t)kInline ? c - (uint32_t)kInline : 0u;
+      ex = c > (uint32_t)t.inl ? c - (uint32_t)t.inl : 0u;
     }
     uint32_t incl = ex;
 #pragma unroll
@@ -796,7 +798,7 @@ __global__ __launch_bounds__(kThreads) void pair_kernel(SweepParams p) {
     const int32_t s = t.big[j];
     const uint32_t c = t.count[s];
     SlotResult r;
-    if (c < 64u && (c <= (uint32_t)kInline || npool <= (uint32_t)t.pool))
+    if (c < 64u && (c <= (uint32_t)t.inl || npool <= (uint32_t)t.pool))
       mid_slot(s, c, t, G2, lane, p.pp, r, p.dbg_rows ? p.dbg_rows + (int64_t)s * 64 : nullptr);
     else
       big_slot(s, c, t, G2, sortbuf[wave], lane, p.pp, r, p.dbg_rows ? p.dbg_rows + (int64_t)s * 64 : nullptr);
@@ -974,6 +976,21 @@ __global__ __launch_bounds__(64) void fsum_kernel(SweepParams p) {
   if (lane == 0) R.fsum_done = 1;
 }
 
+// ---- acquire_kernel: one work-group per orientation region ---------------------------------
+// The sweep kernels expect every slot counter of a region at zero (they restore that
+// themselves between orientations of one layout).  When regions shared by several meshes are
+// laid out for a new grid or inline capacity, the new counter arrays sit on the previous
+// layout's data: zero them, and the large-slot list that aabb_kernel would otherwise replay.
+__global__ __launch_bounds__(kThreads) void acquire_kernel(SweepParams p) {
+  const int32_t b = blockIdx.x;
+  const Tables t = tables_for(p, b);
+  const int32_t G2 = p.grid * p.grid;
+  uint4 *c4 = reinterpret_cast<uint4 *>(t.count);  // count is 256-byte aligned
+  for (int32_t i = threadIdx.x; i < G2 / 4; i += kThreads) c4[i] = make_uint4(0u, 0u, 0u, 0u);
+  for (int32_t i = (G2 / 4) * 4 + threadIdx.x; i < G2; i += kThreads) t.count[i] = 0u;
+  if (threadIdx.x == 0) p.hdr[b].big_top = 0u;
+}
+
 hipError_t launch_stage(int stage, const SweepParams &p, hipStream_t stream) {
   const int32_t nb = p.n_batch;
   const int32_t nb8 = (nb + 7) / 8 * 8;  // xcd_map pads the orientation count to whole XCD rounds
@@ -987,6 +1004,9 @@ hipError_t launch_stage(int stage, const SweepParams &p, hipStream_t stream) {
       break;
     case 5:
       if (p.exact_fsum) hipLaunchKernelGGL(fsum_kernel, dim3(nb), dim3(64), 0, stream, p);
+      break;
+    case 6:  // n_batch = regions to clear for this layout
+      hipLaunchKernelGGL(acquire_kernel, dim3(nb), dim3(kThreads), 0, stream, p);
       break;
     default: return hipErrorInvalidValue;
   }

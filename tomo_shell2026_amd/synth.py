@@ -364,3 +364,72 @@ def manikin_shell(seed=7, spacing=None, wall=2.0, open_mm=15.0, target_tris=8700
         if best is None or abs(len(f) - target_tris) < abs(len(best[1]) - target_tris):
             best = (v, f, hh)
     return best[0], best[1]
+
+
+# ---------------------------------------------------------------- batch of uneven meshes
+def displaced_sphere_n(n_tri_target, radius, seed):
+    """Displaced lat-long sphere with about n_tri_target triangles (same construction as the
+    69,120-triangle one: longitudes : bands ~ 5 : 3, SH l <= 8, sigma = 4 mm x radius / 60)."""
+    nr = max(2, int(round(math.sqrt(n_tri_target / (2 * 1.655)))))
+    n_lon = max(3, int(round(n_tri_target / (2 * nr))))
+    return displaced_latlong_sphere(n_lon, nr + 1, radius, 8, 4.0 * radius / 60.0, seed)
+
+
+def cut_by_planes(v, f, n_parts=6, lo=2000, hi=30000, seed=5):
+    """Split a mesh into n_parts open parts by seeded planes: repeatedly cut the largest part by
+    a random plane through its triangle-centroid median (triangles go whole to one side).
+    Retries until every part has lo..hi triangles."""
+    rng = np.random.default_rng(seed)
+    cen_all = v[f].mean(1)
+    for _ in range(500):
+        parts = [np.arange(len(f))]
+        while len(parts) < n_parts:
+            k = int(np.argmax([len(p) for p in parts]))
+            idx = parts.pop(k)
+            n = rng.normal(size=3)
+            n /= np.linalg.norm(n)
+            proj = cen_all[idx] @ n
+            d = np.median(proj) + rng.normal(0.0, 0.2) * proj.std()
+            side = proj > d
+            parts += [idx[side], idx[~side]]
+        if all(lo <= len(p) <= hi for p in parts):
+            out = []
+            for p in sorted(parts, key=len, reverse=True):
+                pf = f[p]
+                used, inv = np.unique(pf, return_inverse=True)
+                out.append((v[used], inv.reshape(-1, 3)))
+            return out
+    raise RuntimeError("no plane cut met the part-size bounds")
+
+
+def member_workload(name, v_mm, f, shell, ypr=None):
+    """Scale a mesh (mm) to a voxel size of 0.5..2 mm chosen from its size (diameter / 200),
+    and pick the smallest grid of 128 / 256 / 512 that holds it in every orientation."""
+    c = 0.5 * (v_mm.min(0) + v_mm.max(0))
+    diam = 2.0 * np.linalg.norm(v_mm - c, axis=1).max()
+    voxel = float(np.clip(diam / 200.0, 0.5, 2.0))
+    ext = diam / voxel + 4.0
+    grid = 128 if ext <= 124 else (256 if ext <= 252 else 512)
+    v, s = fit_to_grid(v_mm / voxel, grid)
+    note = "" if s == 1.0 else f"scaled by {s:.4f} to fit the {grid} grid"
+    if ypr is None:
+        ypr = ypr_grid(np.arange(0, 360, 1), np.arange(0, 61, 1))
+    return Workload(name, v, f, grid, ypr, voxel, note, shell=shell)
+
+
+def batch70_workloads(seed=3, ypr=None):
+    """70 meshes of uneven size: the thin-shell manikin cut by seeded planes into 6 open parts of
+    2k-30k triangles (shell mode), and 64 closed displaced spheres with triangle counts
+    log-uniform in 1k-100k and radii uniform in 20-80 mm.  Each has its own voxel size and grid
+    (member_workload) and, by default, the 1 degree yaw x 0..60 pitch grid (21,960)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    v, f = manikin_shell(seed=7)
+    for i, (pv, pf) in enumerate(cut_by_planes(v, f, 6, 2000, 30000, seed=seed + 2)):
+        out.append(member_workload(f"manikin_part{i}_{len(pf)}", pv, pf, shell=True, ypr=ypr))
+    for k in range(64):
+        n = int(round(math.exp(rng.uniform(math.log(1000.0), math.log(100000.0)))))
+        radius = float(rng.uniform(20.0, 80.0))
+        sv, sf = displaced_sphere_n(n, radius, seed=100 + k)
+        out.append(member_workload(f"dsphere{k}_{len(sf)}", sv, sf, shell=False, ypr=ypr))
+    return out
