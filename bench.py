@@ -184,6 +184,8 @@ def main():
                     help="dsphere: 69,120-triangle displaced sphere, 256 grid (the headline workload); "
                          "manikin: ~87k-triangle thin-shell manikin, 512 grid, shell mode; "
                          "batch70: 70 uneven meshes in one tuned, captured graph launch")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the 70-mesh batched graph launch measurement")
     ap.add_argument("--no-refine", action="store_true",
                     help="skip the dense 0.1 degree grid vs coarse + gradient-refined search comparison")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -285,6 +287,11 @@ def main():
     s.close()
     if not args.no_refine:
         out["refine"] = refine_comparison(w)
+    if not args.no_batch and args.workload == "dsphere":
+        b = bench_batch70(argparse.Namespace(steps=2, warmup=1))
+        out["batch70"] = {k: b[k] for k in ("value", "unit", "config", "graph_ms", "plain_per_mesh_ms",
+                                            "graph_vs_plain_speedup", "tune_s", "capture_s", "tuned_shapes",
+                                            "flags")}
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(w, rec, args.cpu_stride, threads)
