@@ -111,3 +111,38 @@ def test_run_dll_matches_oracle(oracle_mod):
         else:
             np.testing.assert_array_equal(got[t], exp)
     assert len(want[5]) > 0
+
+
+@pytest.mark.gpu
+def test_manager_batched_group_matches_per_part():
+    """TomoManager.get_best_orientation with all parts in one graph launch gives the same
+    TomoObjects (Mss per orientation, optimum, render groups, placement matrix) as one run_dll
+    per part, the reference's flow."""
+    from tomo_shell2026_amd.cfms_tomo import TomoManager
+    print("[gpu-case] TomoManager batched group of 3 parts", flush=True)
+    so = SlicingOptions()
+    so.theta_YP = 30
+    so.bVerbose = False
+    v1, f1 = synth.icosphere(3, 30.0)
+    v2, f2 = synth.displaced_sphere_n(5000, 40.0, seed=7)
+    v3, f3 = synth.icosphere(2, 20.0)
+    meshes = [TomoMesh(v1 + 100.0, f1), TomoMesh(v2 + 90.0, f2), TomoMesh(v3 + 60.0, f3)]
+    group_a = _Group([_Part(m, f"p{i}") for i, m in enumerate(meshes)])
+    group_b = _Group([_Part(m, f"p{i}") for i, m in enumerate(meshes)])
+    M = TomoManager(so)
+    M.get_best_orientation(group_a, batched=True)
+    batched = [(c.tomo_object, M.CppDLL.Mss3D) for c in group_a.cutobjects]
+    per_part = []
+    for c in group_b.cutobjects:
+        t_o = M.CppDLL.run_dll(c, False)
+        per_part.append((t_o, M.CppDLL.Mss3D.copy()))
+    for (a, _), (b, mss_b) in zip(batched, per_part):
+        np.testing.assert_array_equal(a.mat4x4.view(np.uint32), b.mat4x4.view(np.uint32))
+        assert a.best_vminfo.GetMss() == b.best_vminfo.GetMss()
+        assert (a.best_vminfo.vmData.yaw, a.best_vminfo.vmData.pitch) == (b.best_vminfo.vmData.yaw,
+                                                                        b.best_vminfo.vmData.pitch)
+        ga = {g.type: g.pxls for g in a.pxl_groups}
+        gb = {g.type: g.pxls for g in b.pxl_groups}
+        assert ga.keys() == gb.keys()
+        for k in ga:
+            np.testing.assert_array_equal(ga[k], gb[k])

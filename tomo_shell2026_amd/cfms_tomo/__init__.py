@@ -17,10 +17,16 @@ class TomoManager:
         self.slicing_option = s_o
         self.CppDLL = TomoDLLInterface("libtomo_hip.so", s_o, s_o.bUseCUDA, s_o.bShellMesh)
 
-    def get_best_orientation(self, C_G):
-        """C_G: an object with .name and .cutobjects (each with .tmesh and .name)."""
+    def get_best_orientation(self, C_G, batched=True):
+        """C_G: an object with .name and .cutobjects (each with .tmesh and .name).  batched: all
+        parts' sweeps in one graph launch (run_group); otherwise one run_dll per part as the
+        reference does (cfms_tomo/__init__.py:26-30).  Both give the same TomoObjects."""
         self.name = f"{getattr(C_G, 'name', 'group')}_tomo"
         self.input_gr = C_G
+        if batched:
+            for c_o, t_o in zip(C_G.cutobjects, self.CppDLL.run_group(C_G.cutobjects, self.slicing_option.bVerbose)):
+                c_o.tomo_object = t_o
+            return
         for c_o in C_G.cutobjects:
             c_o.tomo_object = self.CppDLL.run_dll(c_o, self.slicing_option.bVerbose)
 

@@ -105,9 +105,55 @@ class TomoDLLInterface:
         if opt < 0:
             raise _lib.TomoError(f"TomoSh_HIP failed ({opt}): {_lib.last_error()}")
         self.Cdll_opt_id = int(opt)
+        Mss3D = np.array(Cptr1d_to_np(self.CppDLL.getMss(), self.YPR.shape[0])).astype(np.float32, copy=True)
+        return self._tomo_object(c_o, T, Mss3D, bVerbose)
+
+    def run_group(self, cutobjects, bVerbose):
+        """run_dll for every part of a cut group, with all parts' sweeps in one captured graph
+        launch (tomo_batch_*).  Per part, the optimum's render groups and placement matrix come
+        from a one-orientation TomoSh_HIP call at that orientation, which gives what the full call
+        gives for it (orientations are independent).  Returns one TomoObject per part."""
+        work = []
+        B = _lib.Batch()
+        try:
+            for c0 in cutobjects:
+                c_o = deepcopy(c0)
+                T = self._mesh_of(c_o)
+                self.bShellMesh = self.slicing_option.bShellMesh and (not T.is_watertight)
+                vtx, tri = self.pack(T, bVerbose)
+                vtx_nrm = np.asarray(T.vertex_normals).astype(np.float32, copy=True)
+                B.add(self.Float32Info, self.Int32Info, 0, tri, vtx, vtx_nrm, self.YPR)
+                work.append((c_o, T, vtx, tri, vtx_nrm, self.Float32Info.copy(), self.Int32Info.copy()))
+            B.launch()
+            B.sync()
+            mss = [B.records(m)["Mss"].astype(np.float32) for m in range(len(work))]
+        finally:
+            B.close()
+        out = []
+        empty_i = np.zeros(3, np.int32)
+        empty_f = np.zeros(3, np.float32)
+        for (c_o, T, vtx, tri, vtx_nrm, f32, i32), Mss3D in zip(work, mss):
+            if bVerbose:
+                print("Tomo processing: ", getattr(c_o, "name", "mesh"))
+            opt = _lib.find_first_optimal(Mss3D)
+            one = np.ascontiguousarray(self.YPR[opt:opt + 1])
+            i1 = i32.copy()
+            i1[6] = 1
+            rc = self.CppFunction(
+                np_to_Cptr1d(f32), np_to_Cptr1iL(i1), np_to_Cptr1d(one), np_to_Cptr1iL(tri), np_to_Cptr1d(vtx),
+                np_to_Cptr1d(vtx_nrm), np_to_Cptr1d(np.zeros(3 * len(tri), np.float32)),
+                np_to_Cptr1iL(empty_i), np_to_Cptr1d(empty_f), np_to_Cptr1d(empty_f))
+            if rc < 0:
+                raise _lib.TomoError(f"TomoSh_HIP failed ({rc}): {_lib.last_error()}")
+            self.Float32Info, self.Int32Info = f32, i32
+            self.Cdll_opt_id = int(opt)
+            out.append(self._tomo_object(c_o, T, Mss3D, bVerbose))
+        return out
+
+    def _tomo_object(self, c_o, T, Mss3D, bVerbose):
+        """TomoObject from per-orientation Mss and the optimum's getters (tomo_dll_interface.py:99-135)."""
         L = self.CppDLL
         t_o = TomoObject(c_o)
-        Mss3D = np.array(Cptr1d_to_np(L.getMss(), self.YPR.shape[0])).astype(np.float32, copy=True)
         if os.environ.get("TOMO_DEBUG_MSS", "0").lower() in ("1", "true", "yes", "on"):
             best_ids = np.argsort(Mss3D)[:8]
             print("TOMO_DEBUG_MSS", "min", float(np.min(Mss3D)), "max", float(np.max(Mss3D)),
