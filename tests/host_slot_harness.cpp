@@ -29,6 +29,18 @@ int harness_pair63(const uint32_t *keys, int c, int cs, int cn, int ex, int bt, 
   if (c < 0 || c > 63) return -1;
   return run<63>(keys, c, cs, cn, ex, bt, br, out);
 }
+int harness_pair16(const uint32_t *keys, int c, int cs, int cn, int ex, int bt, int br, int32_t *out) {
+  if (c < 0 || c > 16) return -1;
+  return run<16>(keys, c, cs, cn, ex, bt, br, out);
+}
+int harness_pair4(const uint32_t *keys, int c, int cs, int cn, int ex, int bt, int br, int32_t *out) {
+  if (c < 0 || c > 4) return -1;
+  return run<4>(keys, c, cs, cn, ex, bt, br, out);
+}
+int harness_pair2(const uint32_t *keys, int c, int cs, int cn, int ex, int bt, int br, int32_t *out) {
+  if (c < 0 || c > 2) return -1;
+  return run<2>(keys, c, cs, cn, ex, bt, br, out);
+}
 uint32_t harness_make_key(int z, int nz) { return make_key(z, nz); }
 
 // Coverage test of the rasteriser for n pixel centres against one margined triangle:

@@ -24,7 +24,7 @@ def harness():
     subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-o", so, src],
                    check=True)
     L = ct.CDLL(so)
-    for name in ("harness_pair8", "harness_pair63"):
+    for name in ("harness_pair2", "harness_pair4", "harness_pair8", "harness_pair16", "harness_pair63"):
         fn = getattr(L, name)
         fn.argtypes = (ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_void_p)
         fn.restype = ct.c_int
@@ -36,7 +36,7 @@ def harness():
 def hpair(L, keys, n_max, cs, cn, explicit, bed_type=0, bed_r=2):
     keys = np.ascontiguousarray(keys, np.uint32)
     out = np.zeros(14 + n_max, np.int32)
-    fn = L.harness_pair8 if n_max == 8 else L.harness_pair63
+    fn = {2: L.harness_pair2, 4: L.harness_pair4, 8: L.harness_pair8, 16: L.harness_pair16}.get(n_max, L.harness_pair63)
     rc = fn(keys.ctypes.data_as(ct.c_void_p), keys.size, cs, cn, int(explicit), bed_type, bed_r,
             out.ctypes.data_as(ct.c_void_p))
     assert rc == 0
@@ -47,13 +47,14 @@ def key(z, nz):
     return ((z + 32768) << 16) | (nz + 32768)
 
 
-def test_sort_network_zero_one(harness):
-    # 0-1 principle: the 19-comparator network sorts every binary input of length 8
+@pytest.mark.parametrize("n", [2, 4, 8, 16])
+def test_sort_network_zero_one(harness, n):
+    # 0-1 principle: the 1-, 5-, 19- and 80-comparator networks sort every binary input of length n
     lo, hi = key(3, 5), key(9, -5)
-    for m in range(256):
-        ks = [hi if (m >> i) & 1 else lo for i in range(8)]
-        out = hpair(harness, ks, 8, -867, -866, True)
-        srt = out[14:22].astype(np.int64) & 0xffffffff
+    for m in range(1 << n):
+        ks = [hi if (m >> i) & 1 else lo for i in range(n)]
+        out = hpair(harness, ks, n, -867, -866, True)
+        srt = out[14:14 + n].astype(np.int64) & 0xffffffff
         assert list(srt) == sorted(srt, reverse=True)
 
 
@@ -66,7 +67,7 @@ def test_known_answer_slot_product(harness):
         assert (out[0], out[1]) == (6, 3)
 
 
-@pytest.mark.parametrize("nmax", [8, 63])
+@pytest.mark.parametrize("nmax", [2, 4, 8, 16, 63])
 @pytest.mark.parametrize("explicit", [True, False])
 def test_pairing_matches_oracle_random(harness, oracle_mod, nmax, explicit):
     rng = np.random.default_rng(11 + nmax + int(explicit))

@@ -778,11 +778,28 @@ __global__ __launch_bounds__(kThreads) void pair_kernel(SweepParams p) {
     if (c > (uint32_t)kSmall) continue;  // a wave pairs it below
     t.count[s] = 0u;
     acc.nslot++;
-    uint32_t k[kSmall];
-#pragma unroll
-    for (int q = 0; q < kSmall; q++) k[q] = (q < (int)c) ? t.ent[(int64_t)q * G2 + s].x : 0u;
     SlotResult r;
-    pair_keys<kSmall>(k, (int)c, p.pp, r, p.dbg_rows ? p.dbg_rows + (int64_t)s * 64 : nullptr);
+    uint64_t *const rows = p.dbg_rows ? p.dbg_rows + (int64_t)s * 64 : nullptr;
+    // Most slots hold two entries (one surface above, one below): size the register network to
+    // the largest slot among this wave's active lanes.
+    if (__ballot(c > 4u) == 0ull) {
+      if (__ballot(c > 2u) == 0ull) {
+        uint32_t k[2];
+#pragma unroll
+        for (int q = 0; q < 2; q++) k[q] = (q < (int)c) ? t.ent[(int64_t)q * G2 + s].x : 0u;
+        pair_keys<2>(k, (int)c, p.pp, r, rows);
+      } else {
+        uint32_t k[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) k[q] = (q < (int)c) ? t.ent[(int64_t)q * G2 + s].x : 0u;
+        pair_keys<4>(k, (int)c, p.pp, r, rows);
+      }
+    } else {
+      uint32_t k[kSmall];
+#pragma unroll
+      for (int q = 0; q < kSmall; q++) k[q] = (q < (int)c) ? t.ent[(int64_t)q * G2 + s].x : 0u;
+      pair_keys<kSmall>(k, (int)c, p.pp, r, rows);
+    }
     account(acc, r, s, p);
     t.bedinfo[s] = bed_info(r, p.pp);
     if (sv) put_sv(sv, G2, s, r);
@@ -792,11 +809,34 @@ __global__ __launch_bounds__(kThreads) void pair_kernel(SweepParams p) {
       if (p.dbg_final) p.dbg_final[s] = r.n;
     }
   }
-  // Large slots: one wave each.  Their counters stay set until the next aabb_kernel on this
-  // scratch region (resetting them here would race with the small-slot scan of other groups).
+  // Large slots keep their counters set until the next aabb_kernel on this scratch region
+  // (resetting them here would race with the small-slot scan of other groups).
+  // Those with at most 16 entries, all inline: one lane each, 16-key register network.
+  const uint32_t lane_cap = t.inl < 16 ? (uint32_t)t.inl : 16u;
+  for (uint32_t j = (uint32_t)(tile * kThreads + tid); j < nbig; j += (uint32_t)(p.pair_tiles * kThreads)) {
+    const int32_t s = t.big[j];
+    const uint32_t c = t.count[s];
+    if (c > lane_cap) continue;
+    uint32_t k[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) k[q] = (q < (int)c) ? t.ent[(int64_t)q * G2 + s].x : 0u;
+    SlotResult r;
+    pair_keys<16>(k, (int)c, p.pp, r, p.dbg_rows ? p.dbg_rows + (int64_t)s * 64 : nullptr);
+    acc.nslot++;
+    account(acc, r, s, p);
+    t.bedinfo[s] = bed_info(r, p.pp);
+    if (sv) put_sv(sv, G2, s, r);
+    if (final_here) {
+      acc.nfinal += r.n;
+      acc.chkfinal += (uint64_t)(s + 1) * (uint64_t)r.n;
+      if (p.dbg_final) p.dbg_final[s] = r.n;
+    }
+  }
+  // Larger ones: one wave each.
   for (uint32_t j = (uint32_t)(tile * kWaves + wave); j < nbig; j += (uint32_t)nwaves) {
     const int32_t s = t.big[j];
     const uint32_t c = t.count[s];
+    if (c <= lane_cap) continue;  // paired by a lane above (uniform across the wave)
     SlotResult r;
     if (c < 64u && (c <= (uint32_t)t.inl || npool <= (uint32_t)t.pool))
       mid_slot(s, c, t, G2, lane, p.pp, r, p.dbg_rows ? p.dbg_rows + (int64_t)s * 64 : nullptr);

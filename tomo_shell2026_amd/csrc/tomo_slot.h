@@ -105,9 +105,44 @@ TOMO_HD uint64_t make_row(int32_t type, uint32_t key) { return ((uint64_t)(uint3
 template <int N>
 TOMO_HD void pair_keys(uint32_t (&k)[N], int c, const PairParams &pp, SlotResult &r,
                        uint64_t *rows = nullptr) {
-  // Sort descending.  N == 8 uses a 19-comparator network; other N an odd-even transposition
-  // network (used by the host unit test for larger N).
-  if constexpr (N == 8) {
+  // Sort descending: optimal networks for N = 2 (1 comparator), 4 (5) and 8 (19), a bitonic
+  // network for N = 16 (80); other N an odd-even transposition network (the host unit test's
+  // N = 63).
+  if constexpr (N == 2) {
+    const uint32_t hi = k[0] > k[1] ? k[0] : k[1];
+    const uint32_t lo = k[0] > k[1] ? k[1] : k[0];
+    k[0] = hi;
+    k[1] = lo;
+  } else if constexpr (N == 4) {
+#define TOMO_CX4(a, b)                                  \
+  {                                                     \
+    uint32_t hi = k[a] > k[b] ? k[a] : k[b];            \
+    uint32_t lo = k[a] > k[b] ? k[b] : k[a];            \
+    k[a] = hi;                                          \
+    k[b] = lo;                                          \
+  }
+    TOMO_CX4(0, 1) TOMO_CX4(2, 3) TOMO_CX4(0, 2) TOMO_CX4(1, 3) TOMO_CX4(1, 2)
+#undef TOMO_CX4
+  } else if constexpr (N == 16) {
+#pragma unroll
+    for (int size = 2; size <= N; size <<= 1) {
+#pragma unroll
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+          const int j = i ^ stride;
+          if (j > i) {
+            // runs of length `size` alternate direction; the final merge is descending
+            const bool desc = (i & size) == 0;
+            const uint32_t a = k[i], b = k[j];
+            const bool sw = desc ? (a < b) : (a > b);
+            k[i] = sw ? b : a;
+            k[j] = sw ? a : b;
+          }
+        }
+      }
+    }
+  } else if constexpr (N == 8) {
 #define TOMO_CX(a, b)                                   \
   {                                                     \
     uint32_t hi = k[a] > k[b] ? k[a] : k[b];            \
