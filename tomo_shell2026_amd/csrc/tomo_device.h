@@ -18,7 +18,7 @@ constexpr int kBigCap = 512;           // entries a single slot may hold for exa
 constexpr int kMaxBedR = 16;           // int(BedOuterBound) supported by the bed kernel
 constexpr int kTriPerThread = 4;       // rasteriser: default triangles per thread per work-group
 constexpr int kPairTile = 2048;        // pairing: touched-box slots per work-group
-constexpr int kBandRows = 4;           // bed: grid rows per work-group
+constexpr int kBandRows = 16;          // bed: grid rows per work-group
 
 // Device record, one per orientation.  Totals are exact 64-bit integers accumulated with
 // atomics by the pairing and bed kernels; the host turns them into the reference's float

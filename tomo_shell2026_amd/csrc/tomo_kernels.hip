@@ -24,7 +24,8 @@ namespace {
 
 constexpr float kNormFactor = 1000.0f;      // g_fNORMALFACTOR (Tomo_types.h:39)
 
-constexpr int kBedLdsBytes = 24 * 1024;  // bed band + halo flags ((4 + 2r) x (width + 2r))
+constexpr int kBedLdsBytes = 28 * 1024;  // bed band + halo flags ((16 + 2r) x (width + 2r))
+constexpr int kBedWords = 1024;          // halo source bit words ((16 + 2r) x ceil((width + 2r) / 32))
 
 // Work-group barrier that also orders the work-group's global-memory slot tables.  A slot
 // table is private to its work-group and all its waves share one CU (and its L1), so the
@@ -876,6 +877,9 @@ __global__ __launch_bounds__(kThreads) void pair_kernel(SweepParams p) {
 // source slot that takes its own bed row stops being a source for slots scanned after it.
 __global__ __launch_bounds__(kThreads) void bed_kernel(SweepParams p) {
   __shared__ uint8_t nb[kBedLdsBytes];
+  // Source bits of the halo rows (one bit per column, filled by ballots), then their OR over
+  // each column's half-open window [y - r, y + r): an exact "no source can be near" test.
+  __shared__ uint32_t srcw[kBedWords], rowany[kBedWords];
   __shared__ int64_t red[kWaves][3];
   __shared__ int32_t fred[kWaves];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -897,33 +901,67 @@ __global__ __launch_bounds__(kThreads) void bed_kernel(SweepParams p) {
   const int32_t hx0 = max(X0 - br, 0), hx1 = min(X1 + br, G);
   const int32_t hy0 = max(ry0 - br, 0), hy1 = min(ry1 + br, G);
   const int32_t hw = hy1 - hy0;
-  const bool lds = (int64_t)(hx1 - hx0) * hw <= kBedLdsBytes;
+  const int32_t nr = hx1 - hx0;
+  const int32_t nw = (hw + 31) >> 5;
+  const bool lds = (int64_t)nr * hw <= kBedLdsBytes;
+  const bool bits = lds && nr * nw <= kBedWords;
   if (lds) {
-    const int32_t nh = (hx1 - hx0) * hw;
-    for (int32_t i = tid; i < nh; i += kThreads) {
-      const int32_t x = hx0 + i / hw, y = hy0 + i % hw;
-      uint8_t f = 0;
-      if (x >= bx0 && x <= bx1 && y >= by0 && y <= by1) {
-        const uint32_t bi = t.bedinfo[x * G + y];
-        f = (uint8_t)(((bi & BI_SRC) ? 1u : 0u) | ((bi & BI_CONV) ? 2u : 0u));
+    for (int32_t r0 = wave; r0 < nr; r0 += kWaves) {
+      const int32_t x = hx0 + r0;
+      const bool xin = x >= bx0 && x <= bx1;
+      for (int32_t c0 = 0; c0 < hw; c0 += 64) {
+        const int32_t yc = c0 + lane, y = hy0 + yc;
+        uint8_t f = 0;
+        if (yc < hw && xin && y >= by0 && y <= by1) {
+          const uint32_t bi = t.bedinfo[x * G + y];
+          f = (uint8_t)(((bi & BI_SRC) ? 1u : 0u) | ((bi & BI_CONV) ? 2u : 0u));
+        }
+        if (yc < hw) nb[r0 * hw + yc] = f;
+        const uint64_t sm = __ballot((f & 1u) != 0u);
+        if (bits && lane == 0) {
+          const int32_t w0 = c0 >> 5;
+          srcw[r0 * nw + w0] = (uint32_t)sm;
+          if (w0 + 1 < nw) srcw[r0 * nw + w0 + 1] = (uint32_t)(sm >> 32);
+        }
       }
-      nb[i] = f;
+    }
+    __syncthreads();
+  }
+  if (bits) {
+    for (int32_t i = tid; i < nr * nw; i += kThreads) {
+      const int32_t w0 = i % nw;
+      const uint32_t cur = srcw[i];
+      const uint32_t prev = w0 > 0 ? srcw[i - 1] : 0u;
+      const uint32_t next = w0 + 1 < nw ? srcw[i + 1] : 0u;
+      uint32_t o = cur;  // d = 0
+      for (int32_t d = 1; d < br; d++) o |= (cur >> d) | (next << (32 - d));   // columns y + d
+      for (int32_t e = 1; e <= br; e++) o |= (cur << e) | (prev >> (32 - e));  // columns y - e
+      rowany[i] = o;
     }
     __syncthreads();
   }
   const int32_t rwy = ry1 - ry0;
-  const int32_t n = (X1 - X0) * rwy;
   int64_t nbed = 0, nfinal = 0;
   uint64_t chk = 0;
   int32_t flags = 0;
-  for (int32_t i = tid; i < n; i += kThreads) {
-    const int32_t X = X0 + i / rwy, Y = ry0 + i % rwy;
+  for (int32_t xr = wave; xr < X1 - X0; xr += kWaves)
+  for (int32_t c0 = 0; c0 < rwy; c0 += 64) {
+    if (c0 + lane >= rwy) continue;
+    const int32_t X = X0 + xr, Y = ry0 + c0 + lane;
     const int32_t P = X * G + Y;
     const bool inb = X >= bx0 && X <= bx1 && Y >= by0 && Y <= by1;
     const uint32_t bi = inb ? t.bedinfo[P] : BI_POSS;
     const int32_t nrow = (int32_t)(bi & 127u);
     int32_t add = 0;
-    if (bi & BI_POSS) {
+    bool near = true;
+    if ((bi & BI_POSS) && bits) {
+      const int32_t xa = max(X - br, 0), xb = min(X + br, G);
+      const int32_t yy = Y - hy0;
+      uint32_t any = 0u;
+      for (int32_t sx = xa; sx < xb; sx++) any |= rowany[(sx - hx0) * nw + (yy >> 5)];
+      near = ((any >> (yy & 31)) & 1u) != 0u;
+    }
+    if ((bi & BI_POSS) && near) {
       float md = 1e5f;
       const int32_t xa = max(X - br, 0), xb = min(X + br, G);
       const int32_t ya = max(Y - br, 0), yb = min(Y + br, G);
