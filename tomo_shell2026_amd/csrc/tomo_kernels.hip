@@ -155,6 +155,45 @@ struct Emit {
     }
     return s;
   }
+
+  // Entry k of slot s: inline layer or overflow pool.
+  __device__ __forceinline__ void place(int32_t s, uint32_t k, uint32_t key, uint32_t rank) {
+    if (k < (uint32_t)t.inl) {
+      t.ent[(int64_t)k * G2 + s] = make_uint2(key, rank);
+    } else {
+      uint32_t j = atomicAdd(pool_top, 1u);
+      if (j < (uint32_t)t.pool) {
+        t.pool_slot[j] = (uint32_t)s;
+        t.pool_key[j] = key;
+        t.pool_rank[j] = rank;
+      }
+    }
+  }
+
+  // A shell pixel and its sub-pixel (same slot, ranks r and r + 1) with one counter atomic.
+  __device__ __forceinline__ int32_t put2(float px, float py, float pz, float nz, float pz2, float nz2,
+                                          uint32_t rank) {
+    int32_t sx = (int32_t)px, sy = (int32_t)py;
+    if (!(px > -2147483648.0f && px < 2147483648.0f && py > -2147483648.0f && py < 2147483648.0f) ||
+        sx < 0 || sy < 0 || sx >= G || sy >= G) {
+      flags |= F_FOOTPRINT_OOB;
+      return -1;
+    }
+    const int32_t z1 = to_i16(pz, flags), n1 = to_i16(nz * kNormFactor, flags);
+    const int32_t z2 = to_i16(pz2, flags), n2 = to_i16(nz2 * kNormFactor, flags);
+    zabs += (uint64_t)(z1 < 0 ? -z1 : z1) + (uint64_t)(z2 < 0 ? -z2 : z2);
+    const int32_t s = sx * G + sy;
+    bx0 = min(bx0, sx); bx1 = max(bx1, sx);
+    by0 = min(by0, sy); by1 = max(by1, sy);
+    const uint32_t k = atomicAdd(&t.count[s], 2u);
+    place(s, k, make_key(z1, n1), rank);
+    place(s, k + 1u, make_key(z2, n2), rank + 1u);
+    if (k == (uint32_t)kSmall || k + 1u == (uint32_t)kSmall) {
+      uint32_t b = atomicAdd(big_top, 1u);
+      t.big[b] = s;
+    }
+    return s;
+  }
 };
 
 
@@ -601,14 +640,15 @@ __global__ __launch_bounds__(kThreads) void raster_kernel(SweepParams p) {
             float pz = rec[2] * u + rec[5] * v + rec[8] * w;
             float nz = rec[9] * u + rec[10] * v + rec[11] * w;
             const uint32_t rank = 2u * (uint32_t)(first + k * kThreads + wave * 64 + owner);
-            const int32_t s = em.put(px, py, pz, nz, rank);
+            int32_t s;
+            if (p.do_shell) {  // with its shell sub-pixel (STomoSh_INT3.cpp:1179-1185)
+              const float pz2 = pz + ((nz >= 0.0f) ? -p.shell_off : p.shell_off);
+              s = em.put2(px, py, pz, nz, pz2, -nz, rank);
+            } else {
+              s = em.put(px, py, pz, nz, rank);
+            }
             if (s >= 0 && s != px_i * G + py_i) em.flags |= F_RANK_TIE;
             atomicAdd(&L.hits[wave][owner], 1);
-            if (p.do_shell) {  // shell sub-pixel (STomoSh_INT3.cpp:1179-1185)
-              pz = pz + ((nz >= 0.0f) ? -p.shell_off : p.shell_off);
-              nz = -nz;
-              em.put(px, py, pz, nz, rank + 1u);
-            }
           }
         }
         wave_sync();
@@ -640,10 +680,11 @@ __global__ __launch_bounds__(kThreads) void raster_kernel(SweepParams p) {
       const float ty = (a[0][1] + a[1][1] + a[2][1]) * 0.333333f + kEps;
       const float tz = (a[0][2] + a[1][2] + a[2][2]) * 0.333333f + kEps;
       const uint32_t rank = 2u * (uint32_t)tr;
-      em.put(tx, ty, tz, nzv[0], rank);
       if (p.do_shell) {
         const float sz = tz + ((nzv[0] >= 0.0f) ? -p.shell_off : p.shell_off);
-        em.put(tx, ty, sz, -nzv[0], rank + 1u);
+        em.put2(tx, ty, tz, nzv[0], sz, -nzv[0], rank);
+      } else {
+        em.put(tx, ty, tz, nzv[0], rank);
       }
     }
     wave_sync();
@@ -1025,7 +1066,8 @@ __global__ __launch_bounds__(kThreads) void bed_kernel(SweepParams p) {
 // exactly representable integer and float(total) is that result; otherwise this kernel replays
 // the sums in the same order from the per-slot values pair_kernel kept.  Lane q < 6 owns total q.
 __global__ __launch_bounds__(64) void fsum_kernel(SweepParams p) {
-  __shared__ int32_t vals[6][64];
+  constexpr int kAhead = 8;                 // chunks of 64 slots in flight per lane
+  __shared__ int32_t vals[6][65];           // padded: lanes 0..5 read one column each
   const int lane = threadIdx.x;
   const int32_t b = blockIdx.x;
   const OrientHdr &h = p.hdr[b];
@@ -1036,18 +1078,41 @@ __global__ __launch_bounds__(64) void fsum_kernel(SweepParams p) {
   const Tables t = tables_for(p, b);
   const int32_t G = p.grid, G2 = G * G;
   const int32_t bx0 = h.bbox[0], bx1 = h.bbox[1], by0 = h.bbox[2], by1 = h.bbox[3];
+  const int32_t wy = by1 - by0 + 1;
+  const int32_t nbb = bx1 >= bx0 ? (bx1 - bx0 + 1) * wy : 0;
+  const int32_t nchunk = (nbb + 63) >> 6;
   const int q = lane < 6 ? lane : 0;
-  float acc = 0.0f;
-  for (int32_t sx = bx0; sx <= bx1; sx++) {
-    for (int32_t y0 = by0; y0 <= by1; y0 += 64) {
-      const int32_t n = min(64, by1 - y0 + 1);
-      const int64_t s0 = (int64_t)sx * G + y0;
+  // Slots in ascending id order: row-major over the bounding box (id = x * G + y).
+  auto load = [&](int32_t c, int32_t (&v)[6]) {
+    const int32_t i = (c << 6) + lane;
+    if (c < nchunk && i < nbb) {
+      const int32_t s = (bx0 + i / wy) * G + by0 + i % wy;
 #pragma unroll
-      for (int k = 0; k < 6; k++) vals[k][lane] = lane < n ? t.sv[(int64_t)k * G2 + s0 + lane] : 0;
-      __syncthreads();
-      if (lane < 6)
-        for (int i = 0; i < n; i++) acc = acc + (float)vals[q][i];
-      __syncthreads();
+      for (int k = 0; k < 6; k++) v[k] = t.sv[(int64_t)k * G2 + s];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 6; k++) v[k] = 0;
+    }
+  };
+  int32_t buf[kAhead][6];
+#pragma unroll
+  for (int j = 0; j < kAhead; j++) load(j, buf[j]);
+  float acc = 0.0f;
+  for (int32_t c0 = 0; c0 < nchunk; c0 += kAhead) {
+#pragma unroll
+    for (int j = 0; j < kAhead; j++) {
+      const int32_t c = c0 + j;
+      if (c < nchunk) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) vals[k][lane] = buf[j][k];
+        __syncthreads();
+        load(c + kAhead, buf[j]);
+        if (lane < 6) {
+          const int32_t n = min(64, nbb - (c << 6));
+          for (int32_t i = 0; i < n; i++) acc = acc + (float)vals[q][i];
+        }
+        __syncthreads();
+      }
     }
   }
   if (lane < 6) R.fsum[lane] = acc;
