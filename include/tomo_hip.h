@@ -121,7 +121,7 @@ int tomo_set_launch(tomo_handle *h, int32_t batch);
 
 /* Launch-shape knobs for tuning (values take effect at the next tomo_run):
  * TOMO_TUNE_BATCH           orientations in flight (0 = automatic), same as tomo_set_launch;
- * TOMO_TUNE_TRI_PER_THREAD  triangles per rasteriser thread (1..64, default 4);
+ * TOMO_TUNE_TRI_PER_THREAD  triangles per rasteriser thread (1..64, default 8);
  * TOMO_TUNE_INLINE          slot entries stored inline per slot (8..32, default 16); the rest go
  *                           to an overflow pool (more inline = fewer pool passes, more memory);
  * TOMO_TUNE_EXACT_FSUM      1 (default): where a float total can round (TOMO_F_FP32_INEXACT),

@@ -75,6 +75,9 @@ def compare_records(gpu, ref, label=""):
         if d.size:
             i = d[0]
             bad.append(f"{label}{k}: {d.size} differ, first #{i}: gpu {gpu[k][i]} ref {ref[k][i]}")
+    un = np.nonzero(gpu["flags"] & 64)[0]  # UNRESOLVED: the library could not keep an entry
+    if un.size:
+        bad.append(f"{label}unresolved entries in {un.size} orientations, first #{un[0]}")
     fl = np.nonzero((gpu["flags"] & 31) != (ref["flags"] & 31))[0]
     if fl.size:
         i = fl[0]

@@ -849,7 +849,7 @@ static float time_shape(tomo_handle *h, const tomo_batch::Tune &t, int32_t b0, i
 
 // Autotuner.  Members are grouped by (grid, ceil(log2(nTri))); each group's largest member is
 // timed on `sample` orientations from the middle of its list with a coordinate descent over
-// triangles per rasteriser thread {1, 2, 4, 8}, inline slot capacity {8, 16, 24} and
+// triangles per rasteriser thread {2, 4, 8, 16}, inline slot capacity {8, 16, 24} and
 // orientations in flight {1024, 2048, 4096} (within a quarter of free memory); shapes that leave
 // a slot unresolved are rejected.  The group's winner applies to all its members.
 int tomo_batch_tune(tomo_batch *b, int32_t sample) {
@@ -884,7 +884,7 @@ int tomo_batch_tune(tomo_batch *b, int32_t sample) {
         best = t;
       }
     };
-    for (int32_t tpt : {1, 2, 8}) {
+    for (int32_t tpt : {2, 4, 16}) {
       tomo_batch::Tune t = best;
       t.tri_per_thread = tpt;
       consider(t);

@@ -16,7 +16,7 @@ constexpr int kInlineDefault = 16;     // entries stored inline per slot (layer-
 constexpr int kInlineMax = 32;         // in [kSmall, kInlineMax]
 constexpr int kBigCap = 512;           // entries a single slot may hold for exact resolution
 constexpr int kMaxBedR = 16;           // int(BedOuterBound) supported by the bed kernel
-constexpr int kTriPerThread = 4;       // rasteriser: default triangles per thread per work-group
+constexpr int kTriPerThread = 8;       // rasteriser: default triangles per thread per work-group
 constexpr int kPairTile = 2048;        // pairing: touched-box slots per work-group
 constexpr int kBandRows = 16;          // bed: grid rows per work-group
 
