@@ -1,4 +1,7 @@
 // Host build of the product's per-slot pairing (tomo_slot.h) for CPU unit tests.
+#include <algorithm>
+#include <cmath>
+
 #include "../tomo_shell2026_amd/csrc/tomo_raster.h"
 #include "../tomo_shell2026_amd/csrc/tomo_slot.h"
 
@@ -43,20 +46,29 @@ int harness_pair2(const uint32_t *keys, int c, int cs, int cn, int ex, int bt, i
 }
 uint32_t harness_make_key(int z, int nz) { return make_key(z, nz); }
 
-// Coverage test of the rasteriser for n pixel centres against one margined triangle:
-// out[i*5] = prefilter candidate, out[i*5+1] = exact accept, out[i*5+2..4] = u, v, w bits.
-int harness_bary(const float *tri6, const float *pix, int n, uint32_t *out) {
+// Coverage test of the rasteriser over the pixel box [x0, x1] x [y0, y1] of one margined
+// triangle, pixels x-major as the kernel walks them: out[i*5] = prefilter candidate,
+// out[i*5+1] = exact accept, out[i*5+2..4] = u, v, w bits.
+int harness_bary(const float *tri6, int x0, int x1, int y0, int y1, float h2, uint32_t *out) {
   TriSetup s = tri_setup(tri6[0], tri6[1], tri6[2], tri6[3], tri6[4], tri6[5]);
-  for (int i = 0; i < n; i++) {
-    float nv, nw, u = -1.f, v = -1.f, w = -1.f;
-    bary_num(s, pix[2 * i], pix[2 * i + 1], nv, nw);
-    const bool cand = bary_candidate(s.den, nv, nw);
-    bool acc = !tri_degenerate(s) && bary_exact(s.den, nv, nw, u, v, w);
-    if (!acc) { u = v = w = -1.f; }
-    uint32_t ub, vb, wb;
-    __builtin_memcpy(&ub, &u, 4); __builtin_memcpy(&vb, &v, 4); __builtin_memcpy(&wb, &w, 4);
-    out[5 * i] = cand; out[5 * i + 1] = acc; out[5 * i + 2] = ub; out[5 * i + 3] = vb; out[5 * i + 4] = wb;
+  const float ex = std::max(std::fabs(centre_offset(x0, h2, s.v0x)), std::fabs(centre_offset(x1, h2, s.v0x)));
+  const float ey = std::max(std::fabs(centre_offset(y0, h2, s.v0y)), std::fabs(centre_offset(y1, h2, s.v0y)));
+  const TriScan sc = tri_scan(s, ex, ey);
+  int i = 0;
+  for (int x = x0; x <= x1; x++) {
+    const float e2x = centre_offset(x, h2, s.v0x);
+    const float rax = e2x * sc.ax, rbx = e2x * sc.bx;
+    for (int y = y0; y <= y1; y++, i++) {
+      float nv, nw, u = -1.f, v = -1.f, w = -1.f;
+      bary_num(s, ((float)x + kEps) + h2, ((float)y + kEps) + h2, nv, nw);
+      const bool cand = scan_candidate(sc, rax, rbx, centre_offset(y, h2, s.v0y));
+      bool acc = !tri_degenerate(s) && bary_exact(s.den, nv, nw, u, v, w);
+      if (!acc) { u = v = w = -1.f; }
+      uint32_t ub, vb, wb;
+      __builtin_memcpy(&ub, &u, 4); __builtin_memcpy(&vb, &v, 4); __builtin_memcpy(&wb, &w, 4);
+      out[5 * i] = cand; out[5 * i + 1] = acc; out[5 * i + 2] = ub; out[5 * i + 3] = vb; out[5 * i + 4] = wb;
+    }
   }
-  return 0;
+  return i;
 }
 }

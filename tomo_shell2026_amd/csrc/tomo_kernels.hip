@@ -530,10 +530,14 @@ __global__ __launch_bounds__(kThreads) void aabb_kernel(SweepParams p) {
 // walks its pixel box with the cheap prefilter (tomo_raster.h); surviving pixels are queued in
 // LDS and the exact coverage test, interpolation and slot append run on compacted lanes.
 // Work-groups follow xcd_map so an orientation's slot table stays in one XCD's L2.
+// Triangle record in LDS (odd stride: lanes reading different owners hit different banks).
+enum : int { TR_V0 = 0, TR_V1 = 3, TR_V2 = 6, TR_NZ = 9, TR_DEN = 12, TR_D00 = 13, TR_D01 = 14,
+             TR_D11 = 15, TR_C = 16, TR_STRIDE = 19 };
+
 struct RasterLds {
-  float tri[kWaves][64][13];       // margined v0, v1, v2 (x, y, z), n0z, n1z, n2z, den
-  uint32_t qmeta[kWaves][128];     // owner lane | (x + 256) << 6 | (y + 256) << 19
-  float qnv[kWaves][128], qnw[kWaves][128];
+  // margined v0, v1, v2 (x, y, z), n0z, n1z, n2z, den, d00, d01, d11, margined centroid (x, y, z)
+  float tri[kWaves][64][TR_STRIDE];
+  uint32_t qmeta[kWaves][128];       // owner lane | (x + 256) << 6 | (y + 256) << 19
   int32_t hits[kWaves][64];
   int32_t red[kWaves][5];
   uint64_t zred[kWaves];
@@ -554,15 +558,16 @@ __global__ __launch_bounds__(kThreads) void raster_kernel(SweepParams p) {
   em.bx0 = G; em.bx1 = -1; em.by0 = G; em.by1 = -1; em.flags = 0; em.zabs = 0ull;
   const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
   const int32_t first = chunk * p.tri_per_thread * kThreads;
-  float (*tri)[13] = L.tri[wave];
+  float (*tri)[TR_STRIDE] = L.tri[wave];
 #pragma unroll 1
   for (int k = 0; k < p.tri_per_thread; k++) {
     const int32_t tr = first + k * kThreads + tid;
     const bool active = tr < p.n_tri;
-    float a[3][3] = {}, nzv[3] = {};
     int32_t x0 = 0, x1 = -1, y0 = 0, y1 = -1;
-    TriSetup ts = {};
+    TriScan sc = {};
+    float v0x = 0.0f, v0y = 0.0f;
     if (active) {
+      float a[3][3], nzv[3];
       const int32_t ids[3] = {p.tri[3 * tr + 0], p.tri[3 * tr + 1], p.tri[3 * tr + 2]};
 #pragma unroll
       for (int q = 0; q < 3; q++) {
@@ -572,7 +577,7 @@ __global__ __launch_bounds__(kThreads) void raster_kernel(SweepParams p) {
         a[q][2] = r.z - kz;
         nzv[q] = r.w;
       }
-      const float v0x = a[0][0] + kEps, v0y = a[0][1] + kEps;
+      v0x = a[0][0] + kEps; v0y = a[0][1] + kEps;
       const float v1x = a[1][0] + kEps, v1y = a[1][1] + kEps;
       const float v2x = a[2][0] + kEps, v2y = a[2][1] + kEps;
       float mnx = (v0x <= v1x) ? v0x : v1x; mnx = (mnx <= v2x) ? mnx : v2x;
@@ -587,39 +592,49 @@ __global__ __launch_bounds__(kThreads) void raster_kernel(SweepParams p) {
         em.flags |= F_FOOTPRINT_OOB;
         x0 = max(x0, lo); y0 = max(y0, lo); x1 = min(x1, hi); y1 = min(y1, hi);
       }
-      ts = tri_setup(v0x, v0y, v1x, v1y, v2x, v2y);
+      const TriSetup ts = tri_setup(v0x, v0y, v1x, v1y, v2x, v2y);
       if (tri_degenerate(ts)) { x1 = x0 - 1; }  // no pixel can pass
+      const float ex = fmaxf(fabsf(centre_offset(x0, p.h2, v0x)), fabsf(centre_offset(x1, p.h2, v0x)));
+      const float ey = fmaxf(fabsf(centre_offset(y0, p.h2, v0y)), fabsf(centre_offset(y1, p.h2, v0y)));
+      sc = tri_scan(ts, ex, ey);
       float *rec = tri[lane];
-      rec[0] = v0x; rec[1] = v0y; rec[2] = a[0][2] + kEps;
-      rec[3] = v1x; rec[4] = v1y; rec[5] = a[1][2] + kEps;
-      rec[6] = v2x; rec[7] = v2y; rec[8] = a[2][2] + kEps;
-      rec[9] = nzv[0]; rec[10] = nzv[1]; rec[11] = nzv[2]; rec[12] = ts.den;
+      rec[TR_V0 + 0] = v0x; rec[TR_V0 + 1] = v0y; rec[TR_V0 + 2] = a[0][2] + kEps;
+      rec[TR_V1 + 0] = v1x; rec[TR_V1 + 1] = v1y; rec[TR_V1 + 2] = a[1][2] + kEps;
+      rec[TR_V2 + 0] = v2x; rec[TR_V2 + 1] = v2y; rec[TR_V2 + 2] = a[2][2] + kEps;
+      rec[TR_NZ + 0] = nzv[0]; rec[TR_NZ + 1] = nzv[1]; rec[TR_NZ + 2] = nzv[2];
+      rec[TR_DEN] = ts.den; rec[TR_D00] = ts.d00; rec[TR_D01] = ts.d01; rec[TR_D11] = ts.d11;
+      // the un-margined centroid pixel a triangle without hits falls back to
+      // (STomoSh_INT3.cpp:1190-1206)
+#pragma unroll
+      for (int q = 0; q < 3; q++) rec[TR_C + q] = (a[0][q] + a[1][q] + a[2][q]) * 0.333333f + kEps;
     }
     L.hits[wave][lane] = 0;
     int32_t x = x0, y = y0;
     bool done = !active || x1 < x0 || y1 < y0;
+    float e2x = centre_offset(x, p.h2, v0x);
+    float rax = e2x * sc.ax, rbx = e2x * sc.bx;
     uint32_t qn = 0;
     wave_sync();
     for (;;) {
       bool cand = false;
-      float nv = 0.0f, nw = 0.0f;
       const int32_t cxi = x, cyi = y;
       if (!done) {
-        const float cx = (float)x + kEps + p.h2;
-        const float cy = (float)y + kEps + p.h2;
-        bary_num(ts, cx, cy, nv, nw);
-        cand = bary_candidate(ts.den, nv, nw);
+        cand = scan_candidate(sc, rax, rbx, centre_offset(y, p.h2, v0y));
         if (++y > y1) {
           y = y0;
-          if (++x > x1) done = true;
+          if (++x > x1) {
+            done = true;
+          } else {
+            e2x = centre_offset(x, p.h2, v0x);
+            rax = e2x * sc.ax;
+            rbx = e2x * sc.bx;
+          }
         }
       }
       const uint64_t m = __ballot(cand);
       if (cand) {
         const uint32_t pos = qn + (uint32_t)__popcll(m & lt);
         L.qmeta[wave][pos] = (uint32_t)lane | ((uint32_t)(cxi + 256) << 6) | ((uint32_t)(cyi + 256) << 19);
-        L.qnv[wave][pos] = nv;
-        L.qnw[wave][pos] = nw;
       }
       qn += (uint32_t)__popcll(m);
       const bool all_done = __ballot(!done) == 0ull;
@@ -632,13 +647,20 @@ __global__ __launch_bounds__(kThreads) void raster_kernel(SweepParams p) {
           const int32_t px_i = (int32_t)((meta >> 6) & 8191u) - 256;
           const int32_t py_i = (int32_t)(meta >> 19) - 256;
           const float *rec = tri[owner];
-          float u, v, w;
-          if (bary_exact(rec[12], L.qnv[wave][lane], L.qnw[wave][lane], u, v, w)) {
+          // the exact test's numerators, formed as _getBaryCoord forms them
+          TriSetup ts;
+          ts.v0x = rec[TR_V0 + 0]; ts.v0y = rec[TR_V0 + 1];
+          ts.e0x = rec[TR_V1 + 0] - ts.v0x; ts.e0y = rec[TR_V1 + 1] - ts.v0y;
+          ts.e1x = rec[TR_V2 + 0] - ts.v0x; ts.e1y = rec[TR_V2 + 1] - ts.v0y;
+          ts.d00 = rec[TR_D00]; ts.d01 = rec[TR_D01]; ts.d11 = rec[TR_D11]; ts.den = rec[TR_DEN];
+          float nv, nw, u, v, w;
+          bary_num(ts, ((float)px_i + kEps) + p.h2, ((float)py_i + kEps) + p.h2, nv, nw);
+          if (bary_exact(ts.den, nv, nw, u, v, w)) {
             // _bary_product (Tomo_types.cpp:75-84) of the margined vertices and the normals
-            const float px = rec[0] * u + rec[3] * v + rec[6] * w;
-            const float py = rec[1] * u + rec[4] * v + rec[7] * w;
-            float pz = rec[2] * u + rec[5] * v + rec[8] * w;
-            float nz = rec[9] * u + rec[10] * v + rec[11] * w;
+            const float px = rec[TR_V0 + 0] * u + rec[TR_V1 + 0] * v + rec[TR_V2 + 0] * w;
+            const float py = rec[TR_V0 + 1] * u + rec[TR_V1 + 1] * v + rec[TR_V2 + 1] * w;
+            float pz = rec[TR_V0 + 2] * u + rec[TR_V1 + 2] * v + rec[TR_V2 + 2] * w;
+            float nz = rec[TR_NZ + 0] * u + rec[TR_NZ + 1] * v + rec[TR_NZ + 2] * w;
             const uint32_t rank = 2u * (uint32_t)(first + k * kThreads + wave * 64 + owner);
             int32_t s;
             if (p.do_shell) {  // with its shell sub-pixel (STomoSh_INT3.cpp:1179-1185)
@@ -648,25 +670,16 @@ __global__ __launch_bounds__(kThreads) void raster_kernel(SweepParams p) {
               s = em.put(px, py, pz, nz, rank);
             }
             if (s >= 0 && s != px_i * G + py_i) em.flags |= F_RANK_TIE;
-            atomicAdd(&L.hits[wave][owner], 1);
+            L.hits[wave][owner] = 1;
           }
         }
         wave_sync();
         if (qn > 64u) {
           const uint32_t rest = qn - 64u;
           uint32_t mm = 0u;
-          float a1 = 0.0f, a2 = 0.0f;
-          if ((uint32_t)lane < rest) {
-            mm = L.qmeta[wave][64 + lane];
-            a1 = L.qnv[wave][64 + lane];
-            a2 = L.qnw[wave][64 + lane];
-          }
+          if ((uint32_t)lane < rest) mm = L.qmeta[wave][64 + lane];
           wave_sync();
-          if ((uint32_t)lane < rest) {
-            L.qmeta[wave][lane] = mm;
-            L.qnv[wave][lane] = a1;
-            L.qnw[wave][lane] = a2;
-          }
+          if ((uint32_t)lane < rest) L.qmeta[wave][lane] = mm;
           wave_sync();
         }
         qn -= np;
@@ -676,15 +689,14 @@ __global__ __launch_bounds__(kThreads) void raster_kernel(SweepParams p) {
     wave_sync();
     if (active && L.hits[wave][lane] == 0) {
       // very small triangle: one pixel at the un-margined centroid (STomoSh_INT3.cpp:1190-1206)
-      const float tx = (a[0][0] + a[1][0] + a[2][0]) * 0.333333f + kEps;
-      const float ty = (a[0][1] + a[1][1] + a[2][1]) * 0.333333f + kEps;
-      const float tz = (a[0][2] + a[1][2] + a[2][2]) * 0.333333f + kEps;
+      const float *rec = tri[lane];
+      const float tx = rec[TR_C + 0], ty = rec[TR_C + 1], tz = rec[TR_C + 2], n0 = rec[TR_NZ + 0];
       const uint32_t rank = 2u * (uint32_t)tr;
       if (p.do_shell) {
-        const float sz = tz + ((nzv[0] >= 0.0f) ? -p.shell_off : p.shell_off);
-        em.put2(tx, ty, tz, nzv[0], sz, -nzv[0], rank);
+        const float sz = tz + ((n0 >= 0.0f) ? -p.shell_off : p.shell_off);
+        em.put2(tx, ty, tz, n0, sz, -n0, rank);
       } else {
-        em.put(tx, ty, tz, nzv[0], rank);
+        em.put(tx, ty, tz, n0, rank);
       }
     }
     wave_sync();

@@ -10,14 +10,24 @@
 // den, d00, d01, d11 only on the triangle.
 //
 // The prefilter rejects pixels whose barycentric coordinates are clearly outside, without
-// dividing.  With D = |den|, a = sign(den) * num_v, b = sign(den) * num_w (so v ~ a/D, w ~ b/D)
-// and the slack M = 2^-12 (D + |a| + |b|), a pixel is rejected when
+// dividing and without the dot products of the exact test.  Over the triangle's pixel box,
+// a = sign(den) * num_v and b = sign(den) * num_w are linear in the float offsets
+// e2x = cx - v0x, e2y = cy - v0y of the pixel centre:
+//   a = e2x * ax + e2y * ay,  ax = sign(den) (d11 e0x - d01 e1x),  ay = sign(den) (d11 e0y - d01 e1y)
+//   b = e2x * bx + e2y * by,  bx = sign(den) (d00 e1x - d01 e0x),  by = sign(den) (d00 e1y - d01 e0y)
+// With D = |den|, Sa = Ex (|d11 e0x| + |d01 e1x|) + Ey (|d11 e0y| + |d01 e1y|), Sb likewise for b,
+// Ex, Ey the largest |e2x|, |e2y| in the box, and the slack M = 2^-12 D + 2^-16 (Sa + Sb), a
+// pixel is rejected when
 //   a < -eps*D - M   (v < -eps)          a > L*D + M       (v > 1 + eps)
-//   a + b > (1+eps)*D + M  (u < -eps)    b < -eps*D - M    (u + v > 1 + eps, i.e. w < -(L-1))
-// where L = 1.0009999 is the largest float not above 1 + (double)eps.  The correctly rounded
-// quotients differ from a/D, b/D by at most 2^-24 relatively and the float evaluation of these
-// bounds by a few 2^-24 (D + |a| + |b|), far inside M, so every rejected pixel is also rejected
-// by the exact test (checked exhaustively against it in the host tests).
+//   b < -eps*D - M   (u + v > 1 + eps)   a + b > (1+eps)*D + M  (u < -eps)
+// where L = 1.0009999 is the largest float not above 1 + (double)eps.  |e2x| <= Ex holds exactly
+// (the float pixel-centre offset is monotone in the pixel index, so its extremes sit at the box
+// ends).  The affine evaluation and the exact test's float numerators each lie within
+// 4 * 2^-24 Sa (resp. Sb) of the real-number numerator of the same float inputs; the correctly
+// rounded quotients and the double u add 2^-23 D + 2^-24 (Sa + Sb); the float bounds a few
+// 2^-24 (D + M).  The total stays below 2^-20 (D + Sa + Sb), far inside M, so every rejected
+// pixel is also rejected by the exact test (checked against it over whole pixel boxes of
+// ordinary, sliver, tiny and huge triangles in the host tests).
 #pragma once
 #include <stdint.h>
 
@@ -66,17 +76,41 @@ TOMO_RHD void bary_num(const TriSetup &s, float cx, float cy, float &nv, float &
   nw = s.d00 * d21 - s.d01 * d20;
 }
 
-// True when the exact test may accept (never false for an accepted pixel).
-TOMO_RHD bool bary_candidate(float den, float nv, float nw) {
-  const float D = den >= 0.0f ? den : -den;
-  const float a = den >= 0.0f ? nv : -nv;
-  const float b = den >= 0.0f ? nw : -nw;
-  const float M = 0.000244140625f * (D + (a >= 0.0f ? a : -a) + (b >= 0.0f ? b : -b));
-  const float lo = -kEps * D - M;
-  if (a < lo || b < lo) return false;
-  if (a > kOnePlusEpsF * D + M) return false;
-  if (a + b > (1.0f + kEps) * D + M) return false;
-  return true;
+// Per-triangle constants of the prefilter for the pixel-centre offsets |e2x| <= ex, |e2y| <= ey.
+struct TriScan {
+  float ax, ay, bx, by;  // gradients of a and b (sign of den folded in)
+  float lo, hia, his;    // a, b >= lo;  a <= hia;  a + b <= his
+};
+
+TOMO_RHD TriScan tri_scan(const TriSetup &s, float ex, float ey) {
+  const bool pos = s.den >= 0.0f;
+  const float D = pos ? s.den : -s.den;
+  const float p0x = s.d11 * s.e0x, q1x = s.d01 * s.e1x, p0y = s.d11 * s.e0y, q1y = s.d01 * s.e1y;
+  const float p1x = s.d00 * s.e1x, q0x = s.d01 * s.e0x, p1y = s.d00 * s.e1y, q0y = s.d01 * s.e0y;
+  const float ax = p0x - q1x, ay = p0y - q1y, bx = p1x - q0x, by = p1y - q0y;
+  TriScan t;
+  t.ax = pos ? ax : -ax; t.ay = pos ? ay : -ay;
+  t.bx = pos ? bx : -bx; t.by = pos ? by : -by;
+  const float sa = ex * (__builtin_fabsf(p0x) + __builtin_fabsf(q1x)) +
+                   ey * (__builtin_fabsf(p0y) + __builtin_fabsf(q1y));
+  const float sb = ex * (__builtin_fabsf(p1x) + __builtin_fabsf(q0x)) +
+                   ey * (__builtin_fabsf(p1y) + __builtin_fabsf(q0y));
+  const float M = 0.000244140625f * D + 0.0000152587890625f * (sa + sb);
+  t.lo = -kEps * D - M;
+  t.hia = kOnePlusEpsF * D + M;
+  t.his = (1.0f + kEps) * D + M;
+  return t;
+}
+
+// Pixel-centre offset from v0 along one axis, as the exact test forms it.
+TOMO_RHD float centre_offset(int32_t i, float h2, float v0) { return (((float)i + kEps) + h2) - v0; }
+
+// True when the exact test may accept (never false for an accepted pixel; a NaN anywhere leaves
+// the pixel to the exact test).  rax = e2x * ax and rbx = e2x * bx for the pixel's column.
+TOMO_RHD bool scan_candidate(const TriScan &t, float rax, float rbx, float e2y) {
+  const float a = __builtin_fmaf(e2y, t.ay, rax);
+  const float b = __builtin_fmaf(e2y, t.by, rbx);
+  return !(a < t.lo || b < t.lo || a > t.hia || a + b > t.his);
 }
 
 // The exact acceptance; on success u, v, w are the barycentric weights.
