@@ -120,7 +120,7 @@ def bench_batch70(args):
     shapes = {}
     for m in range(len(ws)):
         t = B.tuning(m)
-        k = (t["batch"], t["tri_per_thread"], t["inline"])
+        k = (t["batch"], t["tile_y"], t["inline"])
         shapes[str(k)] = shapes.get(str(k), 0) + 1
     flags = {"cap_drop": 0, "fp32_inexact": 0, "footprint_oob": 0, "unresolved": 0}
     for m in range(len(ws)):

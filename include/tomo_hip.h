@@ -121,7 +121,9 @@ int tomo_set_launch(tomo_handle *h, int32_t batch);
 
 /* Launch-shape knobs for tuning (values take effect at the next tomo_run):
  * TOMO_TUNE_BATCH           orientations in flight (0 = automatic), same as tomo_set_launch;
- * TOMO_TUNE_TRI_PER_THREAD  triangles per rasteriser thread (1..64, default 8);
+ * TOMO_TUNE_TILE_Y          rasteriser tile extent along y in slots (power of two 32..512,
+ *                           default 128); a tile holds 16384 slots, so its x extent is
+ *                           16384 / value;
  * TOMO_TUNE_INLINE          slot entries stored inline per slot (8..32, default 16); the rest go
  *                           to an overflow pool (more inline = fewer pool passes, more memory);
  * TOMO_TUNE_EXACT_FSUM      1 (default): where a float total can round (TOMO_F_FP32_INEXACT),
@@ -129,7 +131,7 @@ int tomo_set_launch(tomo_handle *h, int32_t batch);
  *                           fVss, fVa, fVb, fVnv, fVtc and the masses equal the reference's;
  *                           0: report float(integer total) instead (cheaper, may differ there). */
 #define TOMO_TUNE_BATCH 0
-#define TOMO_TUNE_TRI_PER_THREAD 1
+#define TOMO_TUNE_TILE_Y 1
 #define TOMO_TUNE_EXACT_FSUM 2
 #define TOMO_TUNE_INLINE 3
 int tomo_set_tuning(tomo_handle *h, int32_t knob, int32_t value);
@@ -173,13 +175,13 @@ int tomo_batch_add(tomo_batch *b, const float *float32_info_x12, const int32_t *
                    const float *vtx_nrm, int32_t n_ypr, const float *ypr, int32_t *member);
 
 /* Autotuner: per group of members with the same grid and ceil(log2(nTri)), time `sample`
- * orientations of the group's largest member over triangles per rasteriser thread, inline slot
+ * orientations of the group's largest member over the rasteriser tile shape, inline slot
  * capacity and orientations in flight, and keep the fastest shape that resolves every slot. */
 int tomo_batch_tune(tomo_batch *b, int32_t sample);
 
 /* The launch shape chosen for a member (defaults before tomo_batch_tune) and its measured
  * microseconds per orientation (negative if not tuned). */
-int tomo_batch_get_tuning(tomo_batch *b, int32_t member, int32_t *batch, int32_t *tri_per_thread,
+int tomo_batch_get_tuning(tomo_batch *b, int32_t member, int32_t *batch, int32_t *tile_y,
                           int32_t *inline_capacity, float *us_per_orientation);
 
 /* Allocate the shared scratch and capture every member's sweep into one graph (no launch). */

@@ -21,11 +21,13 @@ def announce(name):
     print(f"[gpu-case] {name}", flush=True)
 
 
-def run_gpu(case, launch=0):
+def run_gpu(case, launch=0, tile_y=0):
     with _lib.Sweep(case["f32"], case["i32"], case["grid"]) as s:
         s.set_mesh(case["tri"], case["vtx"], case["vnrm"])
         if launch:
             s.set_launch(launch)
+        if tile_y:
+            s.set_tuning(s.TUNE_TILE_Y, tile_y)
         return s.sweep(case["ypr"])
 
 
@@ -151,6 +153,19 @@ def test_launch_shapes_agree(oracle_mod):
     for groups in (1, 7, 64):
         g = run_gpu(case, launch=groups)
         assert not C.compare_records(g, ref), groups
+
+
+@pytest.mark.parametrize("tile_y", [32, 64, 256, 512])
+def test_tile_shapes_match_oracle(oracle_mod, tile_y):
+    """Rasteriser tiles of 16384 slots, tile_y along y: tile borders, entries spilled across them
+    and the owner tile's zero-hit fallback move with the shape; results must not."""
+    w = synth.displaced_sphere_workload()
+    case = workload_case(w, w.ypr[np.arange(7, len(w.ypr), 977)])
+    announce(f"dsphere69120 tile_y={tile_y}")
+    g = run_gpu(case, tile_y=tile_y)
+    r = run_ref(oracle_mod, case)
+    bad = C.compare_records(g, r, f"tile_y={tile_y}: ")
+    assert not bad, "\n".join(bad)
 
 
 def test_full_sweep_properties():

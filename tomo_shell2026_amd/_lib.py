@@ -173,7 +173,7 @@ class Sweep:
     def set_launch(self, n_groups):
         check(load().tomo_set_launch(self._h, int(n_groups)), "tomo_set_launch")
 
-    TUNE_BATCH, TUNE_TRI_PER_THREAD, TUNE_EXACT_FSUM, TUNE_INLINE = 0, 1, 2, 3
+    TUNE_BATCH, TUNE_TILE_Y, TUNE_EXACT_FSUM, TUNE_INLINE = 0, 1, 2, 3
 
     def set_tuning(self, knob, value):
         check(load().tomo_set_tuning(self._h, int(knob), int(value)), "tomo_set_tuning")
@@ -283,7 +283,7 @@ class Batch:
         us = ct.c_float()
         check(load().tomo_batch_get_tuning(self._h, int(member), ct.byref(b), ct.byref(t), ct.byref(i),
                                            ct.byref(us)), "tomo_batch_get_tuning")
-        return {"batch": b.value, "tri_per_thread": t.value, "inline": i.value,
+        return {"batch": b.value, "tile_y": t.value, "inline": i.value,
                 "us_per_orientation": round(us.value, 4)}
 
     def capture(self):

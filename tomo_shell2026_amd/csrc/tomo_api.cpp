@@ -164,9 +164,14 @@ ScratchLayout make_layout(int32_t G, int32_t inl) {
   L.cmp_key = take(4 * (int64_t)pool);
   L.cmp_rank = take(4 * (int64_t)pool);
   L.sv = take(4 * 6 * G2);
+  const int32_t spill = 16 * G + 4096;
+  L.spill_slot = take(4 * (int64_t)spill);
+  L.spill_key = take(4 * (int64_t)spill);
+  L.spill_rank = take(4 * (int64_t)spill);
   L.bytes = at;
   L.pool = pool;
   L.inl = inl;
+  L.spill = spill;
   return L;
 }
 
@@ -188,7 +193,10 @@ struct tomo_handle {
   // mesh
   int32_t n_tri = 0, n_vtx = 0;
   float center[3] = {0, 0, 0};
-  int32_t *d_tri = nullptr;
+  int32_t *d_tri = nullptr;     // triangles in culling order
+  int32_t *d_tri_id = nullptr;  // the caller's index of each
+  float *d_chunk = nullptr;     // [n_chunks][6] raw AABB of each kChunk triangles
+  int32_t n_chunks = 0;
   float *d_soa = nullptr;  // 6 * n_vtx: x, y, z, nx, ny, nz
   float *d_bed_dist = nullptr;
   // orientations
@@ -210,7 +218,7 @@ struct tomo_handle {
   bool shared = false;
   hipStream_t own_stream = nullptr;
   int32_t batch_req = 0;
-  int32_t tri_per_thread = kTriPerThread;
+  int32_t tile_y = kTileYDefault;
   int32_t exact_fsum = 1;
   int32_t last_batch = 0;
   int32_t last_begin = 0, last_end = 0;
@@ -331,6 +339,9 @@ void vm_from_record(const tomo_handle *h, const tomo_record &r, float *vm) {
 SweepParams base_params(tomo_handle *h) {
   SweepParams p{};
   p.tri = h->d_tri;
+  p.tri_id = h->d_tri_id;
+  p.chunk_box = h->d_chunk;
+  p.n_chunks = h->n_chunks;
   p.vx = h->d_soa;
   p.vy = h->d_soa + h->n_vtx;
   p.vz = h->d_soa + 2 * (size_t)h->n_vtx;
@@ -358,9 +369,13 @@ SweepParams base_params(tomo_handle *h) {
   p.pv = h->d_pv;
   p.scratch = h->d_scratch;
   const int32_t g2 = h->opt.grid * h->opt.grid;
-  p.tri_per_thread = h->tri_per_thread;
-  const int32_t per_group = kThreads * h->tri_per_thread;
-  p.tri_groups = (h->n_tri + per_group - 1) / per_group;
+  int32_t ys = 0;
+  while ((1 << (ys + 1)) <= h->tile_y) ys++;
+  p.tile_y_shift = ys;
+  p.tile_x = kTileSlots >> ys;
+  p.tiles_x = (h->opt.grid + p.tile_x - 1) / p.tile_x;
+  p.tiles_y = (h->opt.grid + (1 << ys) - 1) >> ys;
+  p.tiles = p.tiles_x * p.tiles_y;
   p.pair_tiles = (g2 + kPairTile - 1) / kPairTile;
   p.bed_bands = (h->opt.grid + kBandRows - 1) / kBandRows;
   p.exact_fsum = h->exact_fsum;
@@ -382,7 +397,6 @@ int enqueue(tomo_handle *h, int32_t begin, int32_t end, int32_t batch, int32_t *
       p.dbg_vss = dbg4 + 3 * g2;
     }
     p.dbg_rows = dbg_rows;
-    if (p.tri_groups == 0) p.tri_groups = 1;
     for (int stage = 0; stage < 6; stage++) {
       if (h->phases) {
         hipEvent_t e;
@@ -477,6 +491,8 @@ int tomo_set_mesh(tomo_handle *h, int32_t n_tri, const int32_t *tri, int32_t n_v
   if (n_tri >= (1 << 30)) return fail(TOMO_E_INVALID_ARG, "too many triangles");
   HIP_TRY(hipStreamSynchronize(h->stream));
   if (h->d_tri) { HIP_TRY(hipFree(h->d_tri)); h->d_tri = nullptr; }
+  if (h->d_tri_id) { HIP_TRY(hipFree(h->d_tri_id)); h->d_tri_id = nullptr; }
+  if (h->d_chunk) { HIP_TRY(hipFree(h->d_chunk)); h->d_chunk = nullptr; }
   if (h->d_soa) { HIP_TRY(hipFree(h->d_soa)); h->d_soa = nullptr; }
   h->n_tri = n_tri;
   h->n_vtx = n_vtx;
@@ -497,9 +513,59 @@ int tomo_set_mesh(tomo_handle *h, int32_t n_tri, const int32_t *tri, int32_t n_v
     }
   HIP_TRY(hipMalloc(&h->d_soa, soa.size() * sizeof(float)));
   HIP_TRY(hipMemcpy(h->d_soa, soa.data(), soa.size() * sizeof(float), hipMemcpyHostToDevice));
-  HIP_TRY(hipMalloc(&h->d_tri, sizeof(int32_t) * 3 * (size_t)(n_tri > 0 ? n_tri : 1)));
-  if (n_tri > 0)
-    HIP_TRY(hipMemcpy(h->d_tri, tri, sizeof(int32_t) * 3 * (size_t)n_tri, hipMemcpyHostToDevice));
+  // Culling order: triangles sorted by the Morton code of their centroid, so each run of kChunk
+  // triangles is compact in space and its AABB bounds where the rasteriser can touch it.
+  std::vector<uint64_t> code((size_t)n_tri);
+  {
+    float span = 0.0f;
+    for (int k = 0; k < 3; k++) span = std::max(span, mx[k] - mn[k]);
+    const double scale = span > 0.0f ? 1023.0 / (double)span : 0.0;
+    auto spread = [](uint64_t v) {  // 10 bits -> every third bit
+      v &= 1023u;
+      v = (v | (v << 16)) & 0x030000FFull;
+      v = (v | (v << 8)) & 0x0300F00Full;
+      v = (v | (v << 4)) & 0x030C30C3ull;
+      v = (v | (v << 2)) & 0x09249249ull;
+      return v;
+    };
+    for (int32_t t = 0; t < n_tri; t++) {
+      uint64_t q[3];
+      for (int k = 0; k < 3; k++) {
+        double c = 0.0;
+        for (int j = 0; j < 3; j++) c += vtx[(size_t)tri[(size_t)t * 3 + j] * 3 + k];
+        double u = (c / 3.0 - (double)mn[k]) * scale;
+        u = u < 0.0 ? 0.0 : (u > 1023.0 ? 1023.0 : u);
+        q[k] = (uint64_t)u;
+      }
+      code[(size_t)t] = ((spread(q[0]) | (spread(q[1]) << 1) | (spread(q[2]) << 2)) << 32) | (uint64_t)t;
+    }
+    std::sort(code.begin(), code.end());
+  }
+  std::vector<int32_t> ord_tri((size_t)3 * (n_tri > 0 ? n_tri : 1)), ord_id((size_t)(n_tri > 0 ? n_tri : 1));
+  for (int32_t i = 0; i < n_tri; i++) {
+    const int32_t t = (int32_t)(code[(size_t)i] & 0xffffffffu);
+    ord_id[(size_t)i] = t;
+    for (int j = 0; j < 3; j++) ord_tri[(size_t)i * 3 + j] = tri[(size_t)t * 3 + j];
+  }
+  h->n_chunks = (n_tri + kChunk - 1) / kChunk;
+  std::vector<float> box((size_t)6 * (h->n_chunks > 0 ? h->n_chunks : 1));
+  for (int32_t c = 0; c < h->n_chunks; c++) {
+    float b[6] = {3.0e38f, 3.0e38f, 3.0e38f, -3.0e38f, -3.0e38f, -3.0e38f};
+    for (int32_t i = c * kChunk; i < std::min(n_tri, (c + 1) * kChunk); i++)
+      for (int j = 0; j < 3; j++)
+        for (int k = 0; k < 3; k++) {
+          const float v = vtx[(size_t)ord_tri[(size_t)i * 3 + j] * 3 + k];
+          b[k] = std::min(b[k], v);
+          b[3 + k] = std::max(b[3 + k], v);
+        }
+    for (int k = 0; k < 6; k++) box[(size_t)c * 6 + k] = b[k];
+  }
+  HIP_TRY(hipMalloc(&h->d_tri, sizeof(int32_t) * ord_tri.size()));
+  HIP_TRY(hipMalloc(&h->d_tri_id, sizeof(int32_t) * ord_id.size()));
+  HIP_TRY(hipMalloc(&h->d_chunk, sizeof(float) * box.size()));
+  HIP_TRY(hipMemcpy(h->d_tri, ord_tri.data(), sizeof(int32_t) * ord_tri.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->d_tri_id, ord_id.data(), sizeof(int32_t) * ord_id.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->d_chunk, box.data(), sizeof(float) * box.size(), hipMemcpyHostToDevice));
   return TOMO_OK;
 }
 
@@ -538,9 +604,10 @@ int tomo_set_tuning(tomo_handle *h, int32_t knob, int32_t value) {
       if (value < 0) return fail(TOMO_E_INVALID_ARG, "batch must be >= 0");
       h->batch_req = value;
       return TOMO_OK;
-    case TOMO_TUNE_TRI_PER_THREAD:
-      if (value < 1 || value > 64) return fail(TOMO_E_INVALID_ARG, "triangles per thread must be 1..64");
-      h->tri_per_thread = value;
+    case TOMO_TUNE_TILE_Y:
+      if (value < 32 || value > 512 || (value & (value - 1)))
+        return fail(TOMO_E_INVALID_ARG, "tile extent along y must be a power of two in 32..512");
+      h->tile_y = value;
       return TOMO_OK;
     case TOMO_TUNE_INLINE:
       if (value < kSmall || value > kInlineMax) return fail(TOMO_E_INVALID_ARG, "inline slot capacity must be 8..32");
@@ -712,6 +779,8 @@ int tomo_destroy(tomo_handle *h) {
   if (h->d_rec) hipFree(h->d_rec);
   if (h->d_rot) hipFree(h->d_rot);
   if (h->d_tri) hipFree(h->d_tri);
+  if (h->d_tri_id) hipFree(h->d_tri_id);
+  if (h->d_chunk) hipFree(h->d_chunk);
   if (h->d_soa) hipFree(h->d_soa);
   if (h->d_bed_dist) hipFree(h->d_bed_dist);
   if (h->ev0) hipEventDestroy(h->ev0);
@@ -735,7 +804,7 @@ struct tomo_batch {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::vector<tomo_handle *> members;
   struct Tune {
-    int32_t batch = 0, tri_per_thread = kTriPerThread, inl = kInlineDefault;
+    int32_t batch = 0, tile_y = kTileYDefault, inl = kInlineDefault;
     float us_per_orientation = -1.0f;
     int32_t tuned = 0;
   };
@@ -827,7 +896,7 @@ int tomo_batch_add(tomo_batch *b, const float *f32, const int32_t *i32, int32_t 
 }
 
 static void apply_tune(tomo_handle *h, const tomo_batch::Tune &t) {
-  h->tri_per_thread = t.tri_per_thread;
+  h->tile_y = t.tile_y;
   h->batch_req = t.batch;
   if (h->lay.inl != t.inl) h->lay = make_layout(h->opt.grid, t.inl);
 }
@@ -849,7 +918,7 @@ static float time_shape(tomo_handle *h, const tomo_batch::Tune &t, int32_t b0, i
 
 // Autotuner.  Members are grouped by (grid, ceil(log2(nTri))); each group's largest member is
 // timed on `sample` orientations from the middle of its list with a coordinate descent over
-// triangles per rasteriser thread {2, 4, 8, 16}, inline slot capacity {8, 16, 24} and
+// rasteriser tile extent along y {64, 128, 256}, inline slot capacity {8, 16, 24} and
 // orientations in flight {1024, 2048, 4096} (within a quarter of free memory); shapes that leave
 // a slot unresolved are rejected.  The group's winner applies to all its members.
 int tomo_batch_tune(tomo_batch *b, int32_t sample) {
@@ -884,9 +953,9 @@ int tomo_batch_tune(tomo_batch *b, int32_t sample) {
         best = t;
       }
     };
-    for (int32_t tpt : {2, 4, 16}) {
+    for (int32_t ty : {64, 256}) {
       tomo_batch::Tune t = best;
-      t.tri_per_thread = tpt;
+      t.tile_y = ty;
       consider(t);
     }
     for (int32_t inl : {8, 24}) {
@@ -918,12 +987,12 @@ int tomo_batch_tune(tomo_batch *b, int32_t sample) {
   return TOMO_OK;
 }
 
-int tomo_batch_get_tuning(tomo_batch *b, int32_t member, int32_t *batch, int32_t *tri_per_thread, int32_t *inl,
+int tomo_batch_get_tuning(tomo_batch *b, int32_t member, int32_t *batch, int32_t *tile_y, int32_t *inl,
                           float *us_per_orientation) {
   if (!b || member < 0 || member >= (int32_t)b->members.size()) return fail(TOMO_E_INVALID_ARG, "bad member");
   const tomo_batch::Tune &t = b->tune[(size_t)member];
   if (batch) *batch = t.batch;
-  if (tri_per_thread) *tri_per_thread = t.tri_per_thread;
+  if (tile_y) *tile_y = t.tile_y;
   if (inl) *inl = t.inl;
   if (us_per_orientation) *us_per_orientation = t.us_per_orientation;
   return TOMO_OK;

@@ -16,7 +16,11 @@ constexpr int kInlineDefault = 16;     // entries stored inline per slot (layer-
 constexpr int kInlineMax = 32;         // in [kSmall, kInlineMax]
 constexpr int kBigCap = 512;           // entries a single slot may hold for exact resolution
 constexpr int kMaxBedR = 16;           // int(BedOuterBound) supported by the bed kernel
-constexpr int kTriPerThread = 8;       // rasteriser: default triangles per thread per work-group
+constexpr int kTileThreads = 512;      // rasteriser work-group: 8 waves
+constexpr int kTileWaves = kTileThreads / 64;
+constexpr int kTileSlots = 16384;      // rasteriser tile: slots with a 16-bit LDS counter each
+constexpr int kTileYDefault = 128;     // tile extent along y (power of two); x extent = kTileSlots / y
+constexpr int kChunk = 64;             // triangles per culling chunk (one per lane)
 constexpr int kPairTile = 2048;        // pairing: touched-box slots per work-group
 constexpr int kBandRows = 16;          // bed: grid rows per work-group
 
@@ -43,29 +47,36 @@ struct OrientHdr {
   int32_t flags;
   float kx, ky, kz;
   unsigned long long zabs;  // sum of |z| over emitted entries: bounds every sum of |slot value|
-  int32_t pad[4];
+  uint32_t spill_top;       // entries whose slot lies outside the tile that produced them
+  int32_t pad[3];
 };
 
 // Per in-flight orientation scratch, carved from one device allocation.  G2 = grid * grid.
 struct ScratchLayout {
   int64_t count, ent, bedinfo, big, off, cursor, pool_slot, pool_key, pool_rank, cmp_key,
-      cmp_rank, sv;  // byte offsets inside one orientation's region
+      cmp_rank, sv, spill_slot, spill_key, spill_rank;  // byte offsets inside one orientation's region
   int64_t bytes; // bytes per region
   int32_t pool;  // overflow pool capacity (entries)
   int32_t inl;   // inline entries per slot
+  int32_t spill; // spill list capacity (entries)
+  int32_t pad_;
 };
 
 struct SweepParams {
   // mesh (structure of arrays, raw coordinates in voxel units)
-  const int32_t *tri;
+  const int32_t *tri;       // [n_tri][3], triangles in culling order (Morton order of centroids)
+  const int32_t *tri_id;    // [n_tri] the mesh's own index of each triangle (entry ranks)
+  const float *chunk_box;   // [n_chunks][6] raw-coordinate AABB (min xyz, max xyz) of kChunk triangles
   const float *vx, *vy, *vz, *nx, *ny, *nz;
-  int32_t n_tri, n_vtx;
+  int32_t n_tri, n_vtx, n_chunks;
   float cx, cy, cz;        // raw AABB centre (STomoSh_Base.cpp:75-82)
   // orientations of this batch: rot/rec already offset to the batch's first orientation
   const float *rot;        // [n_batch][9] row-major rotation matrices
   int32_t n_batch;
-  int32_t tri_groups;      // rasteriser work-groups per orientation
-  int32_t tri_per_thread;  // triangles per rasteriser thread
+  int32_t tiles;           // rasteriser work-groups (slot tiles) per orientation
+  int32_t tiles_x, tiles_y;
+  int32_t tile_x;          // tile extent along x (slots)
+  int32_t tile_y_shift;    // log2 of the tile extent along y
   int32_t pair_tiles;      // pairing work-groups per orientation
   int32_t bed_bands;       // bed work-groups per orientation
   // options

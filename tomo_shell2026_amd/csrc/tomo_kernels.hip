@@ -63,8 +63,10 @@ struct Tables {
       *cmp_key, *cmp_rank;
   int32_t *big;
   int32_t *sv;  // [6][G2] per-slot Vo, Vss, Va, Vb, Vnv, Vtc (kept only when totals may round)
+  uint32_t *spill_slot, *spill_key, *spill_rank;
   int32_t pool;
-  int32_t inl;  // inline entries per slot (>= kSmall)
+  int32_t inl;    // inline entries per slot (>= kSmall)
+  int32_t spill;  // spill list capacity
 };
 
 struct Acc {
@@ -115,82 +117,104 @@ __device__ __forceinline__ int32_t to_i16(float f, int32_t &flags) {
   return (int32_t)(int16_t)i;
 }
 
+// Entry of slot s at position k: inline layer or overflow pool.
+__device__ __forceinline__ void place_entry(const Tables &t, int32_t G2, uint32_t *pool_top, int32_t s,
+                                            uint32_t k, uint32_t key, uint32_t rank) {
+  if (k < (uint32_t)t.inl) {
+    t.ent[(int64_t)k * G2 + s] = make_uint2(key, rank);
+  } else {
+    const uint32_t j = atomicAdd(pool_top, 1u);
+    if (j < (uint32_t)t.pool) {
+      t.pool_slot[j] = (uint32_t)s;
+      t.pool_key[j] = key;
+      t.pool_rank[j] = rank;
+    }
+  }
+}
+
+// SetBit's slot append (STomoVoxelSpaceInfo.cpp:175-230) for one rasteriser tile.  The tile's
+// slot counters live in LDS, 16 bits each, two to a word; an entry whose slot lies outside the
+// tile (the interpolated point can leave the pixel it was tested at, and a centroid pixel can
+// leave the owning tile) goes to the orientation's spill list, merged after the rasteriser.
 struct Emit {
   int32_t G, G2;
   Tables t;
-  uint32_t *pool_top, *big_top;  // this orientation's counters (global)
+  uint32_t *pool_top, *spill_top;  // this orientation's counters (global)
+  uint32_t *lcnt;                  // the tile's LDS counters
+  int32_t xa, xb, ya, yb, yshift;  // the tile's slots: [xa, xb) x [ya, yb)
   int32_t bx0, bx1, by0, by1;
   int32_t flags;
   uint64_t zabs;  // sum of |z| of this lane's entries
 
-  // SetBit's slot addressing (coord2vID, STomoVoxelSpaceInfo.cpp:97-105) + append to the slot.
-  __device__ __forceinline__ int32_t put(float px, float py, float pz, float nz, uint32_t rank) {
-    int32_t sx = (int32_t)px, sy = (int32_t)py;
+  // coord2vID (STomoVoxelSpaceInfo.cpp:97-105) with the footprint check.
+  __device__ __forceinline__ bool slot_xy(float px, float py, int32_t &sx, int32_t &sy) {
+    sx = (int32_t)px;
+    sy = (int32_t)py;
     if (!(px > -2147483648.0f && px < 2147483648.0f && py > -2147483648.0f && py < 2147483648.0f) ||
         sx < 0 || sy < 0 || sx >= G || sy >= G) {
       flags |= F_FOOTPRINT_OOB;
-      return -1;
+      return false;
     }
-    int32_t z16 = to_i16(pz, flags);
-    int32_t nz16 = to_i16(nz * kNormFactor, flags);
-    zabs += (uint64_t)(z16 < 0 ? -z16 : z16);
-    uint32_t key = make_key(z16, nz16);
-    int32_t s = sx * G + sy;
     bx0 = min(bx0, sx); bx1 = max(bx1, sx);
     by0 = min(by0, sy); by1 = max(by1, sy);
-    uint32_t k = atomicAdd(&t.count[s], 1u);
-    if (k < (uint32_t)t.inl) {
-      t.ent[(int64_t)k * G2 + s] = make_uint2(key, rank);
-    } else {
-      uint32_t j = atomicAdd(pool_top, 1u);
-      if (j < (uint32_t)t.pool) {
-        t.pool_slot[j] = (uint32_t)s;
-        t.pool_key[j] = key;
-        t.pool_rank[j] = rank;
-      }
+    return true;
+  }
+
+  __device__ __forceinline__ void spill(int32_t s, uint32_t key, uint32_t rank) {
+    const uint32_t j = atomicAdd(spill_top, 1u);
+    if (j < (uint32_t)t.spill) {
+      t.spill_slot[j] = (uint32_t)s;
+      t.spill_key[j] = key;
+      t.spill_rank[j] = rank;
     }
-    if (k == (uint32_t)kSmall) {
-      uint32_t b = atomicAdd(big_top, 1u);
-      t.big[b] = s;
-    }
+  }
+
+  // Reserve n positions in an in-tile slot's LDS counter; returns the first.
+  __device__ __forceinline__ uint32_t reserve(int32_t sx, int32_t sy, uint32_t n) {
+    const uint32_t li = ((uint32_t)(sx - xa) << yshift) + (uint32_t)(sy - ya);
+    const uint32_t sh = (li & 1u) << 4;
+    const uint32_t old = atomicAdd(&lcnt[li >> 1], n << sh);
+    const uint32_t k = (old >> sh) & 0xffffu;
+    if (k > 0xffffu - n) flags |= F_UNRESOLVED;  // the 16-bit counter would carry
+    return k;
+  }
+
+  __device__ __forceinline__ bool in_tile(int32_t sx, int32_t sy) const {
+    return sx >= xa && sx < xb && sy >= ya && sy < yb;
+  }
+
+  __device__ __forceinline__ int32_t put(float px, float py, float pz, float nz, uint32_t rank) {
+    int32_t sx, sy;
+    if (!slot_xy(px, py, sx, sy)) return -1;
+    const int32_t z16 = to_i16(pz, flags);
+    const int32_t nz16 = to_i16(nz * kNormFactor, flags);
+    zabs += (uint64_t)(z16 < 0 ? -z16 : z16);
+    const uint32_t key = make_key(z16, nz16);
+    const int32_t s = sx * G + sy;
+    if (in_tile(sx, sy))
+      place_entry(t, G2, pool_top, s, reserve(sx, sy, 1u), key, rank);
+    else
+      spill(s, key, rank);
     return s;
   }
 
-  // Entry k of slot s: inline layer or overflow pool.
-  __device__ __forceinline__ void place(int32_t s, uint32_t k, uint32_t key, uint32_t rank) {
-    if (k < (uint32_t)t.inl) {
-      t.ent[(int64_t)k * G2 + s] = make_uint2(key, rank);
-    } else {
-      uint32_t j = atomicAdd(pool_top, 1u);
-      if (j < (uint32_t)t.pool) {
-        t.pool_slot[j] = (uint32_t)s;
-        t.pool_key[j] = key;
-        t.pool_rank[j] = rank;
-      }
-    }
-  }
-
-  // A shell pixel and its sub-pixel (same slot, ranks r and r + 1) with one counter atomic.
+  // A shell pixel and its sub-pixel (same slot, ranks r and r + 1) with one counter update.
   __device__ __forceinline__ int32_t put2(float px, float py, float pz, float nz, float pz2, float nz2,
                                           uint32_t rank) {
-    int32_t sx = (int32_t)px, sy = (int32_t)py;
-    if (!(px > -2147483648.0f && px < 2147483648.0f && py > -2147483648.0f && py < 2147483648.0f) ||
-        sx < 0 || sy < 0 || sx >= G || sy >= G) {
-      flags |= F_FOOTPRINT_OOB;
-      return -1;
-    }
+    int32_t sx, sy;
+    if (!slot_xy(px, py, sx, sy)) return -1;
     const int32_t z1 = to_i16(pz, flags), n1 = to_i16(nz * kNormFactor, flags);
     const int32_t z2 = to_i16(pz2, flags), n2 = to_i16(nz2 * kNormFactor, flags);
     zabs += (uint64_t)(z1 < 0 ? -z1 : z1) + (uint64_t)(z2 < 0 ? -z2 : z2);
     const int32_t s = sx * G + sy;
-    bx0 = min(bx0, sx); bx1 = max(bx1, sx);
-    by0 = min(by0, sy); by1 = max(by1, sy);
-    const uint32_t k = atomicAdd(&t.count[s], 2u);
-    place(s, k, make_key(z1, n1), rank);
-    place(s, k + 1u, make_key(z2, n2), rank + 1u);
-    if (k == (uint32_t)kSmall || k + 1u == (uint32_t)kSmall) {
-      uint32_t b = atomicAdd(big_top, 1u);
-      t.big[b] = s;
+    const uint32_t k1 = make_key(z1, n1), k2 = make_key(z2, n2);
+    if (in_tile(sx, sy)) {
+      const uint32_t k = reserve(sx, sy, 2u);
+      place_entry(t, G2, pool_top, s, k, k1, rank);
+      place_entry(t, G2, pool_top, s, k + 1u, k2, rank + 1u);
+    } else {
+      spill(s, k1, rank);
+      spill(s, k2, rank + 1u);
     }
     return s;
   }
@@ -430,8 +454,12 @@ __device__ __forceinline__ Tables tables_for(const SweepParams &p, int32_t b) {
   t.cmp_key = (uint32_t *)(base + p.lay.cmp_key);
   t.cmp_rank = (uint32_t *)(base + p.lay.cmp_rank);
   t.sv = (int32_t *)(base + p.lay.sv);
+  t.spill_slot = (uint32_t *)(base + p.lay.spill_slot);
+  t.spill_key = (uint32_t *)(base + p.lay.spill_key);
+  t.spill_rank = (uint32_t *)(base + p.lay.spill_rank);
   t.pool = p.lay.pool;
   t.inl = p.lay.inl;
+  t.spill = p.lay.spill;
   return t;
 }
 
@@ -516,7 +544,7 @@ __global__ __launch_bounds__(kThreads) void aabb_kernel(SweepParams p) {
     }
     const float kx = mx - p.bed_out, ky = my - p.bed_out, kz = mz;
     h.bbox[0] = p.grid; h.bbox[1] = -1; h.bbox[2] = p.grid; h.bbox[3] = -1;
-    h.big_top = 0u; h.pool_top = 0u; h.flags = 0;
+    h.big_top = 0u; h.pool_top = 0u; h.flags = 0; h.spill_top = 0u;
     h.kx = kx; h.ky = ky; h.kz = kz;
     h.zabs = 0ull;
     DevRecord &R = p.rec[b];
@@ -525,181 +553,293 @@ __global__ __launch_bounds__(kThreads) void aabb_kernel(SweepParams p) {
   }
 }
 
-// ---- raster_kernel: tri_per_thread * 256 triangles of one orientation per work-group -------
-// Pixelize + triVoxel (STomoSh_INT3.cpp:285-364, 1138-1209).  Each lane owns one triangle and
-// walks its pixel box with the cheap prefilter (tomo_raster.h); surviving pixels are queued in
-// LDS and the exact coverage test, interpolation and slot append run on compacted lanes.
-// Work-groups follow xcd_map so an orientation's slot table stays in one XCD's L2.
-// Triangle record in LDS (odd stride: lanes reading different owners hit different banks).
+// ---- raster_kernel: one tile of slots of one orientation per work-group -------------------
+// Pixelize + triVoxel (STomoSh_INT3.cpp:285-364, 1138-1209).  The work-group owns the slot
+// counters of a tile (in LDS) and rasterises every triangle whose pixel box meets the tile:
+// its waves cull 64 chunks at a time, dropping chunks whose projected box misses the tile, and
+// take the kept chunks in turn, one triangle per lane.  A lane walks its pixel box (clipped to the
+// tile) with the affine prefilter (tomo_raster.h); surviving pixels are queued in LDS and the
+// exact coverage test, interpolation and slot append run on compacted lanes.  Every pixel is
+// tested by exactly one tile; the edge tiles also take the pixels beyond the grid on their side.
+// A triangle's zero-hit fallback (its centroid pixel) is decided by its owner tile, the one
+// holding its clamped box corner (x0, y0), which walks the whole box to count hits.
+// Work-groups follow xcd_map so an orientation's tiles share one XCD's L2.
 enum : int { TR_V0 = 0, TR_V1 = 3, TR_V2 = 6, TR_NZ = 9, TR_DEN = 12, TR_D00 = 13, TR_D01 = 14,
              TR_D11 = 15, TR_C = 16, TR_STRIDE = 19 };
 
 struct RasterLds {
-  // margined v0, v1, v2 (x, y, z), n0z, n1z, n2z, den, d00, d01, d11, margined centroid (x, y, z)
-  float tri[kWaves][64][TR_STRIDE];
-  uint32_t qmeta[kWaves][128];       // owner lane | (x + 256) << 6 | (y + 256) << 19
-  int32_t hits[kWaves][64];
-  int32_t red[kWaves][5];
-  uint64_t zred[kWaves];
+  uint32_t cnt[kTileSlots / 2];  // the tile's slot counters, 16 bits each
+  // margined v0, v1, v2 (x, y, z), n0z, n1z, n2z, den, d00, d01, d11, margined centroid (x, y, z);
+  // odd stride: lanes reading different owners hit different banks
+  float tri[kTileWaves][64][TR_STRIDE];
+  uint32_t qmeta[kTileWaves][128];  // owner lane | (x + 256) << 6 | (y + 256) << 19
+  int32_t hits[kTileWaves][64];
+  int32_t red[kTileWaves][5];
+  uint64_t zred[kTileWaves];
 };
 
-__global__ __launch_bounds__(kThreads) void raster_kernel(SweepParams p) {
+// Conservative pixel range along one axis of a chunk's vertices after rotation (row r of the
+// orientation matrix about the centre c) and placement (minus k): the float rotation and
+// placement of each vertex stay within 1e-6 (|r| . |t| + |k|) of the real-number values, and
+// the rasteriser's pixel box of a triangle lies within [floor(min), floor(max) + 1].
+__device__ __forceinline__ void chunk_span(const float *bmin, const float *bmax, const float *c,
+                                           float r0, float r1, float r2, float k, int32_t &lo,
+                                           int32_t &hi) {
+  const float r[3] = {r0, r1, r2};
+  float mn = 0.0f, mx = 0.0f, mag = fabsf(k);
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    const float a = r[q] * (bmin[q] - c[q]), b = r[q] * (bmax[q] - c[q]);
+    mn += fminf(a, b);
+    mx += fmaxf(a, b);
+    mag += fmaxf(fabsf(a), fabsf(b));
+  }
+  const float pad = 2.0f + 1e-6f * mag;
+  const float flo = fminf(fmaxf(mn - k - pad, -1e6f), 1e6f);
+  const float fhi = fminf(fmaxf(mx - k + pad, -1e6f), 1e6f);
+  lo = (int32_t)floorf(flo);
+  hi = (int32_t)floorf(fhi) + 1;
+  if (!(mn == mn) || !(mx == mx) || !(mag <= 3.0e38f)) {  // NaN or overflow: keep the chunk
+    lo = -(1 << 30);
+    hi = 1 << 30;
+  }
+}
+
+__global__ __launch_bounds__(kTileThreads) void raster_kernel(SweepParams p) {
   __shared__ RasterLds L;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  int32_t b, chunk;
-  if (!xcd_map(p.tri_groups, p.n_batch, b, chunk)) return;
+  int32_t b, tile;
+  if (!xcd_map(p.tiles, p.n_batch, b, tile)) return;
   const int32_t G = p.grid;
+  const int32_t ti = tile / p.tiles_y, tj = tile - ti * p.tiles_y;
+  const int32_t xa = ti * p.tile_x, xb = min(xa + p.tile_x, G);
+  const int32_t ya = tj << p.tile_y_shift, yb = min(ya + (1 << p.tile_y_shift), G);
+  // pixels this tile tests: its slots, and beyond the grid on the sides where it is the edge
+  const int32_t exlo = ti == 0 ? INT32_MIN : xa, exhi = ti == p.tiles_x - 1 ? INT32_MAX : xb - 1;
+  const int32_t eylo = tj == 0 ? INT32_MIN : ya, eyhi = tj == p.tiles_y - 1 ? INT32_MAX : yb - 1;
+  for (int i = tid; i < kTileSlots / 2; i += kTileThreads) L.cnt[i] = 0u;
+  __syncthreads();
   OrientHdr &h = p.hdr[b];
   const float kx = h.kx, ky = h.ky, kz = h.kz;
+  const float *Rg = p.rot + (int64_t)b * 9;
+  const float R0 = Rg[0], R1 = Rg[1], R2 = Rg[2], R3 = Rg[3], R4 = Rg[4], R5 = Rg[5];
+  const float ctr[3] = {p.cx, p.cy, p.cz};
   const float4 *pv = p.pv + (int64_t)b * p.n_vtx;
   Emit em;
   em.G = G; em.G2 = G * G; em.t = tables_for(p, b);
-  em.pool_top = &h.pool_top; em.big_top = &h.big_top;
+  em.pool_top = &h.pool_top; em.spill_top = &h.spill_top;
+  em.lcnt = L.cnt;
+  em.xa = xa; em.xb = xb; em.ya = ya; em.yb = yb; em.yshift = p.tile_y_shift;
   em.bx0 = G; em.bx1 = -1; em.by0 = G; em.by1 = -1; em.flags = 0; em.zabs = 0ull;
   const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-  const int32_t first = chunk * p.tri_per_thread * kThreads;
   float (*tri)[TR_STRIDE] = L.tri[wave];
-#pragma unroll 1
-  for (int k = 0; k < p.tri_per_thread; k++) {
-    const int32_t tr = first + k * kThreads + tid;
-    const bool active = tr < p.n_tri;
-    int32_t x0 = 0, x1 = -1, y0 = 0, y1 = -1;
-    TriScan sc = {};
-    float v0x = 0.0f, v0y = 0.0f;
-    if (active) {
-      float a[3][3], nzv[3];
-      const int32_t ids[3] = {p.tri[3 * tr + 0], p.tri[3 * tr + 1], p.tri[3 * tr + 2]};
-#pragma unroll
-      for (int q = 0; q < 3; q++) {
-        const float4 r = pv[ids[q]];
-        a[q][0] = r.x - kx;
-        a[q][1] = r.y - ky;
-        a[q][2] = r.z - kz;
-        nzv[q] = r.w;
-      }
-      v0x = a[0][0] + kEps; v0y = a[0][1] + kEps;
-      const float v1x = a[1][0] + kEps, v1y = a[1][1] + kEps;
-      const float v2x = a[2][0] + kEps, v2y = a[2][1] + kEps;
-      float mnx = (v0x <= v1x) ? v0x : v1x; mnx = (mnx <= v2x) ? mnx : v2x;
-      float mny = (v0y <= v1y) ? v0y : v1y; mny = (mny <= v2y) ? mny : v2y;
-      float mxx = (v0x >= v1x) ? v0x : v1x; mxx = (mxx >= v2x) ? mxx : v2x;
-      float mxy = (v0y >= v1y) ? v0y : v1y; mxy = (mxy >= v2y) ? mxy : v2y;
-      x0 = (int32_t)mnx; y0 = (int32_t)mny; x1 = (int32_t)mxx; y1 = (int32_t)mxy;
-      // Pixels far outside the grid only yield dropped entries: bound the loop so a wild vertex
-      // cannot stall the group, and report the footprint overflow.
-      const int32_t lo = -64, hi = G + 64;
-      if (x0 < lo || y0 < lo || x1 > hi || y1 > hi) {
-        em.flags |= F_FOOTPRINT_OOB;
-        x0 = max(x0, lo); y0 = max(y0, lo); x1 = min(x1, hi); y1 = min(y1, hi);
-      }
-      const TriSetup ts = tri_setup(v0x, v0y, v1x, v1y, v2x, v2y);
-      if (tri_degenerate(ts)) { x1 = x0 - 1; }  // no pixel can pass
-      const float ex = fmaxf(fabsf(centre_offset(x0, p.h2, v0x)), fabsf(centre_offset(x1, p.h2, v0x)));
-      const float ey = fmaxf(fabsf(centre_offset(y0, p.h2, v0y)), fabsf(centre_offset(y1, p.h2, v0y)));
-      sc = tri_scan(ts, ex, ey);
-      float *rec = tri[lane];
-      rec[TR_V0 + 0] = v0x; rec[TR_V0 + 1] = v0y; rec[TR_V0 + 2] = a[0][2] + kEps;
-      rec[TR_V1 + 0] = v1x; rec[TR_V1 + 1] = v1y; rec[TR_V1 + 2] = a[1][2] + kEps;
-      rec[TR_V2 + 0] = v2x; rec[TR_V2 + 1] = v2y; rec[TR_V2 + 2] = a[2][2] + kEps;
-      rec[TR_NZ + 0] = nzv[0]; rec[TR_NZ + 1] = nzv[1]; rec[TR_NZ + 2] = nzv[2];
-      rec[TR_DEN] = ts.den; rec[TR_D00] = ts.d00; rec[TR_D01] = ts.d01; rec[TR_D11] = ts.d11;
-      // the un-margined centroid pixel a triangle without hits falls back to
-      // (STomoSh_INT3.cpp:1190-1206)
-#pragma unroll
-      for (int q = 0; q < 3; q++) rec[TR_C + q] = (a[0][q] + a[1][q] + a[2][q]) * 0.333333f + kEps;
-    }
-    L.hits[wave][lane] = 0;
-    int32_t x = x0, y = y0;
-    bool done = !active || x1 < x0 || y1 < y0;
-    float e2x = centre_offset(x, p.h2, v0x);
-    float rax = e2x * sc.ax, rbx = e2x * sc.bx;
-    uint32_t qn = 0;
-    wave_sync();
+  // The wave's chunks: every wave culls the same 64 chunks at a time and the kept ones are dealt
+  // out to the waves in turn.
+  int32_t it_c0 = -64, it_turn = 0;
+  uint64_t it_kept = 0ull;
+  auto next_chunk = [&]() -> int32_t {
     for (;;) {
-      bool cand = false;
-      const int32_t cxi = x, cyi = y;
-      if (!done) {
-        cand = scan_candidate(sc, rax, rbx, centre_offset(y, p.h2, v0y));
-        if (++y > y1) {
-          y = y0;
-          if (++x > x1) {
-            done = true;
-          } else {
-            e2x = centre_offset(x, p.h2, v0x);
-            rax = e2x * sc.ax;
-            rbx = e2x * sc.bx;
-          }
+      while (it_kept) {
+        const int cl = __ffsll((unsigned long long)it_kept) - 1;
+        it_kept &= it_kept - 1ull;
+        if ((it_turn++ & (kTileWaves - 1)) == wave) return it_c0 + cl;
+      }
+      it_c0 += 64;
+      if (it_c0 >= p.n_chunks) return -1;
+      bool keep = false;
+      const int32_t ch = it_c0 + lane;
+      if (ch < p.n_chunks) {
+        const float *bx = p.chunk_box + (int64_t)ch * 6;
+        const float bmin[3] = {bx[0], bx[1], bx[2]}, bmax[3] = {bx[3], bx[4], bx[5]};
+        int32_t lx, hx, ly, hy;
+        chunk_span(bmin, bmax, ctr, R0, R1, R2, kx, lx, hx);
+        chunk_span(bmin, bmax, ctr, R3, R4, R5, ky, ly, hy);
+        keep = hx >= exlo && lx <= exhi && hy >= eylo && ly <= eyhi;
+      }
+      it_kept = __ballot(keep);
+    }
+  };
+  for (int32_t ch = next_chunk(); ch >= 0; ch = next_chunk()) {
+    {
+      const int32_t tr = ch * kChunk + lane;
+      const bool active = tr < p.n_tri;
+      int32_t x0 = 0, x1 = -1, y0 = 0, y1 = -1;
+      bool owner = false;
+      TriScan sc = {};
+      float v0x = 0.0f, v0y = 0.0f;
+      uint32_t my_rank = 0u;
+      if (active) {
+        float a[3][3], nzv[3];
+        const int32_t ids[3] = {p.tri[3 * tr + 0], p.tri[3 * tr + 1], p.tri[3 * tr + 2]};
+        my_rank = 2u * (uint32_t)p.tri_id[tr];
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+          const float4 r = pv[ids[q]];
+          a[q][0] = r.x - kx;
+          a[q][1] = r.y - ky;
+          a[q][2] = r.z - kz;
+          nzv[q] = r.w;
         }
+        v0x = a[0][0] + kEps; v0y = a[0][1] + kEps;
+        const float v1x = a[1][0] + kEps, v1y = a[1][1] + kEps;
+        const float v2x = a[2][0] + kEps, v2y = a[2][1] + kEps;
+        float mnx = (v0x <= v1x) ? v0x : v1x; mnx = (mnx <= v2x) ? mnx : v2x;
+        float mny = (v0y <= v1y) ? v0y : v1y; mny = (mny <= v2y) ? mny : v2y;
+        float mxx = (v0x >= v1x) ? v0x : v1x; mxx = (mxx >= v2x) ? mxx : v2x;
+        float mxy = (v0y >= v1y) ? v0y : v1y; mxy = (mxy >= v2y) ? mxy : v2y;
+        x0 = (int32_t)mnx; y0 = (int32_t)mny; x1 = (int32_t)mxx; y1 = (int32_t)mxy;
+        // Pixels far outside the grid only yield dropped entries: bound the loop so a wild
+        // vertex cannot stall the group, and report the footprint overflow.
+        const int32_t lo = -64, hi = G + 64;
+        if (x0 < lo || y0 < lo || x1 > hi || y1 > hi) {
+          em.flags |= F_FOOTPRINT_OOB;
+          x0 = max(x0, lo); y0 = max(y0, lo); x1 = min(x1, hi); y1 = min(y1, hi);
+        }
+        const int32_t ox = min(max(x0, 0), G - 1), oy = min(max(y0, 0), G - 1);
+        owner = ox >= xa && ox < xb && oy >= ya && oy < yb;
+        // The owner walks the whole box when it leaves the tile (hits decide the fallback);
+        // every other walk is clipped to the pixels this tile tests.
+        if (!(owner && (x0 < exlo || x1 > exhi || y0 < eylo || y1 > eyhi))) {
+          x0 = max(x0, exlo); x1 = min(x1, exhi); y0 = max(y0, eylo); y1 = min(y1, eyhi);
+        }
+        const TriSetup ts = tri_setup(v0x, v0y, v1x, v1y, v2x, v2y);
+        if (tri_degenerate(ts)) { x1 = x0 - 1; }  // no pixel can pass
+        const float ex = fmaxf(fabsf(centre_offset(x0, p.h2, v0x)), fabsf(centre_offset(x1, p.h2, v0x)));
+        const float ey = fmaxf(fabsf(centre_offset(y0, p.h2, v0y)), fabsf(centre_offset(y1, p.h2, v0y)));
+        sc = tri_scan(ts, ex, ey);
+        float *rec = tri[lane];
+        rec[TR_V0 + 0] = v0x; rec[TR_V0 + 1] = v0y; rec[TR_V0 + 2] = a[0][2] + kEps;
+        rec[TR_V1 + 0] = v1x; rec[TR_V1 + 1] = v1y; rec[TR_V1 + 2] = a[1][2] + kEps;
+        rec[TR_V2 + 0] = v2x; rec[TR_V2 + 1] = v2y; rec[TR_V2 + 2] = a[2][2] + kEps;
+        rec[TR_NZ + 0] = nzv[0]; rec[TR_NZ + 1] = nzv[1]; rec[TR_NZ + 2] = nzv[2];
+        rec[TR_DEN] = ts.den; rec[TR_D00] = ts.d00; rec[TR_D01] = ts.d01; rec[TR_D11] = ts.d11;
+        // the un-margined centroid pixel a triangle without hits falls back to
+        // (STomoSh_INT3.cpp:1190-1206)
+#pragma unroll
+        for (int q = 0; q < 3; q++) rec[TR_C + q] = (a[0][q] + a[1][q] + a[2][q]) * 0.333333f + kEps;
       }
-      const uint64_t m = __ballot(cand);
-      if (cand) {
-        const uint32_t pos = qn + (uint32_t)__popcll(m & lt);
-        L.qmeta[wave][pos] = (uint32_t)lane | ((uint32_t)(cxi + 256) << 6) | ((uint32_t)(cyi + 256) << 19);
-      }
-      qn += (uint32_t)__popcll(m);
-      const bool all_done = __ballot(!done) == 0ull;
-      if (qn >= 64u || (all_done && qn > 0u)) {
-        wave_sync();
-        const uint32_t np = qn < 64u ? qn : 64u;
-        if ((uint32_t)lane < np) {
-          const uint32_t meta = L.qmeta[wave][lane];
-          const int32_t owner = (int32_t)(meta & 63u);
-          const int32_t px_i = (int32_t)((meta >> 6) & 8191u) - 256;
-          const int32_t py_i = (int32_t)(meta >> 19) - 256;
-          const float *rec = tri[owner];
-          // the exact test's numerators, formed as _getBaryCoord forms them
-          TriSetup ts;
-          ts.v0x = rec[TR_V0 + 0]; ts.v0y = rec[TR_V0 + 1];
-          ts.e0x = rec[TR_V1 + 0] - ts.v0x; ts.e0y = rec[TR_V1 + 1] - ts.v0y;
-          ts.e1x = rec[TR_V2 + 0] - ts.v0x; ts.e1y = rec[TR_V2 + 1] - ts.v0y;
-          ts.d00 = rec[TR_D00]; ts.d01 = rec[TR_D01]; ts.d11 = rec[TR_D11]; ts.den = rec[TR_DEN];
-          float nv, nw, u, v, w;
-          bary_num(ts, ((float)px_i + kEps) + p.h2, ((float)py_i + kEps) + p.h2, nv, nw);
-          if (bary_exact(ts.den, nv, nw, u, v, w)) {
-            // _bary_product (Tomo_types.cpp:75-84) of the margined vertices and the normals
-            const float px = rec[TR_V0 + 0] * u + rec[TR_V1 + 0] * v + rec[TR_V2 + 0] * w;
-            const float py = rec[TR_V0 + 1] * u + rec[TR_V1 + 1] * v + rec[TR_V2 + 1] * w;
-            float pz = rec[TR_V0 + 2] * u + rec[TR_V1 + 2] * v + rec[TR_V2 + 2] * w;
-            float nz = rec[TR_NZ + 0] * u + rec[TR_NZ + 1] * v + rec[TR_NZ + 2] * w;
-            const uint32_t rank = 2u * (uint32_t)(first + k * kThreads + wave * 64 + owner);
-            int32_t s;
-            if (p.do_shell) {  // with its shell sub-pixel (STomoSh_INT3.cpp:1179-1185)
-              const float pz2 = pz + ((nz >= 0.0f) ? -p.shell_off : p.shell_off);
-              s = em.put2(px, py, pz, nz, pz2, -nz, rank);
+      L.hits[wave][lane] = 0;
+      int32_t x = x0, y = y0;
+      bool done = !active || x1 < x0 || y1 < y0;
+      float e2x = centre_offset(x, p.h2, v0x);
+      float rax = e2x * sc.ax, rbx = e2x * sc.bx;
+      uint32_t qn = 0;
+      wave_sync();
+      for (;;) {
+        bool cand = false;
+        const int32_t cxi = x, cyi = y;
+        if (!done) {
+          cand = scan_candidate(sc, rax, rbx, centre_offset(y, p.h2, v0y));
+          if (++y > y1) {
+            y = y0;
+            if (++x > x1) {
+              done = true;
             } else {
-              s = em.put(px, py, pz, nz, rank);
+              e2x = centre_offset(x, p.h2, v0x);
+              rax = e2x * sc.ax;
+              rbx = e2x * sc.bx;
             }
-            if (s >= 0 && s != px_i * G + py_i) em.flags |= F_RANK_TIE;
-            L.hits[wave][owner] = 1;
           }
         }
-        wave_sync();
-        if (qn > 64u) {
-          const uint32_t rest = qn - 64u;
-          uint32_t mm = 0u;
-          if ((uint32_t)lane < rest) mm = L.qmeta[wave][64 + lane];
-          wave_sync();
-          if ((uint32_t)lane < rest) L.qmeta[wave][lane] = mm;
-          wave_sync();
+        const uint64_t m = __ballot(cand);
+        if (cand) {
+          const uint32_t pos = qn + (uint32_t)__popcll(m & lt);
+          L.qmeta[wave][pos] = (uint32_t)lane | ((uint32_t)(cxi + 256) << 6) | ((uint32_t)(cyi + 256) << 19);
         }
-        qn -= np;
+        qn += (uint32_t)__popcll(m);
+        const bool all_done = __ballot(!done) == 0ull;
+        if (qn >= 64u || (all_done && qn > 0u)) {
+          wave_sync();
+          const uint32_t np = qn < 64u ? qn : 64u;
+          const uint32_t meta = (uint32_t)lane < np ? L.qmeta[wave][lane] : 0u;
+          const int32_t owner_lane = (int32_t)(meta & 63u);
+          const uint32_t rank = __shfl(my_rank, owner_lane);
+          if ((uint32_t)lane < np) {
+            const int32_t px_i = (int32_t)((meta >> 6) & 8191u) - 256;
+            const int32_t py_i = (int32_t)(meta >> 19) - 256;
+            const float *rec = tri[owner_lane];
+            // the exact test's numerators, formed as _getBaryCoord forms them
+            TriSetup ts;
+            ts.v0x = rec[TR_V0 + 0]; ts.v0y = rec[TR_V0 + 1];
+            ts.e0x = rec[TR_V1 + 0] - ts.v0x; ts.e0y = rec[TR_V1 + 1] - ts.v0y;
+            ts.e1x = rec[TR_V2 + 0] - ts.v0x; ts.e1y = rec[TR_V2 + 1] - ts.v0y;
+            ts.d00 = rec[TR_D00]; ts.d01 = rec[TR_D01]; ts.d11 = rec[TR_D11]; ts.den = rec[TR_DEN];
+            float nv, nw, u, v, w;
+            bary_num(ts, ((float)px_i + kEps) + p.h2, ((float)py_i + kEps) + p.h2, nv, nw);
+            if (bary_exact(ts.den, nv, nw, u, v, w)) {
+              L.hits[wave][owner_lane] = 1;
+              if (px_i >= exlo && px_i <= exhi && py_i >= eylo && py_i <= eyhi) {
+                // _bary_product (Tomo_types.cpp:75-84) of the margined vertices and the normals
+                const float px = rec[TR_V0 + 0] * u + rec[TR_V1 + 0] * v + rec[TR_V2 + 0] * w;
+                const float py = rec[TR_V0 + 1] * u + rec[TR_V1 + 1] * v + rec[TR_V2 + 1] * w;
+                const float pz = rec[TR_V0 + 2] * u + rec[TR_V1 + 2] * v + rec[TR_V2 + 2] * w;
+                const float nz = rec[TR_NZ + 0] * u + rec[TR_NZ + 1] * v + rec[TR_NZ + 2] * w;
+                int32_t s;
+                if (p.do_shell) {  // with its shell sub-pixel (STomoSh_INT3.cpp:1179-1185)
+                  const float pz2 = pz + ((nz >= 0.0f) ? -p.shell_off : p.shell_off);
+                  s = em.put2(px, py, pz, nz, pz2, -nz, rank);
+                } else {
+                  s = em.put(px, py, pz, nz, rank);
+                }
+                if (s >= 0 && s != px_i * G + py_i) em.flags |= F_RANK_TIE;
+              }
+            }
+          }
+          wave_sync();
+          if (qn > 64u) {
+            const uint32_t rest = qn - 64u;
+            uint32_t mm = 0u;
+            if ((uint32_t)lane < rest) mm = L.qmeta[wave][64 + lane];
+            wave_sync();
+            if ((uint32_t)lane < rest) L.qmeta[wave][lane] = mm;
+            wave_sync();
+          }
+          qn -= np;
+        }
+        if (all_done && qn == 0u) break;
       }
-      if (all_done && qn == 0u) break;
+      wave_sync();
+      if (active && owner && L.hits[wave][lane] == 0) {
+        // very small triangle: one pixel at the un-margined centroid (STomoSh_INT3.cpp:1190-1206)
+        const float *rec = tri[lane];
+        const float tx = rec[TR_C + 0], ty = rec[TR_C + 1], tz = rec[TR_C + 2], n0 = rec[TR_NZ + 0];
+        if (p.do_shell) {
+          const float sz = tz + ((n0 >= 0.0f) ? -p.shell_off : p.shell_off);
+          em.put2(tx, ty, tz, n0, sz, -n0, my_rank);
+        } else {
+          em.put(tx, ty, tz, n0, my_rank);
+        }
+      }
+      wave_sync();
     }
-    wave_sync();
-    if (active && L.hits[wave][lane] == 0) {
-      // very small triangle: one pixel at the un-margined centroid (STomoSh_INT3.cpp:1190-1206)
-      const float *rec = tri[lane];
-      const float tx = rec[TR_C + 0], ty = rec[TR_C + 1], tz = rec[TR_C + 2], n0 = rec[TR_NZ + 0];
-      const uint32_t rank = 2u * (uint32_t)tr;
-      if (p.do_shell) {
-        const float sz = tz + ((n0 >= 0.0f) ? -p.shell_off : p.shell_off);
-        em.put2(tx, ty, tz, n0, sz, -n0, rank);
-      } else {
-        em.put(tx, ty, tz, n0, rank);
+  }
+  __syncthreads();
+  // Write the tile's non-zero counters (the global table is zero elsewhere) and list the slots
+  // with more than kSmall entries.
+  {
+    const Tables &t = em.t;
+    const int32_t yshift = p.tile_y_shift, ymask = (1 << yshift) - 1;
+    const int32_t nwords = ((xb - xa) << yshift) >> 1;
+    for (int32_t w0 = 0; w0 < nwords; w0 += kTileThreads) {
+      const int32_t wi = w0 + tid;
+      const uint32_t word = wi < nwords ? L.cnt[wi] : 0u;
+#pragma unroll
+      for (int half = 0; half < 2; half++) {
+        const uint32_t c = (word >> (16 * half)) & 0xffffu;
+        const int32_t li = 2 * wi + half;
+        const int32_t sx = xa + (li >> yshift), sy = ya + (li & ymask);
+        const int32_t s = sx * G + sy;
+        if (c) t.count[s] = c;
+        const bool big = c > (uint32_t)kSmall;
+        const uint64_t bm = __ballot(big);
+        if (bm) {
+          uint32_t base = 0u;
+          if (lane == 0) base = atomicAdd(&h.big_top, (uint32_t)__popcll(bm));
+          base = __shfl(base, 0);
+          if (big) t.big[base + (uint32_t)__popcll(bm & lt)] = s;
+        }
       }
     }
-    wave_sync();
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -718,7 +858,7 @@ __global__ __launch_bounds__(kThreads) void raster_kernel(SweepParams p) {
   __syncthreads();
   if (tid == 0) {
     int32_t v0 = G, v1 = -1, v2 = G, v3 = -1, f = 0;
-    for (int w = 0; w < kWaves; w++) {
+    for (int w = 0; w < kTileWaves; w++) {
       v0 = min(v0, L.red[w][0]); v1 = max(v1, L.red[w][1]);
       v2 = min(v2, L.red[w][2]); v3 = max(v3, L.red[w][3]);
       f |= L.red[w][4];
@@ -729,22 +869,36 @@ __global__ __launch_bounds__(kThreads) void raster_kernel(SweepParams p) {
     }
     if (f) atomicOr(&h.flags, f);
     uint64_t z = 0;
-    for (int w = 0; w < kWaves; w++) z += L.zred[w];
+    for (int w = 0; w < kTileWaves; w++) z += L.zred[w];
     if (z) atomicAdd(&h.zabs, (unsigned long long)z);
   }
 }
 
 // ---- compact_kernel: one work-group per orientation ----------------------------------------
-// Entries beyond the inline layers were appended to a pool in arrival order; group them per
-// slot (exclusive scan over the large-slot list, then scatter).
+// First appends the rasteriser's spilled entries to their slots (global counters: every tile
+// has written its counters back).  Entries beyond the inline layers were appended to a pool in
+// arrival order; group them per slot (exclusive scan over the large-slot list, then scatter).
 __global__ __launch_bounds__(kThreads) void compact_kernel(SweepParams p) {
   __shared__ uint32_t scan[kWaves];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int32_t b = blockIdx.x;
-  const OrientHdr &h = p.hdr[b];
-  const uint32_t npool = h.pool_top, nbig = h.big_top;
-  if (npool == 0u) return;
+  OrientHdr &h = p.hdr[b];
   const Tables t = tables_for(p, b);
+  const uint32_t nspill = h.spill_top;
+  if (nspill > 0u) {
+    const int32_t G2 = p.grid * p.grid;
+    const uint32_t ns = nspill < (uint32_t)t.spill ? nspill : (uint32_t)t.spill;
+    for (uint32_t j = tid; j < ns; j += kThreads) {
+      const int32_t s = (int32_t)t.spill_slot[j];
+      const uint32_t k = atomicAdd(&t.count[s], 1u);
+      place_entry(t, G2, &h.pool_top, s, k, t.spill_key[j], t.spill_rank[j]);
+      if (k == (uint32_t)kSmall) t.big[atomicAdd(&h.big_top, 1u)] = s;
+    }
+    __syncthreads();
+  }
+  const uint32_t npool = __hip_atomic_load(&h.pool_top, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  const uint32_t nbig = __hip_atomic_load(&h.big_top, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  if (npool == 0u) return;
   uint32_t running = 0;
   for (uint32_t b0 = 0; b0 < nbig; b0 += kThreads) {
     const uint32_t j = b0 + tid;
@@ -807,8 +961,8 @@ __global__ __launch_bounds__(kThreads) void pair_kernel(SweepParams p) {
   const int32_t i0 = tile * kPairTile;
   const int32_t nwaves = p.pair_tiles * kWaves;
   if (tile == 0 && tid == 0) {
-    // rasteriser flags, and pool overflow
-    const int32_t f0 = h.flags | (npool > (uint32_t)p.lay.pool ? F_UNRESOLVED : 0);
+    // rasteriser flags, and pool or spill list overflow
+    const int32_t f0 = h.flags | (npool > (uint32_t)p.lay.pool || h.spill_top > (uint32_t)p.lay.spill ? F_UNRESOLVED : 0);
     if (f0) atomicOr(&p.rec[b].flags, f0);
   }
   if (i0 >= nbb && (uint32_t)(tile * kWaves) >= nbig) return;  // nothing for this group
@@ -1151,7 +1305,7 @@ hipError_t launch_stage(int stage, const SweepParams &p, hipStream_t stream) {
   const int32_t nb8 = (nb + 7) / 8 * 8;  // xcd_map pads the orientation count to whole XCD rounds
   switch (stage) {
     case 0: hipLaunchKernelGGL(aabb_kernel, dim3(nb), dim3(kThreads), 0, stream, p); break;
-    case 1: hipLaunchKernelGGL(raster_kernel, dim3(nb8 * p.tri_groups), dim3(kThreads), 0, stream, p); break;
+    case 1: hipLaunchKernelGGL(raster_kernel, dim3(nb8 * p.tiles), dim3(kTileThreads), 0, stream, p); break;
     case 2: hipLaunchKernelGGL(compact_kernel, dim3(nb), dim3(kThreads), 0, stream, p); break;
     case 3: hipLaunchKernelGGL(pair_kernel, dim3(nb8 * p.pair_tiles), dim3(kThreads), 0, stream, p); break;
     case 4:
