@@ -198,6 +198,7 @@ struct tomo_handle {
   float *d_chunk = nullptr;     // [n_chunks][6] raw AABB of each kChunk triangles
   int32_t n_chunks = 0;
   float *d_soa = nullptr;  // 6 * n_vtx: x, y, z, nx, ny, nz
+  float4 *d_vaos = nullptr;  // 2 * n_vtx: (x, y, z, nx), (ny, nz, 0, 0)
   float *d_bed_dist = nullptr;
   // orientations
   int32_t n_orient = 0;
@@ -342,6 +343,7 @@ SweepParams base_params(tomo_handle *h) {
   p.tri_id = h->d_tri_id;
   p.chunk_box = h->d_chunk;
   p.n_chunks = h->n_chunks;
+  p.vaos = h->d_vaos;
   p.vx = h->d_soa;
   p.vy = h->d_soa + h->n_vtx;
   p.vz = h->d_soa + 2 * (size_t)h->n_vtx;
@@ -494,6 +496,7 @@ int tomo_set_mesh(tomo_handle *h, int32_t n_tri, const int32_t *tri, int32_t n_v
   if (h->d_tri_id) { HIP_TRY(hipFree(h->d_tri_id)); h->d_tri_id = nullptr; }
   if (h->d_chunk) { HIP_TRY(hipFree(h->d_chunk)); h->d_chunk = nullptr; }
   if (h->d_soa) { HIP_TRY(hipFree(h->d_soa)); h->d_soa = nullptr; }
+  if (h->d_vaos) { HIP_TRY(hipFree(h->d_vaos)); h->d_vaos = nullptr; }
   h->n_tri = n_tri;
   h->n_vtx = n_vtx;
   // raw AABB centre (STomoAABB3Df.cpp:42-72): float add, then *0.5 in double
@@ -513,6 +516,13 @@ int tomo_set_mesh(tomo_handle *h, int32_t n_tri, const int32_t *tri, int32_t n_v
     }
   HIP_TRY(hipMalloc(&h->d_soa, soa.size() * sizeof(float)));
   HIP_TRY(hipMemcpy(h->d_soa, soa.data(), soa.size() * sizeof(float), hipMemcpyHostToDevice));
+  std::vector<float> aos((size_t)8 * (n_vtx > 0 ? n_vtx : 1), 0.0f);
+  for (int32_t v = 0; v < n_vtx; v++) {
+    for (int k = 0; k < 3; k++) aos[(size_t)v * 8 + k] = vtx[(size_t)v * 3 + k];
+    for (int k = 0; k < 3; k++) aos[(size_t)v * 8 + 3 + k] = vtx_nrm[(size_t)v * 3 + k];
+  }
+  HIP_TRY(hipMalloc(&h->d_vaos, aos.size() * sizeof(float)));
+  HIP_TRY(hipMemcpy(h->d_vaos, aos.data(), aos.size() * sizeof(float), hipMemcpyHostToDevice));
   // Culling order: triangles sorted by the Morton code of their centroid, so each run of kChunk
   // triangles is compact in space and its AABB bounds where the rasteriser can touch it.
   std::vector<uint64_t> code((size_t)n_tri);
@@ -782,6 +792,7 @@ int tomo_destroy(tomo_handle *h) {
   if (h->d_tri_id) hipFree(h->d_tri_id);
   if (h->d_chunk) hipFree(h->d_chunk);
   if (h->d_soa) hipFree(h->d_soa);
+  if (h->d_vaos) hipFree(h->d_vaos);
   if (h->d_bed_dist) hipFree(h->d_bed_dist);
   if (h->ev0) hipEventDestroy(h->ev0);
   if (h->ev1) hipEventDestroy(h->ev1);

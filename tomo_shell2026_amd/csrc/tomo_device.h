@@ -68,6 +68,7 @@ struct SweepParams {
   const int32_t *tri_id;    // [n_tri] the mesh's own index of each triangle (entry ranks)
   const float *chunk_box;   // [n_chunks][6] raw-coordinate AABB (min xyz, max xyz) of kChunk triangles
   const float *vx, *vy, *vz, *nx, *ny, *nz;
+  const float4 *vaos;       // [n_vtx][2] raw (x, y, z, nx), (ny, nz, 0, 0)
   int32_t n_tri, n_vtx, n_chunks;
   float cx, cy, cz;        // raw AABB centre (STomoSh_Base.cpp:75-82)
   // orientations of this batch: rot/rec already offset to the batch's first orientation

@@ -15,6 +15,8 @@
 // follows the reference's operation order without contraction (built with -ffp-contract=off).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "tomo_device.h"
 #include "tomo_raster.h"
 
@@ -463,10 +465,11 @@ __device__ __forceinline__ Tables tables_for(const SweepParams &p, int32_t b) {
   return t;
 }
 
-// Sum N per-thread values over the work-group and add them to N global 64-bit counters.
+// Sum N per-thread values over the work-group and add them to the N consecutive 64-bit
+// counters at dst (zero sums are skipped).
 template <int N>
 __device__ __forceinline__ void group_add(int64_t (&v)[N], int64_t (*red)[N], int lane, int wave,
-                                          int tid, unsigned long long *const (&dst)[N]) {
+                                          int tid, unsigned long long *dst) {
 #pragma unroll
   for (int q = 0; q < N; q++) v[q] = wave_sum(v[q]);
   if (lane == 0) {
@@ -478,9 +481,14 @@ __device__ __forceinline__ void group_add(int64_t (&v)[N], int64_t (*red)[N], in
     int64_t s = 0;
 #pragma unroll
     for (int w = 0; w < kWaves; w++) s += red[w][tid];
-    if (s != 0) atomicAdd(dst[tid], (unsigned long long)s);
+    if (s != 0) atomicAdd(dst + tid, (unsigned long long)s);
   }
 }
+
+// DevRecord's 64-bit totals, in field order from Va.
+static_assert(offsetof(DevRecord, Vbed) == 6 * 8 && offsetof(DevRecord, n_final) == 9 * 8 &&
+                  offsetof(DevRecord, chk_final) == 11 * 8 && offsetof(DevRecord, aVtc) == 17 * 8,
+              "DevRecord totals must be consecutive 64-bit fields");
 
 __device__ __forceinline__ int32_t group_or(int32_t f, int32_t *red, int lane, int wave, int tid) {
 #pragma unroll
@@ -622,9 +630,9 @@ __global__ __launch_bounds__(kTileThreads) void raster_kernel(SweepParams p) {
   OrientHdr &h = p.hdr[b];
   const float kx = h.kx, ky = h.ky, kz = h.kz;
   const float *Rg = p.rot + (int64_t)b * 9;
-  const float R0 = Rg[0], R1 = Rg[1], R2 = Rg[2], R3 = Rg[3], R4 = Rg[4], R5 = Rg[5];
+  const float R0 = Rg[0], R1 = Rg[1], R2 = Rg[2], R3 = Rg[3], R4 = Rg[4], R5 = Rg[5], R6 = Rg[6],
+              R7 = Rg[7], R8 = Rg[8];
   const float ctr[3] = {p.cx, p.cy, p.cz};
-  const float4 *pv = p.pv + (int64_t)b * p.n_vtx;
   Emit em;
   em.G = G; em.G2 = G * G; em.t = tables_for(p, b);
   em.pool_top = &h.pool_top; em.spill_top = &h.spill_top;
@@ -672,13 +680,21 @@ __global__ __launch_bounds__(kTileThreads) void raster_kernel(SweepParams p) {
         float a[3][3], nzv[3];
         const int32_t ids[3] = {p.tri[3 * tr + 0], p.tri[3 * tr + 1], p.tri[3 * tr + 2]};
         my_rank = 2u * (uint32_t)p.tri_id[tr];
+        // Rotate the three vertices from the raw mesh (shared by every orientation, so it stays
+        // in L2) exactly as aabb_kernel rotates them (STomoSh_Base.cpp:85-101), then place them.
+        float4 va[3], vb[3];
 #pragma unroll
         for (int q = 0; q < 3; q++) {
-          const float4 r = pv[ids[q]];
-          a[q][0] = r.x - kx;
-          a[q][1] = r.y - ky;
-          a[q][2] = r.z - kz;
-          nzv[q] = r.w;
+          va[q] = p.vaos[2 * ids[q]];
+          vb[q] = p.vaos[2 * ids[q] + 1];
+        }
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+          const float t0 = va[q].x - ctr[0], t1 = va[q].y - ctr[1], t2 = va[q].z - ctr[2];
+          a[q][0] = dot_row(R0, R1, R2, t0, t1, t2) - kx;
+          a[q][1] = dot_row(R3, R4, R5, t0, t1, t2) - ky;
+          a[q][2] = dot_row(R6, R7, R8, t0, t1, t2) - kz;
+          nzv[q] = dot_row(R6, R7, R8, va[q].w, vb[q].x, vb[q].y);
         }
         v0x = a[0][0] + kEps; v0y = a[0][1] + kEps;
         const float v1x = a[1][0] + kEps, v1y = a[1][1] + kEps;
@@ -946,7 +962,8 @@ __global__ __launch_bounds__(kThreads) void compact_kernel(SweepParams p) {
 // Pairing (STomoSh_INT3.cpp:366-395) and the per-slot sums of Calculate (:1211-1255).
 __global__ __launch_bounds__(kThreads) void pair_kernel(SweepParams p) {
   __shared__ uint64_t sortbuf[kWaves][kBigCap];
-  __shared__ int64_t red[kWaves][17];
+  __shared__ uint32_t pairq[3][kWaves][128];  // queued slots: s | entries << 24
+  __shared__ int64_t red[kWaves][18];
   __shared__ int32_t fred[kWaves];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   int32_t b, tile;
@@ -970,44 +987,17 @@ __global__ __launch_bounds__(kThreads) void pair_kernel(SweepParams p) {
   const bool final_here = !p.bed_on;  // without a bed the final row count is known here
   int32_t *const sv = (p.exact_fsum && totals_may_round(h, p.pp)) ? t.sv : nullptr;
   Acc acc = {};
-  const int32_t i1 = min(i0 + kPairTile, nbb);
-  for (int32_t i = i0 + tid; i < i1; i += kThreads) {
-    const int32_t sx = bx0 + i / wy, sy = by0 + i % wy;
-    const int32_t s = sx * G + sy;
-    const uint32_t c = t.count[s];
-    if (c == 0u) {
-      t.bedinfo[s] = BI_POSS;
-      if (sv) {
-        const SlotResult z = {};
-        put_sv(sv, G2, s, z);
-      }
-      continue;
-    }
-    if (c > (uint32_t)kSmall) continue;  // a wave pairs it below
+  uint64_t *const dbg_rows = p.dbg_rows;
+  // Pair one slot of at most N entries in registers (one lane).
+  auto pair_small = [&](auto nt, int32_t s, uint32_t c) {
+    constexpr int N = decltype(nt)::value;
     t.count[s] = 0u;
     acc.nslot++;
+    uint32_t k[N];
+#pragma unroll
+    for (int q = 0; q < N; q++) k[q] = (q < (int)c) ? t.ent[(int64_t)q * G2 + s].x : 0u;
     SlotResult r;
-    uint64_t *const rows = p.dbg_rows ? p.dbg_rows + (int64_t)s * 64 : nullptr;
-    // Most slots hold two entries (one surface above, one below): size the register network to
-    // the largest slot among this wave's active lanes.
-    if (__ballot(c > 4u) == 0ull) {
-      if (__ballot(c > 2u) == 0ull) {
-        uint32_t k[2];
-#pragma unroll
-        for (int q = 0; q < 2; q++) k[q] = (q < (int)c) ? t.ent[(int64_t)q * G2 + s].x : 0u;
-        pair_keys<2>(k, (int)c, p.pp, r, rows);
-      } else {
-        uint32_t k[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) k[q] = (q < (int)c) ? t.ent[(int64_t)q * G2 + s].x : 0u;
-        pair_keys<4>(k, (int)c, p.pp, r, rows);
-      }
-    } else {
-      uint32_t k[kSmall];
-#pragma unroll
-      for (int q = 0; q < kSmall; q++) k[q] = (q < (int)c) ? t.ent[(int64_t)q * G2 + s].x : 0u;
-      pair_keys<kSmall>(k, (int)c, p.pp, r, rows);
-    }
+    pair_keys<N>(k, (int)c, p.pp, r, dbg_rows ? dbg_rows + (int64_t)s * 64 : nullptr);
     account(acc, r, s, p);
     t.bedinfo[s] = bed_info(r, p.pp);
     if (sv) put_sv(sv, G2, s, r);
@@ -1015,6 +1005,92 @@ __global__ __launch_bounds__(kThreads) void pair_kernel(SweepParams p) {
       acc.nfinal += r.n;
       acc.chkfinal += (uint64_t)(s + 1) * (uint64_t)r.n;
       if (p.dbg_final) p.dbg_final[s] = r.n;
+    }
+  };
+  // The touched box is walked 64 slots per wave step; slots of at most kSmall entries are
+  // queued by size class (<= 2, <= 4, <= 8) and paired 64 at a time with that class's register
+  // network, so no wave runs a larger network than its slots need.
+  uint32_t n2 = 0u, n4 = 0u, n8 = 0u;  // queue fills (wave-uniform)
+  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  uint32_t *const q2 = pairq[0][wave], *const q4 = pairq[1][wave], *const q8 = pairq[2][wave];
+  const int32_t i1 = min(i0 + kPairTile, nbb);
+  if (i0 + wave * 64 < i1) {
+    // slot i of the box is (bx0 + i / wy, by0 + i % wy); step kThreads slots at a time
+    const int32_t first = i0 + wave * 64 + lane;
+    int32_t sx = bx0 + first / wy, sy = by0 + first % wy;
+    const int32_t dq = kThreads / wy, dr = kThreads % wy;
+    for (int32_t base = i0 + wave * 64;; base += kThreads) {
+      const int32_t i = base + lane;
+      int cls = -1;
+      uint32_t e = 0u;
+      if (i < i1) {
+        const int32_t s = sx * G + sy;
+        const uint32_t c = t.count[s];
+        if (c == 0u) {
+          t.bedinfo[s] = BI_POSS;
+          if (sv) {
+            const SlotResult z = {};
+            put_sv(sv, G2, s, z);
+          }
+        } else if (c <= (uint32_t)kSmall) {  // larger ones are paired below from the big list
+          cls = c <= 2u ? 0 : (c <= 4u ? 1 : 2);
+          e = (uint32_t)s | (c << 24);
+        }
+      }
+      {
+        const uint64_t m = __ballot(cls == 0);
+        if (cls == 0) q2[n2 + (uint32_t)__popcll(m & lt)] = e;
+        n2 += (uint32_t)__popcll(m);
+      }
+      {
+        const uint64_t m = __ballot(cls == 1);
+        if (cls == 1) q4[n4 + (uint32_t)__popcll(m & lt)] = e;
+        n4 += (uint32_t)__popcll(m);
+      }
+      {
+        const uint64_t m = __ballot(cls == 2);
+        if (cls == 2) q8[n8 + (uint32_t)__popcll(m & lt)] = e;
+        n8 += (uint32_t)__popcll(m);
+      }
+      const bool last = base + kThreads >= i1;
+      // Pair full batches of 64 (and, after the last step, what remains), one class at a time:
+      // a single call site per network keeps the code (and the register demand) small.
+      for (;;) {
+        const uint32_t lim = last ? 1u : 64u;
+        const int fc = n2 >= lim ? 0 : (n4 >= lim ? 1 : (n8 >= lim ? 2 : -1));
+        if (fc < 0) break;
+        uint32_t *const q = fc == 0 ? q2 : (fc == 1 ? q4 : q8);
+        const uint32_t n = fc == 0 ? n2 : (fc == 1 ? n4 : n8);
+        const uint32_t np = n < 64u ? n : 64u;
+        wave_sync();
+        if ((uint32_t)lane < np) {
+          const uint32_t qe = q[lane];
+          const int32_t qs = (int32_t)(qe & 0xffffffu);
+          const uint32_t qc = qe >> 24;
+          if (fc == 0) pair_small(std::integral_constant<int, 2>{}, qs, qc);
+          else if (fc == 1) pair_small(std::integral_constant<int, 4>{}, qs, qc);
+          else pair_small(std::integral_constant<int, kSmall>{}, qs, qc);
+        }
+        wave_sync();
+        if (n > 64u) {
+          const uint32_t rest = n - 64u;
+          uint32_t mm = 0u;
+          if ((uint32_t)lane < rest) mm = q[64 + lane];
+          wave_sync();
+          if ((uint32_t)lane < rest) q[lane] = mm;
+          wave_sync();
+        }
+        if (fc == 0) n2 = n - np;
+        else if (fc == 1) n4 = n - np;
+        else n8 = n - np;
+      }
+      if (last) break;
+      sx += dq;
+      sy += dr;
+      if (sy > by1) {
+        sy -= wy;
+        sx += 1;
+      }
     }
   }
   // Large slots keep their counters set until the next aabb_kernel on this scratch region
@@ -1063,18 +1139,11 @@ __global__ __launch_bounds__(kThreads) void pair_kernel(SweepParams p) {
     }
   }
   DevRecord &R = p.rec[b];
-  int64_t v[17] = {acc.Va, acc.Vb, acc.Vo, acc.Vss, acc.Vnv, acc.Vtc, acc.nent, acc.nslot,
-                   (int64_t)acc.chkpre, acc.aVo, acc.aVss, acc.aVa, acc.aVb, acc.aVnv, acc.aVtc,
-                   acc.nfinal, (int64_t)acc.chkfinal};
-  unsigned long long *const dst[17] = {
-      (unsigned long long *)&R.Va, (unsigned long long *)&R.Vb, (unsigned long long *)&R.Vo,
-      (unsigned long long *)&R.Vss, (unsigned long long *)&R.Vnv, (unsigned long long *)&R.Vtc,
-      (unsigned long long *)&R.n_entries, (unsigned long long *)&R.n_slots,
-      (unsigned long long *)&R.chk_pre, (unsigned long long *)&R.aVo,
-      (unsigned long long *)&R.aVss, (unsigned long long *)&R.aVa, (unsigned long long *)&R.aVb,
-      (unsigned long long *)&R.aVnv, (unsigned long long *)&R.aVtc,
-      (unsigned long long *)&R.n_final, (unsigned long long *)&R.chk_final};
-  group_add<17>(v, red, lane, wave, tid, dst);
+  // Va, Vb, Vo, Vss, Vnv, Vtc, Vbed, n_entries, n_slots, n_final, chk_pre, chk_final, aVo .. aVtc
+  int64_t v[18] = {acc.Va, acc.Vb, acc.Vo, acc.Vss, acc.Vnv, acc.Vtc, 0, acc.nent, acc.nslot,
+                   acc.nfinal, (int64_t)acc.chkpre, (int64_t)acc.chkfinal, acc.aVo, acc.aVss,
+                   acc.aVa, acc.aVb, acc.aVnv, acc.aVtc};
+  group_add<18>(v, red, lane, wave, tid, (unsigned long long *)&R.Va);
   const int32_t f = group_or(acc.flags, fred, lane, wave, tid);
   if (tid == 0 && f) atomicOr(&R.flags, f);
 }
@@ -1087,7 +1156,7 @@ __global__ __launch_bounds__(kThreads) void bed_kernel(SweepParams p) {
   // Source bits of the halo rows (one bit per column, filled by ballots), then their OR over
   // each column's half-open window [y - r, y + r): an exact "no source can be near" test.
   __shared__ uint32_t srcw[kBedWords], rowany[kBedWords];
-  __shared__ int64_t red[kWaves][3];
+  __shared__ int64_t red[kWaves][6];
   __shared__ int32_t fred[kWaves];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   int32_t b, band;
@@ -1216,10 +1285,9 @@ __global__ __launch_bounds__(kThreads) void bed_kernel(SweepParams p) {
     if (p.dbg_final) p.dbg_final[P] = nf;
   }
   DevRecord &R = p.rec[b];
-  int64_t v[3] = {nbed, nfinal, (int64_t)chk};
-  unsigned long long *const dst[3] = {(unsigned long long *)&R.Vbed, (unsigned long long *)&R.n_final,
-                                      (unsigned long long *)&R.chk_final};
-  group_add<3>(v, red, lane, wave, tid, dst);
+  // Vbed, n_entries, n_slots, n_final, chk_pre, chk_final
+  int64_t v[6] = {nbed, 0, 0, nfinal, 0, (int64_t)chk};
+  group_add<6>(v, red, lane, wave, tid, (unsigned long long *)&R.Vbed);
   const int32_t f = group_or(flags, fred, lane, wave, tid);
   if (tid == 0 && f) atomicOr(&R.flags, f);
 }
