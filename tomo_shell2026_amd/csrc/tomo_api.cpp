@@ -210,12 +210,10 @@ struct tomo_handle {
   ScratchLayout lay{};
   uint8_t *d_scratch = nullptr;
   OrientHdr *d_hdr = nullptr;
-  float4 *d_pv = nullptr;      // rotated vertices per in-flight orientation
-  size_t pv_cap = 0;           // float4 capacity of d_pv
   int32_t scratch_batch = 0;
   int64_t scratch_region = 0;  // bytes per orientation region of the current allocation
-  // Shared mode (a tomo_batch member): stream, scratch, headers and rotated-vertex buffer
-  // belong to the batch; own_stream is the handle's original stream, kept for destruction.
+  // Shared mode (a tomo_batch member): stream, scratch and headers belong to the batch;
+  // own_stream is the handle's original stream, kept for destruction.
   bool shared = false;
   hipStream_t own_stream = nullptr;
   int32_t batch_req = 0;
@@ -238,24 +236,7 @@ void free_stage_events(tomo_handle *h) {
   h->stage_of.clear();
 }
 
-int ensure_pv(tomo_handle *h, int32_t batch) {
-  const size_t need = (size_t)batch * (size_t)(h->n_vtx > 0 ? h->n_vtx : 1);
-  if (need <= h->pv_cap) return TOMO_OK;
-  if (h->shared) return fail(TOMO_E_STATE, "shared rotated-vertex buffer too small");
-  HIP_TRY(hipStreamSynchronize(h->stream));
-  if (h->d_pv) { HIP_TRY(hipFree(h->d_pv)); h->d_pv = nullptr; }
-  h->pv_cap = 0;
-  if (hipMalloc(&h->d_pv, sizeof(float4) * need) != hipSuccess) {
-    h->d_pv = nullptr;
-    return fail(TOMO_E_OOM, "rotated-vertex buffer allocation failed");
-  }
-  h->pv_cap = need;
-  return TOMO_OK;
-}
-
 int ensure_scratch(tomo_handle *h, int32_t batch) {
-  int rc = ensure_pv(h, batch);
-  if (rc) return rc;
   if (batch <= h->scratch_batch && h->lay.bytes == h->scratch_region) return TOMO_OK;
   if (h->shared) return fail(TOMO_E_STATE, "shared scratch does not fit this launch shape");
   HIP_TRY(hipStreamSynchronize(h->stream));
@@ -368,7 +349,6 @@ SweepParams base_params(tomo_handle *h) {
   p.bed_on = (h->opt.bed_type != 0 && h->opt.bed_r >= 1) ? 1 : 0;
   p.lay = h->lay;
   p.hdr = h->d_hdr;
-  p.pv = h->d_pv;
   p.scratch = h->d_scratch;
   const int32_t g2 = h->opt.grid * h->opt.grid;
   int32_t ys = 0;
@@ -778,12 +758,10 @@ int tomo_destroy(tomo_handle *h) {
   if (h->stream) hipStreamSynchronize(h->stream);
   free_stage_events(h);
   if (h->shared) {  // buffers and stream belong to the batch
-    h->d_pv = nullptr;
     h->d_hdr = nullptr;
     h->d_scratch = nullptr;
     h->stream = h->own_stream;
   }
-  if (h->d_pv) hipFree(h->d_pv);
   if (h->d_hdr) hipFree(h->d_hdr);
   if (h->d_scratch) hipFree(h->d_scratch);
   if (h->d_rec) hipFree(h->d_rec);
@@ -805,8 +783,7 @@ int tomo_destroy(tomo_handle *h) {
 // ---------------------------------------------------------------------------------------------
 // Batched sweeps: many meshes, each with its own orientations and options, in one captured
 // hipGraph launch on one stream.  Members keep their own mesh, rotations and records; they share
-// the batch's stream, slot-table scratch, headers and rotated-vertex buffer, sized for the
-// largest member.  Before each member an acquire pass zeroes the slot counters of the regions it
+// the batch's stream, slot-table scratch and headers, sized for the largest member.  Before each member an acquire pass zeroes the slot counters of the regions it
 // will use, laid out for its grid and inline capacity (the memory held another member's layout).
 
 struct tomo_batch {
@@ -822,8 +799,7 @@ struct tomo_batch {
   std::vector<Tune> tune;
   uint8_t *d_scratch = nullptr;
   OrientHdr *d_hdr = nullptr;
-  float4 *d_pv = nullptr;
-  size_t scratch_bytes = 0, hdr_count = 0, pv_count = 0;
+  size_t scratch_bytes = 0, hdr_count = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool timed = false;
@@ -838,10 +814,8 @@ static void release_private_scratch(tomo_handle *h) {
   hipStreamSynchronize(h->stream);
   if (h->d_scratch) { hipFree(h->d_scratch); h->d_scratch = nullptr; }
   if (h->d_hdr) { hipFree(h->d_hdr); h->d_hdr = nullptr; }
-  if (h->d_pv) { hipFree(h->d_pv); h->d_pv = nullptr; }
   h->scratch_batch = 0;
   h->scratch_region = 0;
-  h->pv_cap = 0;
 }
 
 // Detach members from the shared buffers (before re-tuning or destruction).
@@ -854,15 +828,12 @@ static void batch_unshare(tomo_batch *b) {
       h->stream = h->own_stream;
       h->d_scratch = nullptr;
       h->d_hdr = nullptr;
-      h->d_pv = nullptr;
       h->scratch_batch = 0;
       h->scratch_region = 0;
-      h->pv_cap = 0;
     }
   if (b->d_scratch) { hipFree(b->d_scratch); b->d_scratch = nullptr; }
   if (b->d_hdr) { hipFree(b->d_hdr); b->d_hdr = nullptr; }
-  if (b->d_pv) { hipFree(b->d_pv); b->d_pv = nullptr; }
-  b->scratch_bytes = b->hdr_count = b->pv_count = 0;
+  b->scratch_bytes = b->hdr_count = 0;
 }
 
 int tomo_batch_create(tomo_batch **out) {
@@ -1014,7 +985,7 @@ int tomo_batch_capture(tomo_batch *b) {
   if (!b) return fail(TOMO_E_INVALID_ARG, "null batch");
   if (b->members.empty()) return fail(TOMO_E_STATE, "empty batch");
   batch_unshare(b);
-  size_t sbytes = 0, hcount = 0, pcount = 0;
+  size_t sbytes = 0, hcount = 0;
   for (size_t m = 0; m < b->members.size(); m++) {
     tomo_handle *h = b->members[m];
     release_private_scratch(h);
@@ -1022,20 +993,17 @@ int tomo_batch_capture(tomo_batch *b) {
     const size_t bs = (size_t)b->tune[m].batch;
     sbytes = std::max(sbytes, (size_t)h->lay.bytes * bs);
     hcount = std::max(hcount, bs);
-    pcount = std::max(pcount, bs * (size_t)(h->n_vtx > 0 ? h->n_vtx : 1));
   }
   if (hipMalloc(&b->d_scratch, sbytes) != hipSuccess) {
     b->d_scratch = nullptr;
     return fail(TOMO_E_OOM, "shared scratch allocation failed");
   }
-  if (hipMalloc(&b->d_hdr, sizeof(OrientHdr) * hcount) != hipSuccess ||
-      hipMalloc(&b->d_pv, sizeof(float4) * pcount) != hipSuccess) {
+  if (hipMalloc(&b->d_hdr, sizeof(OrientHdr) * hcount) != hipSuccess) {
     batch_unshare(b);
     return fail(TOMO_E_OOM, "shared buffer allocation failed");
   }
   b->scratch_bytes = sbytes;
   b->hdr_count = hcount;
-  b->pv_count = pcount;
   HIP_TRY(hipMemsetAsync(b->d_scratch, 0, sbytes, b->stream));
   HIP_TRY(hipMemsetAsync(b->d_hdr, 0, sizeof(OrientHdr) * hcount, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
@@ -1045,10 +1013,8 @@ int tomo_batch_capture(tomo_batch *b) {
     h->stream = b->stream;
     h->d_scratch = b->d_scratch;
     h->d_hdr = b->d_hdr;
-    h->d_pv = b->d_pv;
     h->scratch_batch = b->tune[m].batch;
     h->scratch_region = h->lay.bytes;
-    h->pv_cap = pcount;
     h->phases = false;
     free_stage_events(h);
   }

@@ -89,7 +89,6 @@ struct SweepParams {
   const float *bed_dist;   // [2r][2r] float distances sqrtf(dx^2 + dy^2), dx,dy in [-r, r)
   // scratch and outputs
   uint8_t *scratch;
-  float4 *pv;              // [n_batch][n_vtx] rotated vertex x, y, z and normal z
   OrientHdr *hdr;          // [n_batch]
   ScratchLayout lay;
   DevRecord *rec;          // [n_batch]

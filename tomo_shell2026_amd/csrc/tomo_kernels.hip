@@ -504,7 +504,7 @@ __device__ __forceinline__ int32_t group_or(int32_t f, int32_t *red, int lane, i
 }  // namespace
 
 // ---- aabb_kernel: one work-group per orientation ------------------------------------------
-// Rotates every vertex (kept for the rasteriser) and reduces the rotated AABB minimum
+// Rotates every vertex and reduces the rotated AABB minimum
 // (STomoSh_Base.cpp:77-109); resets the orientation's bookkeeping, and the
 // counters of the previous orientation's large slots that pair_kernel left set.
 __global__ __launch_bounds__(kThreads) void aabb_kernel(SweepParams p) {
@@ -521,13 +521,11 @@ __global__ __launch_bounds__(kThreads) void aabb_kernel(SweepParams p) {
     for (uint32_t j = tid; j < nbig_prev; j += kThreads) t.count[t.big[j]] = 0u;
   }
   float mx = 1e5f, my = 1e5f, mz = 1e5f;
-  float4 *pv = p.pv + (int64_t)b * p.n_vtx;
   for (int32_t v = tid; v < p.n_vtx; v += kThreads) {
     const float t0 = p.vx[v] - p.cx, t1 = p.vy[v] - p.cy, t2 = p.vz[v] - p.cz;
     const float x = dot_row(r0, r1, r2, t0, t1, t2);
     const float y = dot_row(r3, r4, r5, t0, t1, t2);
     const float z = dot_row(r6, r7, r8, t0, t1, t2);
-    pv[v] = make_float4(x, y, z, dot_row(r6, r7, r8, p.nx[v], p.ny[v], p.nz[v]));
     mx = fminf(mx, x);
     my = fminf(my, y);
     mz = fminf(mz, z);
