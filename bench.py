@@ -178,7 +178,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=0, help="orientations in flight (0 = auto)")
-    ap.add_argument("--cpu-stride", type=int, default=11)
+    ap.add_argument("--cpu-stride", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=("dsphere", "manikin", "batch70"), default="dsphere",
                     help="dsphere: 69,120-triangle displaced sphere, 256 grid (the headline workload); "
