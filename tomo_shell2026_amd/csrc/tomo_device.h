@@ -16,7 +16,7 @@ constexpr int kInlineDefault = 16;     // entries stored inline per slot (layer-
 constexpr int kInlineMax = 32;         // in [kSmall, kInlineMax]
 constexpr int kBigCap = 512;           // entries a single slot may hold for exact resolution
 constexpr int kMaxBedR = 16;           // int(BedOuterBound) supported by the bed kernel
-constexpr int kTileThreads = 512;      // rasteriser work-group: 8 waves
+constexpr int kTileThreads = 768;      // rasteriser work-group: 12 waves
 constexpr int kTileWaves = kTileThreads / 64;
 constexpr int kTileSlots = 16384;      // rasteriser tile: slots with a 16-bit LDS counter each
 constexpr int kTileYDefault = 128;     // tile extent along y (power of two); x extent = kTileSlots / y

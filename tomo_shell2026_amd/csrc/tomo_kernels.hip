@@ -570,16 +570,16 @@ __global__ __launch_bounds__(kThreads) void aabb_kernel(SweepParams p) {
 // A triangle's zero-hit fallback (its centroid pixel) is decided by its owner tile, the one
 // holding its clamped box corner (x0, y0), which walks the whole box to count hits.
 // Work-groups follow xcd_map so an orientation's tiles share one XCD's L2.
-enum : int { TR_V0 = 0, TR_V1 = 3, TR_V2 = 6, TR_NZ = 9, TR_DEN = 12, TR_D00 = 13, TR_D01 = 14,
-             TR_D11 = 15, TR_C = 16, TR_STRIDE = 19 };
+enum : int { TR_A0 = 0, TR_A1 = 3, TR_A2 = 6, TR_NZ = 9, TR_STRIDE = 13 };
 
 struct RasterLds {
   uint32_t cnt[kTileSlots / 2];  // the tile's slot counters, 16 bits each
-  // margined v0, v1, v2 (x, y, z), n0z, n1z, n2z, den, d00, d01, d11, margined centroid (x, y, z);
-  // odd stride: lanes reading different owners hit different banks
+  // placed (un-margined) vertices a0, a1, a2 (x, y, z) and n0z, n1z, n2z; everything else the
+  // exact test needs is recomputed from them bit for bit.  Odd stride: lanes reading different
+  // owners hit different banks.
   float tri[kTileWaves][64][TR_STRIDE];
   uint32_t qmeta[kTileWaves][128];  // owner lane | (x + 256) << 6 | (y + 256) << 19
-  int32_t hits[kTileWaves][64];
+  uint8_t hits[kTileWaves][64];
   int32_t red[kTileWaves][5];
   uint64_t zred[kTileWaves];
 };
@@ -648,7 +648,9 @@ __global__ __launch_bounds__(kTileThreads) void raster_kernel(SweepParams p) {
       while (it_kept) {
         const int cl = __ffsll((unsigned long long)it_kept) - 1;
         it_kept &= it_kept - 1ull;
-        if ((it_turn++ & (kTileWaves - 1)) == wave) return it_c0 + cl;
+        const bool mine = it_turn == wave;
+        it_turn = it_turn == kTileWaves - 1 ? 0 : it_turn + 1;
+        if (mine) return it_c0 + cl;
       }
       it_c0 += 64;
       if (it_c0 >= p.n_chunks) return -1;
@@ -722,15 +724,13 @@ __global__ __launch_bounds__(kTileThreads) void raster_kernel(SweepParams p) {
         const float ey = fmaxf(fabsf(centre_offset(y0, p.h2, v0y)), fabsf(centre_offset(y1, p.h2, v0y)));
         sc = tri_scan(ts, ex, ey);
         float *rec = tri[lane];
-        rec[TR_V0 + 0] = v0x; rec[TR_V0 + 1] = v0y; rec[TR_V0 + 2] = a[0][2] + kEps;
-        rec[TR_V1 + 0] = v1x; rec[TR_V1 + 1] = v1y; rec[TR_V1 + 2] = a[1][2] + kEps;
-        rec[TR_V2 + 0] = v2x; rec[TR_V2 + 1] = v2y; rec[TR_V2 + 2] = a[2][2] + kEps;
-        rec[TR_NZ + 0] = nzv[0]; rec[TR_NZ + 1] = nzv[1]; rec[TR_NZ + 2] = nzv[2];
-        rec[TR_DEN] = ts.den; rec[TR_D00] = ts.d00; rec[TR_D01] = ts.d01; rec[TR_D11] = ts.d11;
-        // the un-margined centroid pixel a triangle without hits falls back to
-        // (STomoSh_INT3.cpp:1190-1206)
 #pragma unroll
-        for (int q = 0; q < 3; q++) rec[TR_C + q] = (a[0][q] + a[1][q] + a[2][q]) * 0.333333f + kEps;
+        for (int q = 0; q < 3; q++) {
+          rec[TR_A0 + q] = a[0][q];
+          rec[TR_A1 + q] = a[1][q];
+          rec[TR_A2 + q] = a[2][q];
+          rec[TR_NZ + q] = nzv[q];
+        }
       }
       L.hits[wave][lane] = 0;
       int32_t x = x0, y = y0;
@@ -772,21 +772,25 @@ __global__ __launch_bounds__(kTileThreads) void raster_kernel(SweepParams p) {
             const int32_t px_i = (int32_t)((meta >> 6) & 8191u) - 256;
             const int32_t py_i = (int32_t)(meta >> 19) - 256;
             const float *rec = tri[owner_lane];
-            // the exact test's numerators, formed as _getBaryCoord forms them
-            TriSetup ts;
-            ts.v0x = rec[TR_V0 + 0]; ts.v0y = rec[TR_V0 + 1];
-            ts.e0x = rec[TR_V1 + 0] - ts.v0x; ts.e0y = rec[TR_V1 + 1] - ts.v0y;
-            ts.e1x = rec[TR_V2 + 0] - ts.v0x; ts.e1y = rec[TR_V2 + 1] - ts.v0y;
-            ts.d00 = rec[TR_D00]; ts.d01 = rec[TR_D01]; ts.d11 = rec[TR_D11]; ts.den = rec[TR_DEN];
+            // margined vertices and the exact test's numerators, formed as triVoxel and
+            // _getBaryCoord form them (the same operations as the setup above)
+            float m[3][3];
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+              m[0][q] = rec[TR_A0 + q] + kEps;
+              m[1][q] = rec[TR_A1 + q] + kEps;
+              m[2][q] = rec[TR_A2 + q] + kEps;
+            }
+            const TriSetup ts = tri_setup(m[0][0], m[0][1], m[1][0], m[1][1], m[2][0], m[2][1]);
             float nv, nw, u, v, w;
             bary_num(ts, ((float)px_i + kEps) + p.h2, ((float)py_i + kEps) + p.h2, nv, nw);
             if (bary_exact(ts.den, nv, nw, u, v, w)) {
               L.hits[wave][owner_lane] = 1;
               if (px_i >= exlo && px_i <= exhi && py_i >= eylo && py_i <= eyhi) {
                 // _bary_product (Tomo_types.cpp:75-84) of the margined vertices and the normals
-                const float px = rec[TR_V0 + 0] * u + rec[TR_V1 + 0] * v + rec[TR_V2 + 0] * w;
-                const float py = rec[TR_V0 + 1] * u + rec[TR_V1 + 1] * v + rec[TR_V2 + 1] * w;
-                const float pz = rec[TR_V0 + 2] * u + rec[TR_V1 + 2] * v + rec[TR_V2 + 2] * w;
+                const float px = m[0][0] * u + m[1][0] * v + m[2][0] * w;
+                const float py = m[0][1] * u + m[1][1] * v + m[2][1] * w;
+                const float pz = m[0][2] * u + m[1][2] * v + m[2][2] * w;
                 const float nz = rec[TR_NZ + 0] * u + rec[TR_NZ + 1] * v + rec[TR_NZ + 2] * w;
                 int32_t s;
                 if (p.do_shell) {  // with its shell sub-pixel (STomoSh_INT3.cpp:1179-1185)
@@ -816,7 +820,10 @@ __global__ __launch_bounds__(kTileThreads) void raster_kernel(SweepParams p) {
       if (active && owner && L.hits[wave][lane] == 0) {
         // very small triangle: one pixel at the un-margined centroid (STomoSh_INT3.cpp:1190-1206)
         const float *rec = tri[lane];
-        const float tx = rec[TR_C + 0], ty = rec[TR_C + 1], tz = rec[TR_C + 2], n0 = rec[TR_NZ + 0];
+        float c3[3];
+#pragma unroll
+        for (int q = 0; q < 3; q++) c3[q] = (rec[TR_A0 + q] + rec[TR_A1 + q] + rec[TR_A2 + q]) * 0.333333f + kEps;
+        const float tx = c3[0], ty = c3[1], tz = c3[2], n0 = rec[TR_NZ + 0];
         if (p.do_shell) {
           const float sz = tz + ((n0 >= 0.0f) ? -p.shell_off : p.shell_off);
           em.put2(tx, ty, tz, n0, sz, -n0, my_rank);
