@@ -121,9 +121,9 @@ int tomo_set_launch(tomo_handle *h, int32_t batch);
 
 /* Launch-shape knobs for tuning (values take effect at the next tomo_run):
  * TOMO_TUNE_BATCH           orientations in flight (0 = automatic), same as tomo_set_launch;
- * TOMO_TUNE_TILE_Y          rasteriser tile extent along y in slots (power of two 32..512,
- *                           default 128); a tile holds 16384 slots, so its x extent is
- *                           16384 / value;
+ * TOMO_TUNE_TILE_Y          rasteriser tile extent along y in slots: a power of two 32..512,
+ *                           or 0 (default: 128 up to a 256 grid, 64 above); a tile holds
+ *                           16384 slots, so its x extent is 16384 / value;
  * TOMO_TUNE_INLINE          slot entries stored inline per slot (8..32, default 16); the rest go
  *                           to an overflow pool (more inline = fewer pool passes, more memory);
  * TOMO_TUNE_EXACT_FSUM      1 (default): where a float total can round (TOMO_F_FP32_INEXACT),

@@ -217,7 +217,7 @@ struct tomo_handle {
   bool shared = false;
   hipStream_t own_stream = nullptr;
   int32_t batch_req = 0;
-  int32_t tile_y = kTileYDefault;
+  int32_t tile_y = 0;  // 0: kTileYDefault up to a 256 grid, half that above
   int32_t exact_fsum = 1;
   int32_t last_batch = 0;
   int32_t last_begin = 0, last_end = 0;
@@ -351,8 +351,9 @@ SweepParams base_params(tomo_handle *h) {
   p.hdr = h->d_hdr;
   p.scratch = h->d_scratch;
   const int32_t g2 = h->opt.grid * h->opt.grid;
+  const int32_t tile_y = h->tile_y > 0 ? h->tile_y : (h->opt.grid > 256 ? kTileYDefault / 2 : kTileYDefault);
   int32_t ys = 0;
-  while ((1 << (ys + 1)) <= h->tile_y) ys++;
+  while ((1 << (ys + 1)) <= tile_y) ys++;
   p.tile_y_shift = ys;
   p.tile_x = kTileSlots >> ys;
   p.tiles_x = (h->opt.grid + p.tile_x - 1) / p.tile_x;
@@ -595,8 +596,8 @@ int tomo_set_tuning(tomo_handle *h, int32_t knob, int32_t value) {
       h->batch_req = value;
       return TOMO_OK;
     case TOMO_TUNE_TILE_Y:
-      if (value < 32 || value > 512 || (value & (value - 1)))
-        return fail(TOMO_E_INVALID_ARG, "tile extent along y must be a power of two in 32..512");
+      if (value != 0 && (value < 32 || value > 512 || (value & (value - 1))))
+        return fail(TOMO_E_INVALID_ARG, "tile extent along y must be 0 or a power of two in 32..512");
       h->tile_y = value;
       return TOMO_OK;
     case TOMO_TUNE_INLINE:
@@ -792,7 +793,7 @@ struct tomo_batch {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::vector<tomo_handle *> members;
   struct Tune {
-    int32_t batch = 0, tile_y = kTileYDefault, inl = kInlineDefault;
+    int32_t batch = 0, tile_y = 0, inl = kInlineDefault;
     float us_per_orientation = -1.0f;
     int32_t tuned = 0;
   };
@@ -900,7 +901,7 @@ static float time_shape(tomo_handle *h, const tomo_batch::Tune &t, int32_t b0, i
 
 // Autotuner.  Members are grouped by (grid, ceil(log2(nTri))); each group's largest member is
 // timed on `sample` orientations from the middle of its list with a coordinate descent over
-// rasteriser tile extent along y {64, 128, 256}, inline slot capacity {8, 16, 24} and
+// rasteriser tile extent along y {32, 64, 128, 256}, inline slot capacity {8, 16, 24} and
 // orientations in flight {1024, 2048, 4096} (within a quarter of free memory); shapes that leave
 // a slot unresolved are rejected.  The group's winner applies to all its members.
 int tomo_batch_tune(tomo_batch *b, int32_t sample) {
@@ -935,7 +936,9 @@ int tomo_batch_tune(tomo_batch *b, int32_t sample) {
         best = t;
       }
     };
-    for (int32_t ty : {64, 256}) {
+    const int32_t ty0 = h->opt.grid > 256 ? kTileYDefault / 2 : kTileYDefault;  // the default
+    for (int32_t ty : {32, 64, 128, 256}) {
+      if (ty == ty0) continue;
       tomo_batch::Tune t = best;
       t.tile_y = ty;
       consider(t);
