@@ -360,6 +360,7 @@ SweepParams base_params(tomo_handle *h) {
   p.tiles_y = (h->opt.grid + (1 << ys) - 1) >> ys;
   p.tiles = p.tiles_x * p.tiles_y;
   p.pair_tiles = (g2 + kPairTile - 1) / kPairTile;
+  p.big_groups = p.pair_tiles >= 8 ? p.pair_tiles / 8 : 1;
   p.bed_bands = (h->opt.grid + kBandRows - 1) / kBandRows;
   p.exact_fsum = h->exact_fsum;
   return p;

@@ -78,7 +78,8 @@ struct SweepParams {
   int32_t tiles_x, tiles_y;
   int32_t tile_x;          // tile extent along x (slots)
   int32_t tile_y_shift;    // log2 of the tile extent along y
-  int32_t pair_tiles;      // pairing work-groups per orientation
+  int32_t pair_tiles;      // pairing work-groups per orientation (touched box)
+  int32_t big_groups;      // pairing work-groups per orientation (large-slot list)
   int32_t bed_bands;       // bed work-groups per orientation
   // options
   int32_t grid;

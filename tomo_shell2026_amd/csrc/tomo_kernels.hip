@@ -248,24 +248,38 @@ __device__ __forceinline__ int pow2_at_least(int n) {
   return P;
 }
 
+// Sum over the W-lane segment holding this lane (W = 32 or 64).
+template <int W>
+__device__ __forceinline__ int32_t seg_sum(int32_t v) {
+#pragma unroll
+  for (int o = W / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
 // Pair the distinct keys of the lanes set in V (at most 63), in descending key order from low
 // to high lane.  Same semantics as pair_keys in tomo_slot.h, with the sequential state machines
-// turned into "last event before this lane" ballot scans.  The result is valid in every lane.
+// turned into "last event before this lane" ballot scans.  With W = 32 the wave pairs two
+// slots, one per 32-lane half; the result is valid in every lane of its segment.
+template <int W = 64>
 __device__ void wave_pair(uint32_t key, uint64_t V, int lane, const PairParams &pp, SlotResult &r,
                           uint64_t *rows) {
+  const int base = lane & ~(W - 1);
+  const uint64_t seg = (W == 64) ? ~0ull : (0xffffffffull << base);
+  V &= seg;
   const bool valid = ((V >> lane) & 1ull) != 0;
   const int D = __popcll(V);
   const int32_t z = key_z(key), nz = key_nz(key);
-  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  const uint64_t lt = ((lane == 0) ? 0ull : ((~0ull) >> (64 - lane))) & seg;
   const uint32_t k10 = make_key(1, 0);
   r.npre = D;
-  r.sabs = wave_sum(valid ? (z < 0 ? -z : z) : 0);
-  r.has10 = __ballot(valid && key == k10) != 0ull;
+  r.sabs = seg_sum<W>(valid ? (z < 0 ? -z : z) : 0);
+  r.has10 = (__ballot(valid && key == k10) & seg) != 0ull;
   r.flags = 0;
   r.vo = r.vss = r.va = r.vb = r.vnv = r.vtc = 0;
-  if (D <= 1) {
+  if (__ballot(D > 1) == 0ull) {  // every segment has at most one row
+    const int32_t zl = __shfl(z, V ? msb64(V) : base);
     r.n = D;
-    r.last_z = D ? __shfl(z, msb64(V)) : 0;
+    r.last_z = D ? zl : 0;
     r.last_type = 0;
     if (rows && valid) rows[0] = make_row(0, key);
     return;
@@ -276,7 +290,7 @@ __device__ void wave_pair(uint32_t key, uint64_t V, int lane, const PairParams &
   const bool prevA = pv ? (((mA >> msb64(pv)) & 1ull) != 0) : false;
   const bool kA = isA && !prevA, kB = isB && prevA;
   const bool bs = kB && nz <= pp.cut_ssb, bn = kB && nz >= pp.cut_nvb;
-  const uint64_t mKA = __ballot(kA), mBS = __ballot(bs), mBN = __ballot(bn);
+  const uint64_t mKA = __ballot(kA) & seg, mBS = __ballot(bs) & seg, mBN = __ballot(bn) & seg;
   const uint64_t evS = (mBS | mKA), evN = (mBN | mKA);
   const bool prevS = (evS & lt) ? (((mBS >> msb64(evS & lt)) & 1ull) != 0) : false;
   const bool prevN = (evN & lt) ? (((mBN >> msb64(evN & lt)) & 1ull) != 0) : false;
@@ -287,23 +301,30 @@ __device__ void wave_pair(uint32_t key, uint64_t V, int lane, const PairParams &
   const bool tc = !pp.explicit_ss && kA && mKA && lane == (int)__ffsll((long long)mKA) - 1;
   int32_t ty = (kA ? T_AL : 0) | (kB ? T_BE : 0) | (ssb ? T_SSB : 0) | (ssa ? T_SSA : 0) |
                (nvb ? T_NVB : 0) | (nva ? T_NVA : 0) | (tc ? T_TC : 0);
-  const int32_t al = wave_sum(kA ? z : 0), be = wave_sum(kB ? z : 0);
-  const int32_t sSSB = wave_sum(ssb ? z : 0), sSSA = wave_sum(ssa ? z : 0);
-  const int32_t sNVB = wave_sum(nvb ? z : 0), sNVA = wave_sum(nva ? z : 0);
-  const int32_t sTC = wave_sum(tc ? z : 0);
-  const int lastl = msb64(V);
+  const int32_t al = seg_sum<W>(kA ? z : 0), be = seg_sum<W>(kB ? z : 0);
+  const int32_t sSSB = seg_sum<W>(ssb ? z : 0), sSSA = seg_sum<W>(ssa ? z : 0);
+  const int32_t sNVB = seg_sum<W>(nvb ? z : 0), sNVA = seg_sum<W>(nva ? z : 0);
+  const int32_t sTC = seg_sum<W>(tc ? z : 0);
+  const int lastl = V ? msb64(V) : base;
   int32_t last_z = __shfl(z, lastl), last_type = __shfl(ty, lastl);
+  if (D <= 1) {  // this segment's slot has at most one row (Pairing skips it)
+    if (rows && valid) rows[0] = make_row(0, key);
+    r.n = D;
+    r.last_z = D ? last_z : 0;
+    r.last_type = 0;
+    return;
+  }
   if (rows && valid) rows[__popcll(V & lt)] = make_row(ty, key);
   int n = D;
   if (exOpen) {
     if (n < kMaxPix) {
-      if (rows && lane == 0) rows[n] = make_row(T_SSA, make_key(0, 0));
+      if (rows && lane == base) rows[n] = make_row(T_SSA, make_key(0, 0));
       n++; last_z = 0; last_type = T_SSA;
     } else r.flags |= F_SENTINEL_DROP;
   }
   if (imOpen) {
     if (n < kMaxPix) {
-      if (rows && lane == 0) rows[n] = make_row(T_NVA, make_key(0, 0));
+      if (rows && lane == base) rows[n] = make_row(T_NVA, make_key(0, 0));
       n++; last_z = 0; last_type = T_NVA;
     } else r.flags |= F_SENTINEL_DROP;
   }
@@ -407,27 +428,32 @@ __device__ void big_slot(int32_t s, uint32_t c, const Tables &t, int32_t G2, uin
   r.flags |= fl;
 }
 
-// A slot with kSmall < c < 64 entries: one key per lane, register bitonic sort over the wave
-// with lane shuffles, duplicates masked out.  No ranks are needed: at most 63 distinct keys.
+// A slot with kSmall < c < W entries in a W-lane segment (W = 64: the whole wave; W = 32: two
+// slots per wave): one key per lane, register bitonic sort over the segment with lane
+// shuffles, duplicates masked out.  No ranks are needed: at most 63 distinct keys.
+template <int W = 64>
 __device__ void mid_slot(int32_t s, uint32_t c, const Tables &t, int32_t G2, int lane,
                          const PairParams &pp, SlotResult &r, uint64_t *rows) {
+  const int i = lane & (W - 1);
   uint32_t key = 0u;
-  if (lane < (int)c) {
+  if (i < (int)c) {
     const uint32_t off = (c > (uint32_t)t.inl) ? t.off[s] : 0u;
-    key = (lane < t.inl) ? t.ent[(int64_t)lane * G2 + s].x : t.cmp_key[off + (uint32_t)(lane - t.inl)];
+    key = (i < t.inl) ? t.ent[(int64_t)i * G2 + s].x : t.cmp_key[off + (uint32_t)(i - t.inl)];
   }
 #pragma unroll
-  for (int size = 2; size <= 64; size <<= 1) {
+  for (int size = 2; size <= W; size <<= 1) {
 #pragma unroll
     for (int stride = size >> 1; stride > 0; stride >>= 1) {
       const uint32_t other = __shfl_xor(key, stride);
-      const bool keep_max = ((lane & size) == 0) == ((lane & stride) == 0);
+      // runs of length `size` alternate direction; each segment's final merge is descending
+      const bool desc = size == W || (i & size) == 0;
+      const bool keep_max = desc == ((i & stride) == 0);
       key = keep_max ? max(key, other) : min(key, other);
     }
   }
   const uint32_t prev = __shfl_up(key, 1);
-  const bool first = lane < (int)c && (lane == 0 || key != prev);
-  wave_pair(key, __ballot(first), lane, pp, r, rows);
+  const bool first = i < (int)c && (i == 0 || key != prev);
+  wave_pair<W>(key, __ballot(first), lane, pp, r, rows);
 }
 
 // Work-groups are dispatched to the 8 XCDs round-robin by block index.  Map block i to
@@ -965,14 +991,19 @@ __global__ __launch_bounds__(kThreads) void compact_kernel(SweepParams p) {
 
 // ---- pair_kernel: kPairTile slots of the touched box per work-group ------------------------
 // Pairing (STomoSh_INT3.cpp:366-395) and the per-slot sums of Calculate (:1211-1255).
-__global__ __launch_bounds__(kThreads) void pair_kernel(SweepParams p) {
+// pair_kernel (PART 0) pairs the slots of at most kSmall entries found in the work-group's part
+// of the touched box; pair_big_kernel (PART 1, a second launch with its own register budget)
+// pairs the large-slot list.
+template <int PART>
+__device__ __forceinline__ void pair_body(const SweepParams &p) {
   __shared__ uint64_t sortbuf[kWaves][kBigCap];
   __shared__ uint32_t pairq[3][kWaves][128];  // queued slots: s | entries << 24
   __shared__ int64_t red[kWaves][18];
   __shared__ int32_t fred[kWaves];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   int32_t b, tile;
-  if (!xcd_map(p.pair_tiles, p.n_batch, b, tile)) return;
+  const int32_t groups = PART == 0 ? p.pair_tiles : p.big_groups;  // work-groups per orientation
+  if (!xcd_map(groups, p.n_batch, b, tile)) return;
   const int32_t G = p.grid, G2 = G * G;
   const OrientHdr &h = p.hdr[b];
   const int32_t bx0 = h.bbox[0], bx1 = h.bbox[1], by0 = h.bbox[2], by1 = h.bbox[3];
@@ -981,13 +1012,13 @@ __global__ __launch_bounds__(kThreads) void pair_kernel(SweepParams p) {
   const int32_t wy = by1 - by0 + 1;
   const int32_t nbb = any ? (bx1 - bx0 + 1) * wy : 0;
   const int32_t i0 = tile * kPairTile;
-  const int32_t nwaves = p.pair_tiles * kWaves;
-  if (tile == 0 && tid == 0) {
+  const int32_t nwaves = groups * kWaves;
+  if (PART == 0 && tile == 0 && tid == 0) {
     // rasteriser flags, and pool or spill list overflow
     const int32_t f0 = h.flags | (npool > (uint32_t)p.lay.pool || h.spill_top > (uint32_t)p.lay.spill ? F_UNRESOLVED : 0);
     if (f0) atomicOr(&p.rec[b].flags, f0);
   }
-  if (i0 >= nbb && (uint32_t)(tile * kWaves) >= nbig) return;  // nothing for this group
+  if (PART == 0 ? i0 >= nbb : (uint32_t)(tile * kWaves) >= nbig) return;  // nothing for this group
   const Tables t = tables_for(p, b);
   const bool final_here = !p.bed_on;  // without a bed the final row count is known here
   int32_t *const sv = (p.exact_fsum && totals_may_round(h, p.pp)) ? t.sv : nullptr;
@@ -1015,6 +1046,7 @@ __global__ __launch_bounds__(kThreads) void pair_kernel(SweepParams p) {
   // The touched box is walked 64 slots per wave step; slots of at most kSmall entries are
   // queued by size class (<= 2, <= 4, <= 8) and paired 64 at a time with that class's register
   // network, so no wave runs a larger network than its slots need.
+  if constexpr (PART == 0) {
   uint32_t n2 = 0u, n4 = 0u, n8 = 0u;  // queue fills (wave-uniform)
   const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
   uint32_t *const q2 = pairq[0][wave], *const q4 = pairq[1][wave], *const q8 = pairq[2][wave];
@@ -1098,11 +1130,12 @@ __global__ __launch_bounds__(kThreads) void pair_kernel(SweepParams p) {
       }
     }
   }
+  } else {
   // Large slots keep their counters set until the next aabb_kernel on this scratch region
   // (resetting them here would race with the small-slot scan of other groups).
   // Those with at most 16 entries, all inline: one lane each, 16-key register network.
   const uint32_t lane_cap = t.inl < 16 ? (uint32_t)t.inl : 16u;
-  for (uint32_t j = (uint32_t)(tile * kThreads + tid); j < nbig; j += (uint32_t)(p.pair_tiles * kThreads)) {
+  for (uint32_t j = (uint32_t)(tile * kThreads + tid); j < nbig; j += (uint32_t)(groups * kThreads)) {
     const int32_t s = t.big[j];
     const uint32_t c = t.count[s];
     if (c > lane_cap) continue;
@@ -1121,27 +1154,63 @@ __global__ __launch_bounds__(kThreads) void pair_kernel(SweepParams p) {
       if (p.dbg_final) p.dbg_final[s] = r.n;
     }
   }
-  // Larger ones: one wave each.
-  for (uint32_t j = (uint32_t)(tile * kWaves + wave); j < nbig; j += (uint32_t)nwaves) {
-    const int32_t s = t.big[j];
-    const uint32_t c = t.count[s];
-    if (c <= lane_cap) continue;  // paired by a lane above (uniform across the wave)
-    SlotResult r;
-    if (c < 64u && (c <= (uint32_t)t.inl || npool <= (uint32_t)t.pool))
-      mid_slot(s, c, t, G2, lane, p.pp, r, p.dbg_rows ? p.dbg_rows + (int64_t)s * 64 : nullptr);
-    else
-      big_slot(s, c, t, G2, sortbuf[wave], lane, p.pp, r, p.dbg_rows ? p.dbg_rows + (int64_t)s * 64 : nullptr);
-    if (lane == 0) {
-      acc.nslot++;
-      account(acc, r, s, p);
-      t.bedinfo[s] = bed_info(r, p.pp);
-      if (sv) put_sv(sv, G2, s, r);
-      if (final_here) {
-        acc.nfinal += r.n;
-        acc.chkfinal += (uint64_t)(s + 1) * (uint64_t)r.n;
-        if (p.dbg_final) p.dbg_final[s] = r.n;
-      }
+  // Larger ones.  Each wave reads its share of the big list 64 entries at a time; slots of at
+  // most 32 entries are paired two per wave (one per 32-lane half), the rest one per wave.
+  auto finish_slot = [&](int32_t s, const SlotResult &r) {
+    acc.nslot++;
+    account(acc, r, s, p);
+    t.bedinfo[s] = bed_info(r, p.pp);
+    if (sv) put_sv(sv, G2, s, r);
+    if (final_here) {
+      acc.nfinal += r.n;
+      acc.chkfinal += (uint64_t)(s + 1) * (uint64_t)r.n;
+      if (p.dbg_final) p.dbg_final[s] = r.n;
     }
+  };
+  const bool pool_ok = npool <= (uint32_t)t.pool;
+  for (uint32_t j0 = (uint32_t)(tile * kWaves + wave); j0 < nbig; j0 += (uint32_t)nwaves * 64u) {
+    const uint32_t j = j0 + (uint32_t)nwaves * (uint32_t)lane;
+    int32_t ls = 0;
+    uint32_t lc = 0u;
+    if (j < nbig) {
+      ls = t.big[j];
+      lc = t.count[ls];
+    }
+    const bool fits = lc <= (uint32_t)t.inl || pool_ok;
+    const bool half = lc > lane_cap && lc <= 32u && fits;
+    uint64_t m32 = __ballot(half);
+    uint64_t m64 = __ballot(lc > lane_cap && !half);
+    while (m32) {
+      const int la = __ffsll((unsigned long long)m32) - 1;
+      m32 &= m32 - 1ull;
+      int lb = -1;
+      if (m32) {
+        lb = __ffsll((unsigned long long)m32) - 1;
+        m32 &= m32 - 1ull;
+      }
+      const bool upper = lane >= 32;
+      const int src = (upper && lb >= 0) ? lb : la;
+      const int32_t s = __shfl(ls, src);
+      const uint32_t cs = __shfl(lc, src);
+      const uint32_t c = (upper && lb < 0) ? 0u : cs;  // an idle upper half pairs nothing
+      SlotResult r;
+      mid_slot<32>(s, c, t, G2, lane, p.pp, r, (p.dbg_rows && c) ? p.dbg_rows + (int64_t)s * 64 : nullptr);
+      if ((lane & 31) == 0 && c) finish_slot(s, r);
+    }
+    while (m64) {
+      const int la = __ffsll((unsigned long long)m64) - 1;
+      m64 &= m64 - 1ull;
+      const int32_t s = __shfl(ls, la);
+      const uint32_t c = __shfl(lc, la);
+      uint64_t *const rows = p.dbg_rows ? p.dbg_rows + (int64_t)s * 64 : nullptr;
+      SlotResult r;
+      if (c < 64u && (c <= (uint32_t)t.inl || pool_ok))
+        mid_slot<64>(s, c, t, G2, lane, p.pp, r, rows);
+      else
+        big_slot(s, c, t, G2, sortbuf[wave], lane, p.pp, r, rows);
+      if (lane == 0) finish_slot(s, r);
+    }
+  }
   }
   DevRecord &R = p.rec[b];
   // Va, Vb, Vo, Vss, Vnv, Vtc, Vbed, n_entries, n_slots, n_final, chk_pre, chk_final, aVo .. aVtc
@@ -1152,6 +1221,9 @@ __global__ __launch_bounds__(kThreads) void pair_kernel(SweepParams p) {
   const int32_t f = group_or(acc.flags, fred, lane, wave, tid);
   if (tid == 0 && f) atomicOr(&R.flags, f);
 }
+
+__global__ __launch_bounds__(kThreads, 5) void pair_kernel(SweepParams p) { pair_body<0>(p); }
+__global__ __launch_bounds__(kThreads) void pair_big_kernel(SweepParams p) { pair_body<1>(p); }
 
 // ---- bed_kernel: kBandRows grid rows of the bed scan region per work-group -----------------
 // GenerateBed / IsBedCandidate (STomoSh_INT3.cpp:1331-1443) with the raster-order effect: a raft
@@ -1380,7 +1452,10 @@ hipError_t launch_stage(int stage, const SweepParams &p, hipStream_t stream) {
     case 0: hipLaunchKernelGGL(aabb_kernel, dim3(nb), dim3(kThreads), 0, stream, p); break;
     case 1: hipLaunchKernelGGL(raster_kernel, dim3(nb8 * p.tiles), dim3(kTileThreads), 0, stream, p); break;
     case 2: hipLaunchKernelGGL(compact_kernel, dim3(nb), dim3(kThreads), 0, stream, p); break;
-    case 3: hipLaunchKernelGGL(pair_kernel, dim3(nb8 * p.pair_tiles), dim3(kThreads), 0, stream, p); break;
+    case 3:
+      hipLaunchKernelGGL(pair_kernel, dim3(nb8 * p.pair_tiles), dim3(kThreads), 0, stream, p);
+      hipLaunchKernelGGL(pair_big_kernel, dim3(nb8 * p.big_groups), dim3(kThreads), 0, stream, p);
+      break;
     case 4:
       if (p.bed_on) hipLaunchKernelGGL(bed_kernel, dim3(nb8 * p.bed_bands), dim3(kThreads), 0, stream, p);
       break;
