@@ -26,7 +26,7 @@ namespace {
 
 constexpr float kNormFactor = 1000.0f;      // g_fNORMALFACTOR (Tomo_types.h:39)
 
-constexpr int kBedLdsBytes = 28 * 1024;  // bed band + halo flags ((16 + 2r) x (width + 2r))
+constexpr int kBedLdsSlots = 14 * 1024;  // bed band + halo slot info ((16 + 2r) x (width + 2r))
 constexpr int kBedWords = 1024;          // halo source bit words ((16 + 2r) x ceil((width + 2r) / 32))
 
 // Work-group barrier that also orders the work-group's global-memory slot tables.  A slot
@@ -1229,7 +1229,7 @@ __global__ __launch_bounds__(kThreads) void pair_big_kernel(SweepParams p) { pai
 // GenerateBed / IsBedCandidate (STomoSh_INT3.cpp:1331-1443) with the raster-order effect: a raft
 // source slot that takes its own bed row stops being a source for slots scanned after it.
 __global__ __launch_bounds__(kThreads) void bed_kernel(SweepParams p) {
-  __shared__ uint8_t nb[kBedLdsBytes];
+  __shared__ uint16_t nb[kBedLdsSlots];  // bed bookkeeping (bed_info bits) of the band and halo
   // Source bits of the halo rows (one bit per column, filled by ballots), then their OR over
   // each column's half-open window [y - r, y + r): an exact "no source can be near" test.
   __shared__ uint32_t srcw[kBedWords], rowany[kBedWords];
@@ -1256,7 +1256,7 @@ __global__ __launch_bounds__(kThreads) void bed_kernel(SweepParams p) {
   const int32_t hw = hy1 - hy0;
   const int32_t nr = hx1 - hx0;
   const int32_t nw = (hw + 31) >> 5;
-  const bool lds = (int64_t)nr * hw <= kBedLdsBytes;
+  const bool lds = (int64_t)nr * hw <= kBedLdsSlots;
   const bool bits = lds && nr * nw <= kBedWords;
   if (lds) {
     for (int32_t r0 = wave; r0 < nr; r0 += kWaves) {
@@ -1264,13 +1264,10 @@ __global__ __launch_bounds__(kThreads) void bed_kernel(SweepParams p) {
       const bool xin = x >= bx0 && x <= bx1;
       for (int32_t c0 = 0; c0 < hw; c0 += 64) {
         const int32_t yc = c0 + lane, y = hy0 + yc;
-        uint8_t f = 0;
-        if (yc < hw && xin && y >= by0 && y <= by1) {
-          const uint32_t bi = t.bedinfo[x * G + y];
-          f = (uint8_t)(((bi & BI_SRC) ? 1u : 0u) | ((bi & BI_CONV) ? 2u : 0u));
-        }
-        if (yc < hw) nb[r0 * hw + yc] = f;
-        const uint64_t sm = __ballot((f & 1u) != 0u);
+        uint32_t bi = BI_POSS;  // outside the touched box: an empty slot
+        if (yc < hw && xin && y >= by0 && y <= by1) bi = t.bedinfo[x * G + y];
+        if (yc < hw) nb[r0 * hw + yc] = (uint16_t)bi;
+        const uint64_t sm = __ballot((bi & BI_SRC) != 0u);
         if (bits && lane == 0) {
           const int32_t w0 = c0 >> 5;
           srcw[r0 * nw + w0] = (uint32_t)sm;
@@ -1303,7 +1300,7 @@ __global__ __launch_bounds__(kThreads) void bed_kernel(SweepParams p) {
     const int32_t X = X0 + xr, Y = ry0 + c0 + lane;
     const int32_t P = X * G + Y;
     const bool inb = X >= bx0 && X <= bx1 && Y >= by0 && Y <= by1;
-    const uint32_t bi = inb ? t.bedinfo[P] : BI_POSS;
+    const uint32_t bi = lds ? (uint32_t)nb[(X - hx0) * hw + (Y - hy0)] : (inb ? t.bedinfo[P] : BI_POSS);
     const int32_t nrow = (int32_t)(bi & 127u);
     int32_t add = 0;
     bool near = true;
@@ -1320,16 +1317,12 @@ __global__ __launch_bounds__(kThreads) void bed_kernel(SweepParams p) {
       const int32_t ya = max(Y - br, 0), yb = min(Y + br, G);
       for (int32_t sx = xa; sx < xb; sx++) {
         for (int32_t sy = ya; sy < yb; sy++) {
-          uint32_t f;
-          if (lds) {
-            f = nb[(sx - hx0) * hw + (sy - hy0)];
-          } else {
-            f = 0u;
-            if (sx >= bx0 && sx <= bx1 && sy >= by0 && sy <= by1) {
-              const uint32_t bs = t.bedinfo[sx * G + sy];
-              f = ((bs & BI_SRC) ? 1u : 0u) | ((bs & BI_CONV) ? 2u : 0u);
-            }
-          }
+          uint32_t bs = 0u;
+          if (lds)
+            bs = nb[(sx - hx0) * hw + (sy - hy0)];
+          else if (sx >= bx0 && sx <= bx1 && sy >= by0 && sy <= by1)
+            bs = t.bedinfo[sx * G + sy];
+          const uint32_t f = ((bs & BI_SRC) ? 1u : 0u) | ((bs & BI_CONV) ? 2u : 0u);
           if (!(f & 1u)) continue;
           const bool before = (sx < X) || (sx == X && sy < Y);
           if ((f & 2u) && before) continue;  // already covered by its own bed row
