@@ -1259,19 +1259,31 @@ __global__ __launch_bounds__(kThreads) void bed_kernel(SweepParams p) {
   const bool lds = (int64_t)nr * hw <= kBedLdsSlots;
   const bool bits = lds && nr * nw <= kBedWords;
   if (lds) {
+    // One halo row per wave step, its loads (up to 8 column chunks of 64) issued together.
+    constexpr int kChunks = 8;
     for (int32_t r0 = wave; r0 < nr; r0 += kWaves) {
       const int32_t x = hx0 + r0;
       const bool xin = x >= bx0 && x <= bx1;
-      for (int32_t c0 = 0; c0 < hw; c0 += 64) {
-        const int32_t yc = c0 + lane, y = hy0 + yc;
-        uint32_t bi = BI_POSS;  // outside the touched box: an empty slot
-        if (yc < hw && xin && y >= by0 && y <= by1) bi = t.bedinfo[x * G + y];
-        if (yc < hw) nb[r0 * hw + yc] = (uint16_t)bi;
-        const uint64_t sm = __ballot((bi & BI_SRC) != 0u);
-        if (bits && lane == 0) {
-          const int32_t w0 = c0 >> 5;
-          srcw[r0 * nw + w0] = (uint32_t)sm;
-          if (w0 + 1 < nw) srcw[r0 * nw + w0 + 1] = (uint32_t)(sm >> 32);
+      for (int32_t c00 = 0; c00 < hw; c00 += 64 * kChunks) {
+        uint32_t v[kChunks];
+#pragma unroll
+        for (int q = 0; q < kChunks; q++) {
+          const int32_t yc = c00 + 64 * q + lane, y = hy0 + yc;
+          v[q] = BI_POSS;  // outside the touched box: an empty slot
+          if (yc < hw && xin && y >= by0 && y <= by1) v[q] = t.bedinfo[x * G + y];
+        }
+#pragma unroll
+        for (int q = 0; q < kChunks; q++) {
+          const int32_t c0 = c00 + 64 * q;
+          if (c0 >= hw) break;
+          const int32_t yc = c0 + lane;
+          if (yc < hw) nb[r0 * hw + yc] = (uint16_t)v[q];
+          const uint64_t sm = __ballot(yc < hw && (v[q] & BI_SRC) != 0u);
+          if (bits && lane == 0) {
+            const int32_t w0 = c0 >> 5;
+            srcw[r0 * nw + w0] = (uint32_t)sm;
+            if (w0 + 1 < nw) srcw[r0 * nw + w0 + 1] = (uint32_t)(sm >> 32);
+          }
         }
       }
     }
