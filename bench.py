@@ -77,6 +77,14 @@ def summarize_pmc(fetch_dir, write_dir, cfetch_dir, cwrite_dir, kernel="raster_k
             "bytes_per_launch": fk * 1024.0 * f_scale + wk * 1024.0 * w_scale}
 
 
+_T0 = time.perf_counter()
+
+
+def progress(msg):
+    """One progress line on stderr (the JSON result is the only stdout line)."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
+
+
 def cpu_baseline(w, records, stride, threads):
     from oracle import oracle as O
     O.build()
@@ -102,7 +110,9 @@ def bench_batch70(args):
     for w in ws:
         B.add(w.f32, w.i32, w.grid, w.tri, w.vtx, w.vnrm, w.ypr)
     t0 = time.perf_counter()
+    progress("batch70: tuning launch shapes")
     B.tune(4096)
+    progress("batch70: capturing the graph")
     tune_s = time.perf_counter() - t0
     t0 = time.perf_counter()
     B.capture()
@@ -132,7 +142,9 @@ def bench_batch70(args):
     B.close()
     # the same sweeps, mesh by mesh, default launch shape, no graph
     plain_ms = 0.0
-    for w in ws:
+    for k, w in enumerate(ws):
+        if k % 10 == 0:
+            progress(f"batch70: mesh-by-mesh sweep {k} of {len(ws)}")
         with _lib.Sweep(w.f32, w.i32, w.grid) as s:
             s.set_mesh(w.tri, w.vtx, w.vnrm)
             s.set_orientations(w.ypr)
@@ -164,6 +176,7 @@ def refine_comparison(w):
     srch = OrientationSearch(w.f32, w.i32, w.grid, w.tri, w.vtx, w.vnrm)
     srch.refined()  # warm-up (kernel loads, torch initialisation)
     ref = srch.refined()
+    progress("refine: dense 0.1 degree grid")
     dense = srch.dense(0.1)
     srch.close()
     return {"dense_grid_deg": 0.1, "dense": dense, "refined": ref,
@@ -285,16 +298,20 @@ def main():
         "argmin": int(_lib.find_first_optimal(rec["Mss"])),
     }
     s.close()
+    progress(f"sweep: {value:.0f} orientations/s, batch {info['batch']}")
     if not args.no_refine:
         out["refine"] = refine_comparison(w)
+        progress(f"refine: {out['refine']['wall_speedup']}x less wall time")
     if not args.no_batch and args.workload == "dsphere":
         b = bench_batch70(argparse.Namespace(steps=2, warmup=1))
+        progress(f"batch70: {b['value']:.0f} orientations/s")
         out["batch70"] = {k: b[k] for k in ("value", "unit", "config", "graph_ms", "plain_per_mesh_ms",
                                             "graph_vs_plain_speedup", "tune_s", "capture_s", "tuned_shapes",
                                             "flags")}
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(w, rec, args.cpu_stride, threads)
+        progress(f"cpu baseline: {out['cpu_baseline']['value']} orientations/s")
     print(json.dumps(out), flush=True)
 
 

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -209,6 +210,7 @@ struct tomo_handle {
   // scratch: one slot-table region per in-flight orientation
   ScratchLayout lay{};
   uint8_t *d_scratch = nullptr;
+  size_t scratch_cap = 0;      // bytes of the block behind d_scratch
   OrientHdr *d_hdr = nullptr;
   int32_t scratch_batch = 0;
   int64_t scratch_region = 0;  // bytes per orientation region of the current allocation
@@ -236,15 +238,75 @@ void free_stage_events(tomo_handle *h) {
   h->stage_of.clear();
 }
 
+// Slot-table scratch is large (tens of GB at the default batch) and hipMalloc / hipFree of such
+// blocks take about a second each, so the most recently released block is kept for the next
+// handle that needs one (any size up to 4x larger than asked for) instead of being freed.
+std::mutex g_scratch_mu;
+struct {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  int device = -1;
+} g_scratch_cache;
+
+hipError_t scratch_alloc(size_t bytes, int device, uint8_t **out, size_t *cap) {
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_cache.ptr && g_scratch_cache.device == device && g_scratch_cache.bytes >= bytes &&
+        g_scratch_cache.bytes / 4 <= bytes) {
+      *out = (uint8_t *)g_scratch_cache.ptr;
+      *cap = g_scratch_cache.bytes;
+      g_scratch_cache.ptr = nullptr;
+      return hipSuccess;
+    }
+  }
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {  // the cached block may be what stands in the way
+    void *old = nullptr;
+    {
+      std::lock_guard<std::mutex> lk(g_scratch_mu);
+      old = g_scratch_cache.ptr;
+      g_scratch_cache.ptr = nullptr;
+    }
+    if (!old) return e;
+    (void)hipGetLastError();
+    hipFree(old);
+    e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return e;
+  }
+  *out = (uint8_t *)p;
+  *cap = bytes;
+  return hipSuccess;
+}
+
+size_t scratch_cached(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return (g_scratch_cache.ptr && g_scratch_cache.device == device) ? g_scratch_cache.bytes : 0;
+}
+
+// Give a block back (its stream must be idle): it replaces the cached one, which is freed.
+void scratch_release(uint8_t *p, size_t cap, int device) {
+  if (!p) return;
+  void *old = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    old = g_scratch_cache.ptr;
+    g_scratch_cache.ptr = p;
+    g_scratch_cache.bytes = cap;
+    g_scratch_cache.device = device;
+  }
+  if (old) hipFree(old);
+}
+
 int ensure_scratch(tomo_handle *h, int32_t batch) {
   if (batch <= h->scratch_batch && h->lay.bytes == h->scratch_region) return TOMO_OK;
   if (h->shared) return fail(TOMO_E_STATE, "shared scratch does not fit this launch shape");
   HIP_TRY(hipStreamSynchronize(h->stream));
-  if (h->d_scratch) { HIP_TRY(hipFree(h->d_scratch)); h->d_scratch = nullptr; }
+  if (h->d_scratch) { scratch_release(h->d_scratch, h->scratch_cap, h->device); h->d_scratch = nullptr; }
   if (h->d_hdr) { HIP_TRY(hipFree(h->d_hdr)); h->d_hdr = nullptr; }
   h->scratch_batch = 0;
   const size_t bytes = (size_t)h->lay.bytes * (size_t)batch;
-  if (hipMalloc(&h->d_scratch, bytes) != hipSuccess) {
+  if (scratch_alloc(bytes, h->device, &h->d_scratch, &h->scratch_cap) != hipSuccess) {
     h->d_scratch = nullptr;
     return fail(TOMO_E_OOM, "scratch allocation failed");
   }
@@ -261,16 +323,24 @@ int ensure_scratch(tomo_handle *h, int32_t batch) {
   return TOMO_OK;
 }
 
-// Orientations in flight per batch: as requested, else 2048, within a quarter of free memory.
+// Orientations in flight per batch: as requested; else up to 8192 within a quarter of free
+// memory (counting the handle's current scratch and the cached block as free), split evenly over
+// the launches so the last one is not a short tail.
 int32_t auto_batch(tomo_handle *h, int32_t n) {
-  int32_t g = h->batch_req > 0 ? h->batch_req : 2048;
+  const bool requested = h->batch_req > 0;
+  int32_t g = requested ? h->batch_req : 8192;
   size_t freeb = 0, total = 0;
   if (g > h->scratch_batch && hipMemGetInfo(&freeb, &total) == hipSuccess && h->lay.bytes > 0) {
-    const size_t cap = (freeb / 4) / (size_t)h->lay.bytes;
+    const size_t mine = (h->shared ? 0 : h->scratch_cap) + scratch_cached(h->device);
+    const size_t cap = ((freeb + mine) / 4) / (size_t)h->lay.bytes;
     if (cap >= 1 && (size_t)g > cap) g = (int32_t)cap;
   }
   if (g > n) g = n;
   if (g < 1) g = 1;
+  if (!requested) {
+    const int32_t launches = (n + g - 1) / g;
+    g = (n + launches - 1) / launches;
+  }
   return g;
 }
 
@@ -765,7 +835,7 @@ int tomo_destroy(tomo_handle *h) {
     h->stream = h->own_stream;
   }
   if (h->d_hdr) hipFree(h->d_hdr);
-  if (h->d_scratch) hipFree(h->d_scratch);
+  if (h->d_scratch) scratch_release(h->d_scratch, h->scratch_cap, h->device);
   if (h->d_rec) hipFree(h->d_rec);
   if (h->d_rot) hipFree(h->d_rot);
   if (h->d_tri) hipFree(h->d_tri);
@@ -800,6 +870,7 @@ struct tomo_batch {
   };
   std::vector<Tune> tune;
   uint8_t *d_scratch = nullptr;
+  size_t scratch_cap = 0;      // bytes of the block behind d_scratch
   OrientHdr *d_hdr = nullptr;
   size_t scratch_bytes = 0, hdr_count = 0;
   hipGraph_t graph = nullptr;
@@ -814,7 +885,7 @@ static void batch_drop_graph(tomo_batch *b) {
 
 static void release_private_scratch(tomo_handle *h) {
   hipStreamSynchronize(h->stream);
-  if (h->d_scratch) { hipFree(h->d_scratch); h->d_scratch = nullptr; }
+  if (h->d_scratch) { scratch_release(h->d_scratch, h->scratch_cap, h->device); h->d_scratch = nullptr; }
   if (h->d_hdr) { hipFree(h->d_hdr); h->d_hdr = nullptr; }
   h->scratch_batch = 0;
   h->scratch_region = 0;
@@ -833,7 +904,7 @@ static void batch_unshare(tomo_batch *b) {
       h->scratch_batch = 0;
       h->scratch_region = 0;
     }
-  if (b->d_scratch) { hipFree(b->d_scratch); b->d_scratch = nullptr; }
+  if (b->d_scratch) { scratch_release(b->d_scratch, b->scratch_cap, b->device); b->d_scratch = nullptr; }
   if (b->d_hdr) { hipFree(b->d_hdr); b->d_hdr = nullptr; }
   b->scratch_bytes = b->hdr_count = 0;
 }
@@ -903,7 +974,7 @@ static float time_shape(tomo_handle *h, const tomo_batch::Tune &t, int32_t b0, i
 // Autotuner.  Members are grouped by (grid, ceil(log2(nTri))); each group's largest member is
 // timed on `sample` orientations from the middle of its list with a coordinate descent over
 // rasteriser tile extent along y {32, 64, 128, 256}, inline slot capacity {8, 16, 24} and
-// orientations in flight {1024, 2048, 4096} (within a quarter of free memory); shapes that leave
+// orientations in flight {1024, 2048, 4096, 8192} (within a quarter of free memory); shapes that leave
 // a slot unresolved are rejected.  The group's winner applies to all its members.
 int tomo_batch_tune(tomo_batch *b, int32_t sample) {
   if (!b) return fail(TOMO_E_INVALID_ARG, "null batch");
@@ -951,7 +1022,7 @@ int tomo_batch_tune(tomo_batch *b, int32_t sample) {
     }
     size_t freeb = 0, total = 0;
     hipMemGetInfo(&freeb, &total);
-    for (int32_t bs : {1024, 4096}) {
+    for (int32_t bs : {1024, 4096, 8192}) {
       tomo_batch::Tune t = best;
       t.batch = bs < ns ? bs : ns;
       const size_t need = (size_t)make_layout(h->opt.grid, t.inl).bytes * (size_t)t.batch;
@@ -998,7 +1069,7 @@ int tomo_batch_capture(tomo_batch *b) {
     sbytes = std::max(sbytes, (size_t)h->lay.bytes * bs);
     hcount = std::max(hcount, bs);
   }
-  if (hipMalloc(&b->d_scratch, sbytes) != hipSuccess) {
+  if (scratch_alloc(sbytes, b->device, &b->d_scratch, &b->scratch_cap) != hipSuccess) {
     b->d_scratch = nullptr;
     return fail(TOMO_E_OOM, "shared scratch allocation failed");
   }
