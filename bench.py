@@ -32,10 +32,10 @@ def algorithmic_bytes(n_vtx, n_tri, rec):
 
 
 def raster_bytes(n_vtx, n_tri, p_bar, s_bar):
-    """Bytes the rasteriser kernel must move per orientation: int32 indices, a 16-byte rotated
-    vertex (x, y, z, normal z) per vertex, an 8-byte slot entry written per rasterised row and a
-    4-byte slot counter read and written per touched slot."""
-    return 12.0 * n_tri + 16.0 * n_vtx + 8.0 * p_bar + 8.0 * s_bar
+    """Bytes the rasteriser kernel must move per orientation: int32 indices, the raw fp32
+    position and normal of every vertex (it rotates them itself), an 8-byte slot entry written
+    per rasterised row and a 4-byte slot counter read and written per touched slot."""
+    return 12.0 * n_tri + 24.0 * n_vtx + 8.0 * p_bar + 8.0 * s_bar
 
 
 def copy_calibration(gib=1):

@@ -194,12 +194,10 @@ struct tomo_handle {
   // mesh
   int32_t n_tri = 0, n_vtx = 0;
   float center[3] = {0, 0, 0};
-  int32_t *d_tri = nullptr;     // triangles in culling order
-  int32_t *d_tri_id = nullptr;  // the caller's index of each
   float *d_chunk = nullptr;     // [n_chunks][6] raw AABB of each kChunk triangles
   int32_t n_chunks = 0;
-  float *d_soa = nullptr;  // 6 * n_vtx: x, y, z, nx, ny, nz
-  float4 *d_vaos = nullptr;  // 2 * n_vtx: (x, y, z, nx), (ny, nz, 0, 0)
+  float *d_soa = nullptr;  // 3 * n_vtx: x, y, z
+  float4 *d_trec = nullptr;  // 5 * n_tri: per-triangle record (culling order)
   float *d_bed_dist = nullptr;
   // orientations
   int32_t n_orient = 0;
@@ -390,17 +388,12 @@ void vm_from_record(const tomo_handle *h, const tomo_record &r, float *vm) {
 
 SweepParams base_params(tomo_handle *h) {
   SweepParams p{};
-  p.tri = h->d_tri;
-  p.tri_id = h->d_tri_id;
   p.chunk_box = h->d_chunk;
   p.n_chunks = h->n_chunks;
-  p.vaos = h->d_vaos;
+  p.trec = h->d_trec;
   p.vx = h->d_soa;
   p.vy = h->d_soa + h->n_vtx;
   p.vz = h->d_soa + 2 * (size_t)h->n_vtx;
-  p.nx = h->d_soa + 3 * (size_t)h->n_vtx;
-  p.ny = h->d_soa + 4 * (size_t)h->n_vtx;
-  p.nz = h->d_soa + 5 * (size_t)h->n_vtx;
   p.n_tri = h->n_tri;
   p.n_vtx = h->n_vtx;
   p.cx = h->center[0]; p.cy = h->center[1]; p.cz = h->center[2];
@@ -544,11 +537,9 @@ int tomo_set_mesh(tomo_handle *h, int32_t n_tri, const int32_t *tri, int32_t n_v
     if (tri[i] < 0 || tri[i] >= n_vtx) return fail(TOMO_E_INVALID_ARG, "triangle index out of range");
   if (n_tri >= (1 << 30)) return fail(TOMO_E_INVALID_ARG, "too many triangles");
   HIP_TRY(hipStreamSynchronize(h->stream));
-  if (h->d_tri) { HIP_TRY(hipFree(h->d_tri)); h->d_tri = nullptr; }
-  if (h->d_tri_id) { HIP_TRY(hipFree(h->d_tri_id)); h->d_tri_id = nullptr; }
+  if (h->d_trec) { HIP_TRY(hipFree(h->d_trec)); h->d_trec = nullptr; }
   if (h->d_chunk) { HIP_TRY(hipFree(h->d_chunk)); h->d_chunk = nullptr; }
   if (h->d_soa) { HIP_TRY(hipFree(h->d_soa)); h->d_soa = nullptr; }
-  if (h->d_vaos) { HIP_TRY(hipFree(h->d_vaos)); h->d_vaos = nullptr; }
   h->n_tri = n_tri;
   h->n_vtx = n_vtx;
   // raw AABB centre (STomoAABB3Df.cpp:42-72): float add, then *0.5 in double
@@ -560,21 +551,11 @@ int tomo_set_mesh(tomo_handle *h, int32_t n_tri, const int32_t *tri, int32_t n_v
       mx[k] = (mx[k] >= c) ? mx[k] : c;
     }
   for (int k = 0; k < 3; k++) h->center[k] = (float)((double)(mn[k] + mx[k]) * 0.5);
-  std::vector<float> soa((size_t)6 * (n_vtx > 0 ? n_vtx : 1));
+  std::vector<float> soa((size_t)3 * (n_vtx > 0 ? n_vtx : 1));
   for (int32_t v = 0; v < n_vtx; v++)
-    for (int k = 0; k < 3; k++) {
-      soa[(size_t)k * n_vtx + v] = vtx[(size_t)v * 3 + k];
-      soa[(size_t)(3 + k) * n_vtx + v] = vtx_nrm[(size_t)v * 3 + k];
-    }
+    for (int k = 0; k < 3; k++) soa[(size_t)k * n_vtx + v] = vtx[(size_t)v * 3 + k];
   HIP_TRY(hipMalloc(&h->d_soa, soa.size() * sizeof(float)));
   HIP_TRY(hipMemcpy(h->d_soa, soa.data(), soa.size() * sizeof(float), hipMemcpyHostToDevice));
-  std::vector<float> aos((size_t)8 * (n_vtx > 0 ? n_vtx : 1), 0.0f);
-  for (int32_t v = 0; v < n_vtx; v++) {
-    for (int k = 0; k < 3; k++) aos[(size_t)v * 8 + k] = vtx[(size_t)v * 3 + k];
-    for (int k = 0; k < 3; k++) aos[(size_t)v * 8 + 3 + k] = vtx_nrm[(size_t)v * 3 + k];
-  }
-  HIP_TRY(hipMalloc(&h->d_vaos, aos.size() * sizeof(float)));
-  HIP_TRY(hipMemcpy(h->d_vaos, aos.data(), aos.size() * sizeof(float), hipMemcpyHostToDevice));
   // Culling order: triangles sorted by the Morton code of their centroid, so each run of kChunk
   // triangles is compact in space and its AABB bounds where the rasteriser can touch it.
   std::vector<uint64_t> code((size_t)n_tri);
@@ -622,11 +603,23 @@ int tomo_set_mesh(tomo_handle *h, int32_t n_tri, const int32_t *tri, int32_t n_v
         }
     for (int k = 0; k < 6; k++) box[(size_t)c * 6 + k] = b[k];
   }
-  HIP_TRY(hipMalloc(&h->d_tri, sizeof(int32_t) * ord_tri.size()));
-  HIP_TRY(hipMalloc(&h->d_tri_id, sizeof(int32_t) * ord_id.size()));
+  // One 80-byte record per triangle in culling order: raw positions, raw normals, the caller's
+  // index (so a wave reads its 64 triangles with five coalesced loads and no index indirection).
+  std::vector<float> rec((size_t)20 * (n_tri > 0 ? n_tri : 1), 0.0f);
+  for (int32_t i = 0; i < n_tri; i++) {
+    float *r = &rec[(size_t)i * 20];
+    for (int j = 0; j < 3; j++) {
+      const size_t v = (size_t)ord_tri[(size_t)i * 3 + j];
+      for (int k = 0; k < 3; k++) {
+        r[j * 3 + k] = vtx[v * 3 + k];
+        r[9 + j * 3 + k] = vtx_nrm[v * 3 + k];
+      }
+    }
+    std::memcpy(&r[18], &ord_id[(size_t)i], sizeof(int32_t));
+  }
+  HIP_TRY(hipMalloc(&h->d_trec, rec.size() * sizeof(float)));
+  HIP_TRY(hipMemcpy(h->d_trec, rec.data(), rec.size() * sizeof(float), hipMemcpyHostToDevice));
   HIP_TRY(hipMalloc(&h->d_chunk, sizeof(float) * box.size()));
-  HIP_TRY(hipMemcpy(h->d_tri, ord_tri.data(), sizeof(int32_t) * ord_tri.size(), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(h->d_tri_id, ord_id.data(), sizeof(int32_t) * ord_id.size(), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(h->d_chunk, box.data(), sizeof(float) * box.size(), hipMemcpyHostToDevice));
   return TOMO_OK;
 }
@@ -690,7 +683,7 @@ int tomo_set_tuning(tomo_handle *h, int32_t knob, int32_t value) {
 int tomo_run(tomo_handle *h, int32_t begin, int32_t end) {
   if (!h) return fail(TOMO_E_INVALID_ARG, "null handle");
   if (begin < 0 || end > h->n_orient || begin > end) return fail(TOMO_E_INVALID_ARG, "bad orientation range");
-  if (!h->d_soa || !h->d_tri) return fail(TOMO_E_STATE, "no mesh");
+  if (!h->d_soa || !h->d_trec) return fail(TOMO_E_STATE, "no mesh");
   h->last_begin = begin;
   h->last_end = end;
   h->timed = false;
@@ -756,7 +749,7 @@ int tomo_get_info(tomo_handle *h, int32_t *grid, int32_t *batch, int64_t *scratc
 static int debug_run(tomo_handle *h, const float ypr[3], tomo_record *rec, int32_t *cnt_pre,
                      int32_t *cnt_final, int32_t *slot_vo, int32_t *slot_vss, uint64_t *rows) {
   if (!h || !ypr || !rec) return fail(TOMO_E_INVALID_ARG, "null argument");
-  if (!h->d_soa || !h->d_tri) return fail(TOMO_E_STATE, "no mesh");
+  if (!h->d_soa || !h->d_trec) return fail(TOMO_E_STATE, "no mesh");
   int rc = tomo_set_orientations(h, 1, ypr);
   if (rc) return rc;
   rc = ensure_scratch(h, 1);
@@ -838,11 +831,9 @@ int tomo_destroy(tomo_handle *h) {
   if (h->d_scratch) scratch_release(h->d_scratch, h->scratch_cap, h->device);
   if (h->d_rec) hipFree(h->d_rec);
   if (h->d_rot) hipFree(h->d_rot);
-  if (h->d_tri) hipFree(h->d_tri);
-  if (h->d_tri_id) hipFree(h->d_tri_id);
   if (h->d_chunk) hipFree(h->d_chunk);
   if (h->d_soa) hipFree(h->d_soa);
-  if (h->d_vaos) hipFree(h->d_vaos);
+  if (h->d_trec) hipFree(h->d_trec);
   if (h->d_bed_dist) hipFree(h->d_bed_dist);
   if (h->ev0) hipEventDestroy(h->ev0);
   if (h->ev1) hipEventDestroy(h->ev1);

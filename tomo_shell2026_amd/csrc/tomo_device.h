@@ -63,12 +63,10 @@ struct ScratchLayout {
 };
 
 struct SweepParams {
-  // mesh (structure of arrays, raw coordinates in voxel units)
-  const int32_t *tri;       // [n_tri][3], triangles in culling order (Morton order of centroids)
-  const int32_t *tri_id;    // [n_tri] the mesh's own index of each triangle (entry ranks)
+  // mesh (raw coordinates in voxel units); triangles in culling order (Morton order of centroids)
   const float *chunk_box;   // [n_chunks][6] raw-coordinate AABB (min xyz, max xyz) of kChunk triangles
-  const float *vx, *vy, *vz, *nx, *ny, *nz;
-  const float4 *vaos;       // [n_vtx][2] raw (x, y, z, nx), (ny, nz, 0, 0)
+  const float *vx, *vy, *vz;  // [n_vtx] raw vertex positions (structure of arrays)
+  const float4 *trec;       // [n_tri][5] raw v0, v1, v2 (x, y, z), n0, n1, n2, caller's index bits, 0
   int32_t n_tri, n_vtx, n_chunks;
   float cx, cy, cz;        // raw AABB centre (STomoSh_Base.cpp:75-82)
   // orientations of this batch: rot/rec already offset to the batch's first orientation
