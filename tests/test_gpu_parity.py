@@ -168,6 +168,18 @@ def test_tile_shapes_match_oracle(oracle_mod, tile_y):
     assert not bad, "\n".join(bad)
 
 
+def test_one_tile_many_chunks(oracle_mod):
+    """The 69,120-triangle sphere in a 128 x 128 grid: one rasteriser tile holds every one of its
+    1,080 chunks, more than one LDS chunk list holds, so the tile culls and rasterises in several
+    windows (one ending inside a culling pass)."""
+    v, f = synth.displaced_latlong_sphere(seed=1)
+    v, _ = synth.fit_to_grid(v / 0.5, 128)
+    assert len(f) // 64 > 896
+    case = C.make(v, f, 128, synth.ypr_grid(np.arange(0, 360, 53), np.arange(0, 61, 20)), bed=synth.BED_RAFT)
+    g, r = assert_parity(oracle_mod, case, "dsphere69120 in a 128 grid (one tile)")
+    assert (g["n_slots"] > 0).all()
+
+
 def test_full_sweep_properties():
     """configs-size sweep: 21,960 orientations of the 69,120-triangle mesh."""
     w = synth.displaced_sphere_workload()

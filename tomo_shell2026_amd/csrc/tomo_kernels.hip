@@ -59,6 +59,15 @@ __device__ __forceinline__ T wave_sum(T v) {
 
 __device__ __forceinline__ int msb64(uint64_t m) { return 63 - __clzll((long long)m); }
 
+// Index of the calling wave in its work-group, as a wave-uniform (scalar) value: state derived
+// from it (turn counters, LDS row bases) then stays in scalar registers.
+__device__ __forceinline__ int wave_index() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
+
+// Number of lanes below the caller's whose bit is set in m.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 struct Tables {
   uint2 *ent;  // [inl][G2] {key, rank}, layer-major
   uint32_t *count, *bedinfo, *off, *cursor, *pool_slot, *pool_key, *pool_rank,
@@ -535,7 +544,7 @@ __device__ __forceinline__ int32_t group_or(int32_t f, int32_t *red, int lane, i
 // counters of the previous orientation's large slots that pair_kernel left set.
 __global__ __launch_bounds__(kThreads) void aabb_kernel(SweepParams p) {
   __shared__ float red[kWaves][3];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = wave_index();
   const int32_t b = blockIdx.x;
   const float *Rg = p.rot + (int64_t)b * 9;
   const float r0 = Rg[0], r1 = Rg[1], r2 = Rg[2], r3 = Rg[3], r4 = Rg[4], r5 = Rg[5], r6 = Rg[6],
@@ -597,6 +606,7 @@ __global__ __launch_bounds__(kThreads) void aabb_kernel(SweepParams p) {
 // holding its clamped box corner (x0, y0), which walks the whole box to count hits.
 // Work-groups follow xcd_map so an orientation's tiles share one XCD's L2.
 enum : int { TR_A0 = 0, TR_A1 = 3, TR_A2 = 6, TR_NZ = 9, TR_STRIDE = 13 };
+constexpr int kCullCap = 896;  // chunk list entries per window (LDS)
 
 struct RasterLds {
   uint32_t cnt[kTileSlots / 2];  // the tile's slot counters, 16 bits each
@@ -608,7 +618,12 @@ struct RasterLds {
   uint8_t hits[kTileWaves][64];
   int32_t red[kTileWaves][5];
   uint64_t zred[kTileWaves];
+  uint16_t clist[kCullCap];  // kept chunks of the current window (offsets from its first chunk)
+  uint32_t wcnt[kTileWaves];
+  uint32_t take;             // next list position to hand out
+  int32_t cut;
 };
+static_assert(sizeof(RasterLds) <= 80 * 1024, "two rasteriser work-groups must fit one CU's LDS");
 
 // Conservative pixel range along one axis of a chunk's vertices after rotation (row r of the
 // orientation matrix about the centre c) and placement (minus k): the float rotation and
@@ -639,7 +654,7 @@ __device__ __forceinline__ void chunk_span(const float *bmin, const float *bmax,
 
 __global__ __launch_bounds__(kTileThreads) void raster_kernel(SweepParams p) {
   __shared__ RasterLds L;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = wave_index();
   int32_t b, tile;
   if (!xcd_map(p.tiles, p.n_batch, b, tile)) return;
   const int32_t G = p.grid;
@@ -663,25 +678,18 @@ __global__ __launch_bounds__(kTileThreads) void raster_kernel(SweepParams p) {
   em.lcnt = L.cnt;
   em.xa = xa; em.xb = xb; em.ya = ya; em.yb = yb; em.yshift = p.tile_y_shift;
   em.bx0 = G; em.bx1 = -1; em.by0 = G; em.by1 = -1; em.flags = 0; em.zabs = 0ull;
-  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
   float (*tri)[TR_STRIDE] = L.tri[wave];
-  // The wave's chunks: every wave culls the same 64 chunks at a time and the kept ones are dealt
-  // out to the waves in turn.
-  int32_t it_c0 = -64, it_turn = 0;
-  uint64_t it_kept = 0ull;
-  auto next_chunk = [&]() -> int32_t {
-    for (;;) {
-      while (it_kept) {
-        const int cl = __ffsll((unsigned long long)it_kept) - 1;
-        it_kept &= it_kept - 1ull;
-        const bool mine = it_turn == wave;
-        it_turn = it_turn == kTileWaves - 1 ? 0 : it_turn + 1;
-        if (mine) return it_c0 + cl;
-      }
-      it_c0 += 64;
-      if (it_c0 >= p.n_chunks) return -1;
+  // Chunks in windows: the work-group culls chunks together (one per thread) and lists the kept
+  // ones in LDS, in chunk order, until the list is full; then each wave takes the listed chunks
+  // one at a time from a shared LDS counter (the waves stay balanced whatever each chunk costs).
+  int32_t next = 0;  // first chunk not yet culled (the same in every thread)
+  while (next < p.n_chunks) {
+    const int32_t wbase = next;
+    uint32_t n = 0;  // listed chunks of this window
+    if (tid == 0) L.take = 0u;
+    while (next < p.n_chunks && n < (uint32_t)kCullCap && next - wbase <= 65535 - kTileThreads) {
+      const int32_t ch = next + tid;
       bool keep = false;
-      const int32_t ch = it_c0 + lane;
       if (ch < p.n_chunks) {
         const float *bx = p.chunk_box + (int64_t)ch * 6;
         const float bmin[3] = {bx[0], bx[1], bx[2]}, bmax[3] = {bx[3], bx[4], bx[5]};
@@ -690,11 +698,34 @@ __global__ __launch_bounds__(kTileThreads) void raster_kernel(SweepParams p) {
         chunk_span(bmin, bmax, ctr, R3, R4, R5, ky, ly, hy);
         keep = hx >= exlo && lx <= exhi && hy >= eylo && ly <= eyhi;
       }
-      it_kept = __ballot(keep);
+      const uint64_t km = __ballot(keep);
+      if (lane == 0) L.wcnt[wave] = (uint32_t)__popcll(km);
+      __syncthreads();
+      uint32_t before = 0u, total = 0u;
+#pragma unroll
+      for (int w2 = 0; w2 < kTileWaves; w2++) {
+        const uint32_t c = L.wcnt[w2];
+        before += w2 < wave ? c : 0u;
+        total += c;
+      }
+      const uint32_t pos = n + before + lanes_below(km);
+      if (keep && pos < (uint32_t)kCullCap) L.clist[pos] = (uint16_t)(ch - wbase);
+      if (keep && pos == (uint32_t)kCullCap) L.cut = ch;  // the first kept chunk that did not fit
+      __syncthreads();
+      if (n + total <= (uint32_t)kCullCap) {
+        next = min(next + kTileThreads, p.n_chunks);
+        n += total;
+      } else {
+        next = L.cut;
+        n = (uint32_t)kCullCap;
+      }
     }
-  };
-  for (int32_t ch = next_chunk(); ch >= 0; ch = next_chunk()) {
-    {
+    for (;;) {
+      uint32_t idx = 0u;
+      if (lane == 0) idx = atomicAdd(&L.take, 1u);
+      idx = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl(idx, 0));
+      if (idx >= n) break;
+      const int32_t ch = wbase + __builtin_amdgcn_readfirstlane((int)L.clist[idx]);
       const int32_t tr = ch * kChunk + lane;
       const bool active = tr < p.n_tri;
       int32_t x0 = 0, x1 = -1, y0 = 0, y1 = -1;
@@ -764,27 +795,22 @@ __global__ __launch_bounds__(kTileThreads) void raster_kernel(SweepParams p) {
       float rax = e2x * sc.ax, rbx = e2x * sc.bx;
       uint32_t qn = 0;
       wave_sync();
+      // Column-major walk of the box, written without divergent branches: every lane advances
+      // every step (a finished lane's position runs on past x1 and is never tested).
+      uint32_t xmeta = (uint32_t)lane | ((uint32_t)(x + 256) << 6);
       for (;;) {
-        bool cand = false;
-        const int32_t cxi = x, cyi = y;
-        if (!done) {
-          cand = scan_candidate(sc, rax, rbx, centre_offset(y, p.h2, v0y));
-          if (++y > y1) {
-            y = y0;
-            if (++x > x1) {
-              done = true;
-            } else {
-              e2x = centre_offset(x, p.h2, v0x);
-              rax = e2x * sc.ax;
-              rbx = e2x * sc.bx;
-            }
-          }
-        }
+        const bool cand = (!done) & scan_candidate(sc, rax, rbx, centre_offset(y, p.h2, v0y));
+        const uint32_t meta = xmeta | ((uint32_t)(y + 256) << 19);
+        const bool wrap = y >= y1;
+        y = wrap ? y0 : y + 1;
+        x += wrap ? 1 : 0;
+        done = done | (x > x1);
+        e2x = centre_offset(x, p.h2, v0x);
+        rax = e2x * sc.ax;
+        rbx = e2x * sc.bx;
+        xmeta = (uint32_t)lane | ((uint32_t)(x + 256) << 6);
         const uint64_t m = __ballot(cand);
-        if (cand) {
-          const uint32_t pos = qn + (uint32_t)__popcll(m & lt);
-          L.qmeta[wave][pos] = (uint32_t)lane | ((uint32_t)(cxi + 256) << 6) | ((uint32_t)(cyi + 256) << 19);
-        }
+        if (cand) L.qmeta[wave][qn + lanes_below(m)] = meta;
         qn += (uint32_t)__popcll(m);
         const bool all_done = __ballot(!done) == 0ull;
         if (qn >= 64u || (all_done && qn > 0u)) {
@@ -858,8 +884,8 @@ __global__ __launch_bounds__(kTileThreads) void raster_kernel(SweepParams p) {
       }
       wave_sync();
     }
+    __syncthreads();  // every listed chunk is done before the list is refilled
   }
-  __syncthreads();
   // Write the tile's non-zero counters (the global table is zero elsewhere) and list the slots
   // with more than kSmall entries.
   {
@@ -882,7 +908,7 @@ __global__ __launch_bounds__(kTileThreads) void raster_kernel(SweepParams p) {
           uint32_t base = 0u;
           if (lane == 0) base = atomicAdd(&h.big_top, (uint32_t)__popcll(bm));
           base = __shfl(base, 0);
-          if (big) t.big[base + (uint32_t)__popcll(bm & lt)] = s;
+          if (big) t.big[base + lanes_below(bm)] = s;
         }
       }
     }
@@ -926,7 +952,7 @@ __global__ __launch_bounds__(kTileThreads) void raster_kernel(SweepParams p) {
 // arrival order; group them per slot (exclusive scan over the large-slot list, then scatter).
 __global__ __launch_bounds__(kThreads) void compact_kernel(SweepParams p) {
   __shared__ uint32_t scan[kWaves];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = wave_index();
   const int32_t b = blockIdx.x;
   OrientHdr &h = p.hdr[b];
   const Tables t = tables_for(p, b);
@@ -999,7 +1025,7 @@ __device__ __forceinline__ void pair_body(const SweepParams &p) {
   __shared__ uint32_t pairq[3][kWaves][128];  // queued slots: s | entries << 24
   __shared__ int64_t red[kWaves][18];
   __shared__ int32_t fred[kWaves];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = wave_index();
   int32_t b, tile;
   const int32_t groups = PART == 0 ? p.pair_tiles : p.big_groups;  // work-groups per orientation
   if (!xcd_map(groups, p.n_batch, b, tile)) return;
@@ -1047,7 +1073,6 @@ __device__ __forceinline__ void pair_body(const SweepParams &p) {
   // network, so no wave runs a larger network than its slots need.
   if constexpr (PART == 0) {
   uint32_t n2 = 0u, n4 = 0u, n8 = 0u;  // queue fills (wave-uniform)
-  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
   uint32_t *const q2 = pairq[0][wave], *const q4 = pairq[1][wave], *const q8 = pairq[2][wave];
   const int32_t i1 = min(i0 + kPairTile, nbb);
   if (i0 + wave * 64 < i1) {
@@ -1075,17 +1100,17 @@ __device__ __forceinline__ void pair_body(const SweepParams &p) {
       }
       {
         const uint64_t m = __ballot(cls == 0);
-        if (cls == 0) q2[n2 + (uint32_t)__popcll(m & lt)] = e;
+        if (cls == 0) q2[n2 + lanes_below(m)] = e;
         n2 += (uint32_t)__popcll(m);
       }
       {
         const uint64_t m = __ballot(cls == 1);
-        if (cls == 1) q4[n4 + (uint32_t)__popcll(m & lt)] = e;
+        if (cls == 1) q4[n4 + lanes_below(m)] = e;
         n4 += (uint32_t)__popcll(m);
       }
       {
         const uint64_t m = __ballot(cls == 2);
-        if (cls == 2) q8[n8 + (uint32_t)__popcll(m & lt)] = e;
+        if (cls == 2) q8[n8 + lanes_below(m)] = e;
         n8 += (uint32_t)__popcll(m);
       }
       const bool last = base + kThreads >= i1;
@@ -1234,7 +1259,7 @@ __global__ __launch_bounds__(kThreads) void bed_kernel(SweepParams p) {
   __shared__ uint32_t srcw[kBedWords], rowany[kBedWords];
   __shared__ int64_t red[kWaves][6];
   __shared__ int32_t fred[kWaves];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = wave_index();
   int32_t b, band;
   if (!xcd_map(p.bed_bands, p.n_batch, b, band)) return;
   const int32_t G = p.grid;
