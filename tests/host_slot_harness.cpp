@@ -49,6 +49,22 @@ uint32_t harness_make_key(int z, int nz) { return make_key(z, nz); }
 // Coverage test of the rasteriser over the pixel box [x0, x1] x [y0, y1] of one margined
 // triangle, pixels x-major as the kernel walks them: out[i*5] = prefilter candidate,
 // out[i*5+1] = exact accept, out[i*5+2..4] = u, v, w bits.
+// The walk box [x0, x1] x [y0, y1] of one margined triangle after accept_range / clip_span, as
+// the rasteriser tightens it: out = {x0, x1, y0, y1, ok}.
+int harness_accept_box(const float *tri6, int x0, int x1, int y0, int y1, float h2, int32_t *out) {
+  TriSetup s = tri_setup(tri6[0], tri6[1], tri6[2], tri6[3], tri6[4], tri6[5]);
+  const float ex = std::max(std::fabs(centre_offset(x0, h2, s.v0x)), std::fabs(centre_offset(x1, h2, s.v0x)));
+  const float ey = std::max(std::fabs(centre_offset(y0, h2, s.v0y)), std::fabs(centre_offset(y1, h2, s.v0y)));
+  const TriScan sc = tri_scan(s, ex, ey);
+  const AcceptRange ar = accept_range(s, sc);
+  if (ar.ok && !tri_degenerate(s)) {
+    clip_span(x0, x1, ar.lox, ar.hix, s.v0x, h2);
+    clip_span(y0, y1, ar.loy, ar.hiy, s.v0y, h2);
+  }
+  out[0] = x0; out[1] = x1; out[2] = y0; out[3] = y1; out[4] = ar.ok ? 1 : 0;
+  return 0;
+}
+
 int harness_bary(const float *tri6, int x0, int x1, int y0, int y1, float h2, uint32_t *out) {
   TriSetup s = tri_setup(tri6[0], tri6[1], tri6[2], tri6[3], tri6[4], tri6[5]);
   const float ex = std::max(std::fabs(centre_offset(x0, h2, s.v0x)), std::fabs(centre_offset(x1, h2, s.v0x)));

@@ -194,6 +194,39 @@ def test_raster_coverage_matches_oracle(harness, oracle_mod):
     assert n_cand < 1.05 * n_acc + 0.01 * n_pix
 
 
+def test_accept_box_holds_every_accepted_pixel(harness):
+    """The rasteriser walks only the part of a triangle's pixel box that accept_range / clip_span
+    leave: every pixel the exact test accepts must lie inside it (ordinary, sliver, tiny, huge,
+    near-degenerate triangles, vertices on pixel centres, far from the origin), and for ordinary
+    small triangles the walk must actually shrink."""
+    harness.harness_bary.argtypes = (ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_float, ct.c_void_p)
+    harness.harness_bary.restype = ct.c_int
+    harness.harness_accept_box.argtypes = (ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_float,
+                                           ct.c_void_p)
+    harness.harness_accept_box.restype = ct.c_int
+    rng = np.random.default_rng(11)
+    full = walked = 0
+    for h2 in (np.float32(0.5), np.float32(0.25)):
+        for k, (t, x0, x1, y0, y1) in enumerate(_bary_cases(rng, 2800)):
+            tri6 = np.ascontiguousarray(t.ravel(), np.float32)
+            npx = (x1 - x0 + 1) * (y1 - y0 + 1)
+            out = np.zeros((npx, 5), np.uint32)
+            harness.harness_bary(tri6.ctypes.data_as(ct.c_void_p), x0, x1, y0, y1, h2, out.ctypes.data_as(ct.c_void_p))
+            box = np.zeros(5, np.int32)
+            harness.harness_accept_box(tri6.ctypes.data_as(ct.c_void_p), x0, x1, y0, y1, h2,
+                                       box.ctypes.data_as(ct.c_void_p))
+            acc = out[:, 1].reshape(x1 - x0 + 1, y1 - y0 + 1).astype(bool)
+            xs, ys = np.nonzero(acc)
+            if xs.size:
+                assert box[0] <= x0 + xs.min() and x0 + xs.max() <= box[1], (k, t, box)
+                assert box[2] <= y0 + ys.min() and y0 + ys.max() <= box[3], (k, t, box)
+            assert x0 <= box[0] and box[1] <= x1 and y0 <= box[2] and box[3] <= y1
+            if k % 7 in (0, 2):  # ordinary and tiny triangles
+                full += npx
+                walked += max(0, box[1] - box[0] + 1) * max(0, box[3] - box[2] + 1)
+    assert walked < 0.5 * full
+
+
 def test_manikin_generator():
     """Thin-shell manikin: ~87k triangles, two closed sheets before the opening cut, an open
     shell after it, deterministic per seed, and a footprint inside the 512 grid."""

@@ -652,7 +652,7 @@ __device__ __forceinline__ void chunk_span(const float *bmin, const float *bmax,
   }
 }
 
-__global__ __launch_bounds__(kTileThreads) void raster_kernel(SweepParams p) {
+__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6))) void raster_kernel(SweepParams p) {
   __shared__ RasterLds L;
   const int tid = threadIdx.x, lane = tid & 63, wave = wave_index();
   int32_t b, tile;
@@ -779,6 +779,12 @@ __global__ __launch_bounds__(kTileThreads) void raster_kernel(SweepParams p) {
         const float ex = fmaxf(fabsf(centre_offset(x0, p.h2, v0x)), fabsf(centre_offset(x1, p.h2, v0x)));
         const float ey = fmaxf(fabsf(centre_offset(y0, p.h2, v0y)), fabsf(centre_offset(y1, p.h2, v0y)));
         sc = tri_scan(ts, ex, ey);
+        // Walk only the columns and rows that can hold an accepted pixel.
+        const AcceptRange ar = accept_range(ts, sc);
+        if (ar.ok) {
+          clip_span(x0, x1, ar.lox, ar.hix, v0x, p.h2);
+          clip_span(y0, y1, ar.loy, ar.hiy, v0y, p.h2);
+        }
         float *rec = tri[lane];
 #pragma unroll
         for (int q = 0; q < 3; q++) {

@@ -113,6 +113,75 @@ TOMO_RHD bool scan_candidate(const TriScan &t, float rax, float rbx, float e2y) 
   return !(a < t.lo || b < t.lo || a > t.hia || a + b > t.his);
 }
 
+// Upper bound on 1 / x for finite x > 0 (at least 2^-10 above it).
+TOMO_RHD float rcp_up(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf(x) * 1.0009765625f;
+#else
+  return (1.0f / x) * 1.0009765625f;
+#endif
+}
+
+// Offsets from v0 that an accepted pixel centre can have: a tightening of the pixel box.
+// Every accepted pixel satisfies a >= lo, b >= lo and a + b <= his in real arithmetic (the
+// prefilter's bounds above).  Dividing by the real determinant den_g = d00 d11 - d01^2 of the
+// float Gram matrix, |den_g| >= Dl = D - Ed with Ed = 2^-19 d00 d11 + 2^-23 D (it bounds the
+// rounding of den), gives v_g, w_g >= -s1 and v_g + w_g <= 1 + s1 with s1 = (eps D + M + Ed) / Dl,
+// where (v_g, w_g) solves the float Gram system.  The true barycentric coordinates (v, w) of the
+// centre offset (e2 = v e0 + w e1 exactly) differ from them by at most
+// d = 2^-20 K (2 + 4 s1), K = (d00 + d11)^2 / Dl (the Gram matrix's rounding times its
+// conditioning), so (v, w) lies in v, w >= -s, v + w <= 1 + s with s = s1 + 2 d.  That region has
+// corners (-s, -s), (1 + 2s, -s), (-s, 1 + 2s), so
+//   min(0, e0x, e1x) - 2 s (|e0x| + |e1x|) <= e2x <= max(0, e0x, e1x) + 2 s (|e0x| + |e1x|)
+// and likewise along y.  ok is false (no tightening) for near-degenerate or non-finite input.
+struct AcceptRange {
+  float lox, hix, loy, hiy;
+  bool ok;
+};
+
+TOMO_RHD AcceptRange accept_range(const TriSetup &s, const TriScan &t) {
+  AcceptRange r;
+  const float D = __builtin_fabsf(s.den);
+  const float Ed = 0.0000019073486328125f * (s.d00 * s.d11) + 0.00000011920928955078125f * D;
+  const float Dl = D - Ed;
+  r.ok = Dl > 0.5f * D;  // false for NaN and infinities
+  const float inv = rcp_up(r.ok ? Dl : 1.0f);
+  const float s1 = ((-t.lo) + Ed) * inv;  // -lo = eps D + M
+  const float dsum = s.d00 + s.d11;
+  const float dk = 0.00000095367431640625f * (dsum * dsum * inv) * (2.0f + 4.0f * s1);
+  const float sl = (s1 + 2.0f * dk) * 1.0009765625f;
+  const float mx = 2.0f * sl * (__builtin_fabsf(s.e0x) + __builtin_fabsf(s.e1x)) * 1.0009765625f;
+  const float my = 2.0f * sl * (__builtin_fabsf(s.e0y) + __builtin_fabsf(s.e1y)) * 1.0009765625f;
+  const float nx = s.e0x < s.e1x ? s.e0x : s.e1x, xx = s.e0x > s.e1x ? s.e0x : s.e1x;
+  const float ny = s.e0y < s.e1y ? s.e0y : s.e1y, xy = s.e0y > s.e1y ? s.e0y : s.e1y;
+  r.lox = (nx < 0.0f ? nx : 0.0f) - mx;
+  r.hix = (xx > 0.0f ? xx : 0.0f) + mx;
+  r.loy = (ny < 0.0f ? ny : 0.0f) - my;
+  r.hiy = (xy > 0.0f ? xy : 0.0f) + my;
+  r.ok = r.ok && (r.hix - r.lox) < 1.0e6f && (r.hiy - r.loy) < 1.0e6f;
+  return r;
+}
+
+// Drop from [i0, i1] the pixels whose float centre offset (centre_offset(i, h2, v0)) is surely
+// below lo or above hi.  The centre offset is within 2^-22 (|i| + |v0| + h2 + 2) of
+// i + eps + h2 - v0, and the bounds below are formed with errors of the same order; the margin
+// covers both several times over.
+TOMO_RHD void clip_span(int32_t &i0, int32_t &i1, float lo, float hi, float v0, float h2) {
+  if (i1 < i0) return;
+  const float fi0 = (float)i0, fi1 = (float)i1;
+  const float mag = __builtin_fabsf(lo) + __builtin_fabsf(hi) + __builtin_fabsf(v0) + __builtin_fabsf(h2) +
+                    __builtin_fabsf(fi0) + __builtin_fabsf(fi1) + 2.0f;
+  const float margin = 0.015625f + 0.00000095367431640625f * mag;
+  float ta = ((lo + v0) - (kEps + h2)) - margin;  // every i < ta is below lo
+  float tb = ((hi + v0) - (kEps + h2)) + margin;  // every i > tb is above hi
+  ta = ta > fi0 ? ta : fi0;                      // also clamps NaN to the box
+  ta = ta < fi1 + 1.0f ? ta : fi1 + 1.0f;
+  tb = tb < fi1 ? tb : fi1;
+  tb = tb > fi0 - 1.0f ? tb : fi0 - 1.0f;
+  i0 = (int32_t)__builtin_ceilf(ta);
+  i1 = (int32_t)__builtin_floorf(tb);
+}
+
 // The exact acceptance; on success u, v, w are the barycentric weights.
 TOMO_RHD bool bary_exact(float den, float nv, float nw, float &u, float &v, float &w) {
   v = nv / den;
