@@ -1086,13 +1086,22 @@ __device__ __forceinline__ void pair_body(const SweepParams &p) {
     const int32_t first = i0 + wave * 64 + lane;
     int32_t sx = bx0 + first / wy, sy = by0 + first % wy;
     const int32_t dq = kThreads / wy, dr = kThreads % wy;
+    // Each step's counters are loaded one step ahead, so their latency overlaps the pairing.
+    uint32_t c_next = first < i1 ? t.count[sx * G + sy] : 0u;
     for (int32_t base = i0 + wave * 64;; base += kThreads) {
       const int32_t i = base + lane;
+      const int32_t s = sx * G + sy;
+      const uint32_t c = c_next;
+      sx += dq;
+      sy += dr;
+      if (sy > by1) {
+        sy -= wy;
+        sx += 1;
+      }
+      if (i + kThreads < i1) c_next = t.count[sx * G + sy];
       int cls = -1;
       uint32_t e = 0u;
       if (i < i1) {
-        const int32_t s = sx * G + sy;
-        const uint32_t c = t.count[s];
         if (c == 0u) {
           t.bedinfo[s] = BI_POSS;
           if (sv) {
@@ -1152,12 +1161,6 @@ __device__ __forceinline__ void pair_body(const SweepParams &p) {
         else n8 = n - np;
       }
       if (last) break;
-      sx += dq;
-      sy += dr;
-      if (sy > by1) {
-        sy -= wy;
-        sx += 1;
-      }
     }
   }
   } else {
