@@ -58,6 +58,16 @@ def test_icosphere_684(oracle_mod, bed):
     assert (g["flags"] & ~32 == 0).all()
 
 
+@pytest.mark.parametrize("bed_outer,bed_inner", [(1.5, 0.0), (3.5, 1.0), (5.0, 2.0)])
+@pytest.mark.parametrize("bed", [synth.BED_RAFT, synth.BED_BRIM, synth.BED_SKIRT])
+def test_bed_radii(oracle_mod, bed, bed_outer, bed_inner):
+    """Bed radii 1, 3 (the bit-window neighbour search) and 5 (the slot-by-slot search)."""
+    w = synth.icosphere_workload(bed_type=bed)
+    case = C.make(w.vtx, w.tri, w.grid, w.ypr[::37], bed=bed, bed_outer=bed_outer, bed_inner=bed_inner)
+    g, r = assert_parity(oracle_mod, case, f"icosphere bed={bed} outer={bed_outer} inner={bed_inner}")
+    assert (r["Vbed"] > 0).any()
+
+
 def test_icosphere_implicit_ss(oracle_mod):
     w = synth.icosphere_workload(bed_type=synth.BED_RAFT)
     w.explicit_ss = False

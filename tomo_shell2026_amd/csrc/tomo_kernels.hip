@@ -26,8 +26,9 @@ namespace {
 
 constexpr float kNormFactor = 1000.0f;      // g_fNORMALFACTOR (Tomo_types.h:39)
 
-constexpr int kBedLdsSlots = 14 * 1024;  // bed band + halo slot info ((16 + 2r) x (width + 2r))
+constexpr int kBedLdsSlots = 13 * 1024;  // bed band + halo slot info ((16 + 2r) x (width + 2r))
 constexpr int kBedWords = 1024;          // halo source bit words ((16 + 2r) x ceil((width + 2r) / 32))
+constexpr int kBedTabR = 3;              // largest bed radius r served by the bit-window search
 
 // Work-group barrier that also orders the work-group's global-memory slot tables.  A slot
 // table is private to its work-group and all its waves share one CU (and its L1), so the
@@ -1265,7 +1266,9 @@ __global__ __launch_bounds__(kThreads) void bed_kernel(SweepParams p) {
   __shared__ uint16_t nb[kBedLdsSlots];  // bed bookkeeping (bed_info bits) of the band and halo
   // Source bits of the halo rows (one bit per column, filled by ballots), then their OR over
   // each column's half-open window [y - r, y + r): an exact "no source can be near" test.
-  __shared__ uint32_t srcw[kBedWords], rowany[kBedWords];
+  __shared__ uint32_t srcw[kBedWords], rowany[kBedWords], convw[kBedWords];
+  // dtab[dx][m]: the least bed_dist[dx][j] over the window columns j set in m (1e5 if none)
+  __shared__ float dtab[2 * kBedTabR][1 << (2 * kBedTabR)];
   __shared__ int64_t red[kWaves][6];
   __shared__ int32_t fred[kWaves];
   const int tid = threadIdx.x, lane = tid & 63, wave = wave_index();
@@ -1291,6 +1294,17 @@ __global__ __launch_bounds__(kThreads) void bed_kernel(SweepParams p) {
   const int32_t nw = (hw + 31) >> 5;
   const bool lds = (int64_t)nr * hw <= kBedLdsSlots;
   const bool bits = lds && nr * nw <= kBedWords;
+  const bool wtab = bits && br >= 1 && br <= kBedTabR;  // bit-window neighbour search
+  if (wtab) {
+    const int32_t wd = 2 * br, nm = 1 << wd;
+    for (int32_t i = tid; i < wd * nm; i += kThreads) {
+      const int32_t dx = i / nm, m = i - dx * nm;
+      float md = 1e5f;
+      for (int32_t j = 0; j < wd; j++)
+        if ((m >> j) & 1) md = fminf(md, p.bed_dist[dx * wd + j]);
+      dtab[dx][m] = md;
+    }
+  }
   if (lds) {
     // One halo row per wave step, its loads (up to 8 column chunks of 64) issued together.
     constexpr int kChunks = 8;
@@ -1312,10 +1326,15 @@ __global__ __launch_bounds__(kThreads) void bed_kernel(SweepParams p) {
           const int32_t yc = c0 + lane;
           if (yc < hw) nb[r0 * hw + yc] = (uint16_t)v[q];
           const uint64_t sm = __ballot(yc < hw && (v[q] & BI_SRC) != 0u);
+          const uint64_t cm = __ballot(yc < hw && (v[q] & BI_CONV) != 0u);
           if (bits && lane == 0) {
             const int32_t w0 = c0 >> 5;
             srcw[r0 * nw + w0] = (uint32_t)sm;
-            if (w0 + 1 < nw) srcw[r0 * nw + w0 + 1] = (uint32_t)(sm >> 32);
+            convw[r0 * nw + w0] = (uint32_t)cm;
+            if (w0 + 1 < nw) {
+              srcw[r0 * nw + w0 + 1] = (uint32_t)(sm >> 32);
+              convw[r0 * nw + w0 + 1] = (uint32_t)(cm >> 32);
+            }
           }
         }
       }
@@ -1360,6 +1379,26 @@ __global__ __launch_bounds__(kThreads) void bed_kernel(SweepParams p) {
       float md = 1e5f;
       const int32_t xa = max(X - br, 0), xb = min(X + br, G);
       const int32_t ya = max(Y - br, 0), yb = min(Y + br, G);
+      if (wtab) {
+        // The window's source bits per row (columns Y - r .. Y + r - 1, bit j <-> column
+        // Y - r + j), minus the converted sources scanned before (Y, X), looked up in dtab.
+        const int32_t wd = 2 * br;
+        const uint32_t wmask = (1u << wd) - 1u, lowm = (1u << br) - 1u;
+        const int32_t idx = Y - br - hy0;  // bit position of the window's first column
+        const int32_t w0 = idx >> 5, sh = idx & 31;
+        for (int32_t sx = xa; sx < xb; sx++) {
+          const int32_t row = (sx - hx0) * nw;
+          const uint32_t s0 = (w0 >= 0 && w0 < nw) ? srcw[row + w0] : 0u;
+          const uint32_t s1 = (w0 + 1 >= 0 && w0 + 1 < nw) ? srcw[row + w0 + 1] : 0u;
+          const uint32_t c0w = (w0 >= 0 && w0 < nw) ? convw[row + w0] : 0u;
+          const uint32_t c1w = (w0 + 1 >= 0 && w0 + 1 < nw) ? convw[row + w0 + 1] : 0u;
+          uint32_t sb = (uint32_t)((((uint64_t)s1 << 32) | s0) >> sh) & wmask;
+          const uint32_t cb = (uint32_t)((((uint64_t)c1w << 32) | c0w) >> sh) & wmask;
+          if (sx < X) sb &= ~cb;
+          else if (sx == X) sb &= ~(cb & lowm);
+          md = fminf(md, dtab[sx - X + br][sb]);
+        }
+      } else
       for (int32_t sx = xa; sx < xb; sx++) {
         for (int32_t sy = ya; sy < yb; sy++) {
           uint32_t bs = 0u;
