@@ -129,6 +129,17 @@ def test_stacked_planes_cap(oracle_mod):
     assert ((r["flags"] & 8) != 0).any()
 
 
+@pytest.mark.parametrize("size,layers", [(40.0, 40), (70.0, 20)])
+def test_stacked_planes_many_large_slots(oracle_mod, size, layers):
+    """Stacked squares seen face-on: about 1,600 slots of 40 entries and 4,900 slots of 20, all on
+    the large-slot list with entries in the overflow pool, so the pool's per-slot cursors run in
+    LDS (at most 4,096 large slots) and in global memory (more)."""
+    v, f = C.stacked_planes(n_layers=layers, size=size)
+    case = C.make(v, f, 256, C.coarse_ypr(90, 30), bed=synth.BED_RAFT)
+    g, r = assert_parity(oracle_mod, case, f"stacked planes {layers} x {size:.0f} voxels")
+    assert (r["n_slots"] > (4096 if size > 60 else 1000)).any()
+
+
 def test_tiny_triangles_fallback(oracle_mod):
     v, f = C.tiny_triangles()
     assert_parity(oracle_mod, C.make(v, f, 64, C.coarse_ypr(60)), "tiny triangles")

@@ -957,8 +957,11 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6)
 // First appends the rasteriser's spilled entries to their slots (global counters: every tile
 // has written its counters back).  Entries beyond the inline layers were appended to a pool in
 // arrival order; group them per slot (exclusive scan over the large-slot list, then scatter).
+constexpr int kCompactCap = 4096;  // large slots whose overflow cursors fit in LDS
+
 __global__ __launch_bounds__(kThreads) void compact_kernel(SweepParams p) {
   __shared__ uint32_t scan[kWaves];
+  __shared__ uint32_t lcur[kCompactCap];  // overflow cursor of each large-slot list entry
   const int tid = threadIdx.x, lane = tid & 63, wave = wave_index();
   const int32_t b = blockIdx.x;
   OrientHdr &h = p.hdr[b];
@@ -978,6 +981,11 @@ __global__ __launch_bounds__(kThreads) void compact_kernel(SweepParams p) {
   const uint32_t npool = __hip_atomic_load(&h.pool_top, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   const uint32_t nbig = __hip_atomic_load(&h.big_top, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   if (npool == 0u) return;
+  // With at most kCompactCap large slots their cursors live in LDS (many entries of one slot
+  // would otherwise serialise on one memory-side atomic) and cursor[s] holds s's list index.
+  const bool lcnt = nbig <= (uint32_t)kCompactCap;
+  if (lcnt)
+    for (uint32_t i = tid; i < nbig; i += kThreads) lcur[i] = 0u;
   uint32_t running = 0;
   for (uint32_t b0 = 0; b0 < nbig; b0 += kThreads) {
     const uint32_t j = b0 + tid;
@@ -1004,7 +1012,7 @@ __global__ __launch_bounds__(kThreads) void compact_kernel(SweepParams p) {
     }
     if (s >= 0 && ex > 0) {
       t.off[s] = running + wbase + incl - ex;
-      t.cursor[s] = 0u;
+      t.cursor[s] = lcnt ? j : 0u;
     }
     running += total;
     __syncthreads();
@@ -1012,7 +1020,7 @@ __global__ __launch_bounds__(kThreads) void compact_kernel(SweepParams p) {
   const uint32_t np = npool < (uint32_t)t.pool ? npool : (uint32_t)t.pool;
   for (uint32_t j = tid; j < np; j += kThreads) {
     const uint32_t s = t.pool_slot[j];
-    const uint32_t q = atomicAdd(&t.cursor[s], 1u);
+    const uint32_t q = lcnt ? atomicAdd(&lcur[t.cursor[s]], 1u) : atomicAdd(&t.cursor[s], 1u);
     const uint32_t d = t.off[s] + q;
     if (d < (uint32_t)t.pool) {
       t.cmp_key[d] = t.pool_key[j];
