@@ -197,7 +197,8 @@ struct tomo_handle {
   float *d_chunk = nullptr;     // [n_chunks][6] raw AABB of each kChunk triangles
   int32_t n_chunks = 0;
   float *d_soa = nullptr;  // 3 * n_vtx: x, y, z
-  float4 *d_trec = nullptr;  // 5 * n_tri: per-triangle record (culling order)
+  int4 *d_tidx = nullptr;    // n_tri: vertex indices + caller's index (culling order)
+  float4 *d_vaos = nullptr;  // 2 * n_vtx: (x, y, z, nx), (ny, nz, 0, 0)
   float *d_bed_dist = nullptr;
   // orientations
   int32_t n_orient = 0;
@@ -390,7 +391,8 @@ SweepParams base_params(tomo_handle *h) {
   SweepParams p{};
   p.chunk_box = h->d_chunk;
   p.n_chunks = h->n_chunks;
-  p.trec = h->d_trec;
+  p.tidx = h->d_tidx;
+  p.vaos = h->d_vaos;
   p.vx = h->d_soa;
   p.vy = h->d_soa + h->n_vtx;
   p.vz = h->d_soa + 2 * (size_t)h->n_vtx;
@@ -537,7 +539,8 @@ int tomo_set_mesh(tomo_handle *h, int32_t n_tri, const int32_t *tri, int32_t n_v
     if (tri[i] < 0 || tri[i] >= n_vtx) return fail(TOMO_E_INVALID_ARG, "triangle index out of range");
   if (n_tri >= (1 << 30)) return fail(TOMO_E_INVALID_ARG, "too many triangles");
   HIP_TRY(hipStreamSynchronize(h->stream));
-  if (h->d_trec) { HIP_TRY(hipFree(h->d_trec)); h->d_trec = nullptr; }
+  if (h->d_tidx) { HIP_TRY(hipFree(h->d_tidx)); h->d_tidx = nullptr; }
+  if (h->d_vaos) { HIP_TRY(hipFree(h->d_vaos)); h->d_vaos = nullptr; }
   if (h->d_chunk) { HIP_TRY(hipFree(h->d_chunk)); h->d_chunk = nullptr; }
   if (h->d_soa) { HIP_TRY(hipFree(h->d_soa)); h->d_soa = nullptr; }
   h->n_tri = n_tri;
@@ -603,22 +606,23 @@ int tomo_set_mesh(tomo_handle *h, int32_t n_tri, const int32_t *tri, int32_t n_v
         }
     for (int k = 0; k < 6; k++) box[(size_t)c * 6 + k] = b[k];
   }
-  // One 80-byte record per triangle in culling order: raw positions, raw normals, the caller's
-  // index (so a wave reads its 64 triangles with five coalesced loads and no index indirection).
-  std::vector<float> rec((size_t)20 * (n_tri > 0 ? n_tri : 1), 0.0f);
+  // One 16-byte index record per triangle in culling order (its three vertex indices and the
+  // caller's index) and two float4 per vertex (raw position and normal): 32 bytes per vertex plus
+  // 16 per triangle stay resident in an XCD's L2 while every orientation reads them.
+  std::vector<int32_t> tix((size_t)4 * (n_tri > 0 ? n_tri : 1), 0);
   for (int32_t i = 0; i < n_tri; i++) {
-    float *r = &rec[(size_t)i * 20];
-    for (int j = 0; j < 3; j++) {
-      const size_t v = (size_t)ord_tri[(size_t)i * 3 + j];
-      for (int k = 0; k < 3; k++) {
-        r[j * 3 + k] = vtx[v * 3 + k];
-        r[9 + j * 3 + k] = vtx_nrm[v * 3 + k];
-      }
-    }
-    std::memcpy(&r[18], &ord_id[(size_t)i], sizeof(int32_t));
+    for (int j = 0; j < 3; j++) tix[(size_t)i * 4 + j] = ord_tri[(size_t)i * 3 + j];
+    tix[(size_t)i * 4 + 3] = ord_id[(size_t)i];
   }
-  HIP_TRY(hipMalloc(&h->d_trec, rec.size() * sizeof(float)));
-  HIP_TRY(hipMemcpy(h->d_trec, rec.data(), rec.size() * sizeof(float), hipMemcpyHostToDevice));
+  std::vector<float> aos((size_t)8 * (n_vtx > 0 ? n_vtx : 1), 0.0f);
+  for (int32_t v = 0; v < n_vtx; v++) {
+    for (int k = 0; k < 3; k++) aos[(size_t)v * 8 + k] = vtx[(size_t)v * 3 + k];
+    for (int k = 0; k < 3; k++) aos[(size_t)v * 8 + 3 + k] = vtx_nrm[(size_t)v * 3 + k];
+  }
+  HIP_TRY(hipMalloc(&h->d_tidx, tix.size() * sizeof(int32_t)));
+  HIP_TRY(hipMemcpy(h->d_tidx, tix.data(), tix.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIP_TRY(hipMalloc(&h->d_vaos, aos.size() * sizeof(float)));
+  HIP_TRY(hipMemcpy(h->d_vaos, aos.data(), aos.size() * sizeof(float), hipMemcpyHostToDevice));
   HIP_TRY(hipMalloc(&h->d_chunk, sizeof(float) * box.size()));
   HIP_TRY(hipMemcpy(h->d_chunk, box.data(), sizeof(float) * box.size(), hipMemcpyHostToDevice));
   return TOMO_OK;
@@ -683,7 +687,7 @@ int tomo_set_tuning(tomo_handle *h, int32_t knob, int32_t value) {
 int tomo_run(tomo_handle *h, int32_t begin, int32_t end) {
   if (!h) return fail(TOMO_E_INVALID_ARG, "null handle");
   if (begin < 0 || end > h->n_orient || begin > end) return fail(TOMO_E_INVALID_ARG, "bad orientation range");
-  if (!h->d_soa || !h->d_trec) return fail(TOMO_E_STATE, "no mesh");
+  if (!h->d_soa || !h->d_tidx) return fail(TOMO_E_STATE, "no mesh");
   h->last_begin = begin;
   h->last_end = end;
   h->timed = false;
@@ -749,7 +753,7 @@ int tomo_get_info(tomo_handle *h, int32_t *grid, int32_t *batch, int64_t *scratc
 static int debug_run(tomo_handle *h, const float ypr[3], tomo_record *rec, int32_t *cnt_pre,
                      int32_t *cnt_final, int32_t *slot_vo, int32_t *slot_vss, uint64_t *rows) {
   if (!h || !ypr || !rec) return fail(TOMO_E_INVALID_ARG, "null argument");
-  if (!h->d_soa || !h->d_trec) return fail(TOMO_E_STATE, "no mesh");
+  if (!h->d_soa || !h->d_tidx) return fail(TOMO_E_STATE, "no mesh");
   int rc = tomo_set_orientations(h, 1, ypr);
   if (rc) return rc;
   rc = ensure_scratch(h, 1);
@@ -833,7 +837,8 @@ int tomo_destroy(tomo_handle *h) {
   if (h->d_rot) hipFree(h->d_rot);
   if (h->d_chunk) hipFree(h->d_chunk);
   if (h->d_soa) hipFree(h->d_soa);
-  if (h->d_trec) hipFree(h->d_trec);
+  if (h->d_tidx) hipFree(h->d_tidx);
+  if (h->d_vaos) hipFree(h->d_vaos);
   if (h->d_bed_dist) hipFree(h->d_bed_dist);
   if (h->ev0) hipEventDestroy(h->ev0);
   if (h->ev1) hipEventDestroy(h->ev1);

@@ -66,7 +66,8 @@ struct SweepParams {
   // mesh (raw coordinates in voxel units); triangles in culling order (Morton order of centroids)
   const float *chunk_box;   // [n_chunks][6] raw-coordinate AABB (min xyz, max xyz) of kChunk triangles
   const float *vx, *vy, *vz;  // [n_vtx] raw vertex positions (structure of arrays)
-  const float4 *trec;       // [n_tri][5] raw v0, v1, v2 (x, y, z), n0, n1, n2, caller's index bits, 0
+  const int4 *tidx;         // [n_tri] vertex indices i0, i1, i2 and the caller's triangle index
+  const float4 *vaos;       // [n_vtx][2] raw (x, y, z, nx), (ny, nz, 0, 0)
   int32_t n_tri, n_vtx, n_chunks;
   float cx, cy, cz;        // raw AABB centre (STomoSh_Base.cpp:75-82)
   // orientations of this batch: rot/rec already offset to the batch's first orientation

@@ -736,22 +736,24 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6)
       uint32_t my_rank = 0u;
       if (active) {
         float a[3][3], nzv[3];
-        float4 q4[5];
-#pragma unroll
-        for (int q = 0; q < 5; q++) q4[q] = p.trec[(int64_t)tr * 5 + q];
-        const float rw[20] = {q4[0].x, q4[0].y, q4[0].z, q4[0].w, q4[1].x, q4[1].y, q4[1].z, q4[1].w,
-                              q4[2].x, q4[2].y, q4[2].z, q4[2].w, q4[3].x, q4[3].y, q4[3].z, q4[3].w,
-                              q4[4].x, q4[4].y, q4[4].z, q4[4].w};
-        my_rank = 2u * __float_as_uint(rw[18]);
-        // Rotate the three vertices from the raw triangle record (shared by every orientation)
-        // exactly as aabb_kernel rotates them (STomoSh_Base.cpp:85-101), then place them.
+        const int4 ix = p.tidx[tr];
+        my_rank = 2u * (uint32_t)ix.w;
+        const int32_t ids[3] = {ix.x, ix.y, ix.z};
+        // Rotate the three vertices from the raw mesh (shared by every orientation, so it stays
+        // in L2) exactly as aabb_kernel rotates them (STomoSh_Base.cpp:85-101), then place them.
+        float4 va[3], vb[3];
 #pragma unroll
         for (int q = 0; q < 3; q++) {
-          const float t0 = rw[3 * q] - ctr[0], t1 = rw[3 * q + 1] - ctr[1], t2 = rw[3 * q + 2] - ctr[2];
+          va[q] = p.vaos[2 * ids[q]];
+          vb[q] = p.vaos[2 * ids[q] + 1];
+        }
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+          const float t0 = va[q].x - ctr[0], t1 = va[q].y - ctr[1], t2 = va[q].z - ctr[2];
           a[q][0] = dot_row(R0, R1, R2, t0, t1, t2) - kx;
           a[q][1] = dot_row(R3, R4, R5, t0, t1, t2) - ky;
           a[q][2] = dot_row(R6, R7, R8, t0, t1, t2) - kz;
-          nzv[q] = dot_row(R6, R7, R8, rw[9 + 3 * q], rw[10 + 3 * q], rw[11 + 3 * q]);
+          nzv[q] = dot_row(R6, R7, R8, va[q].w, vb[q].x, vb[q].y);
         }
         v0x = a[0][0] + kEps; v0y = a[0][1] + kEps;
         const float v1x = a[1][0] + kEps, v1y = a[1][1] + kEps;
