@@ -598,10 +598,13 @@ __global__ __launch_bounds__(kThreads) void aabb_kernel(SweepParams p) {
 // ---- raster_kernel: one tile of slots of one orientation per work-group -------------------
 // Pixelize + triVoxel (STomoSh_INT3.cpp:285-364, 1138-1209).  The work-group owns the slot
 // counters of a tile (in LDS) and rasterises every triangle whose pixel box meets the tile:
-// its waves cull 64 chunks at a time, dropping chunks whose projected box misses the tile, and
-// take the kept chunks in turn, one triangle per lane.  A lane walks its pixel box (clipped to the
-// tile) with the affine prefilter (tomo_raster.h); surviving pixels are queued in LDS and the
-// exact coverage test, interpolation and slot append run on compacted lanes.  Every pixel is
+// the work-group culls the chunks (one per thread), dropping those whose projected box misses
+// the tile, into an LDS list, and each wave takes listed chunks from a shared counter, one
+// triangle per lane.  A lane walks its pixel box (clipped to the tile and to the columns and rows
+// that can hold an accepted pixel) with the affine prefilter (tomo_raster.h); surviving pixels
+// are queued in LDS and the exact coverage test, interpolation and slot append run on compacted
+// lanes.  The kernel is bound by vector instruction issue; it is held to 80 VGPRs so that two
+// 80 KB work-groups (6 waves per SIMD) stay resident per CU.  Every pixel is
 // tested by exactly one tile; the edge tiles also take the pixels beyond the grid on their side.
 // A triangle's zero-hit fallback (its centroid pixel) is decided by its owner tile, the one
 // holding its clamped box corner (x0, y0), which walks the whole box to count hits.
