@@ -206,8 +206,8 @@ def test_accept_box_holds_every_accepted_pixel(harness):
     harness.harness_accept_box.restype = ct.c_int
     rng = np.random.default_rng(11)
     full = walked = 0
-    for h2 in (np.float32(0.5), np.float32(0.25)):
-        for k, (t, x0, x1, y0, y1) in enumerate(_bary_cases(rng, 2800)):
+    for h2 in (np.float32(0.5), np.float32(0.25), np.float32(0.0)):
+        for k, (t, x0, x1, y0, y1) in enumerate(_bary_cases(rng, 4200)):
             tri6 = np.ascontiguousarray(t.ravel(), np.float32)
             npx = (x1 - x0 + 1) * (y1 - y0 + 1)
             out = np.zeros((npx, 5), np.uint32)
