@@ -610,6 +610,7 @@ __global__ __launch_bounds__(kThreads) void aabb_kernel(SweepParams p) {
 // holding its clamped box corner (x0, y0), which walks the whole box to count hits.
 // Work-groups follow xcd_map so an orientation's tiles share one XCD's L2.
 enum : int { TR_A0 = 0, TR_A1 = 3, TR_A2 = 6, TR_NZ = 9, TR_STRIDE = 13 };
+constexpr int kLaneBox = 48;  // pixel boxes walked lane by lane; larger ones by the whole wave
 constexpr int kCullCap = 896;  // chunk list entries per window (LDS)
 
 struct RasterLds {
@@ -803,9 +804,68 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6)
       L.hits[wave][lane] = 0;
       int32_t x = x0, y = y0;
       bool done = !active || x1 < x0 || y1 < y0;
+      // Boxes of more than kLaneBox pixels are left out of the lane-by-lane walk and walked
+      // afterwards by the whole wave, 64 pixels per step, so that one large box does not hold
+      // the wave for its whole length.
+      const bool big = !done && (int64_t)(x1 - x0 + 1) * (int64_t)(y1 - y0 + 1) > kLaneBox;
+      done = done || big;
+      uint64_t bigm = __ballot(big);
       float e2x = centre_offset(x, p.h2, v0x);
       float rax = e2x * sc.ax, rbx = e2x * sc.bx;
       uint32_t qn = 0;
+      // Exact test, interpolation and slot append of the first min(qn, 64) queued candidates.
+      auto flush = [&]() {
+        wave_sync();
+        const uint32_t np = qn < 64u ? qn : 64u;
+        const uint32_t meta = (uint32_t)lane < np ? L.qmeta[wave][lane] : 0u;
+        const int32_t owner_lane = (int32_t)(meta & 63u);
+        const uint32_t rank = __shfl(my_rank, owner_lane);
+        if ((uint32_t)lane < np) {
+          const int32_t px_i = (int32_t)((meta >> 6) & 8191u) - 256;
+          const int32_t py_i = (int32_t)(meta >> 19) - 256;
+          const float *rec = tri[owner_lane];
+          // margined vertices and the exact test's numerators, formed as triVoxel and
+          // _getBaryCoord form them (the same operations as the setup above)
+          float m[3][3];
+#pragma unroll
+          for (int q = 0; q < 3; q++) {
+            m[0][q] = rec[TR_A0 + q] + kEps;
+            m[1][q] = rec[TR_A1 + q] + kEps;
+            m[2][q] = rec[TR_A2 + q] + kEps;
+          }
+          const TriSetup ts = tri_setup(m[0][0], m[0][1], m[1][0], m[1][1], m[2][0], m[2][1]);
+          float nv, nw, u, v, w;
+          bary_num(ts, ((float)px_i + kEps) + p.h2, ((float)py_i + kEps) + p.h2, nv, nw);
+          if (bary_exact(ts.den, nv, nw, u, v, w)) {
+            L.hits[wave][owner_lane] = 1;
+            if (px_i >= exlo && px_i <= exhi && py_i >= eylo && py_i <= eyhi) {
+              // _bary_product (Tomo_types.cpp:75-84) of the margined vertices and the normals
+              const float px = m[0][0] * u + m[1][0] * v + m[2][0] * w;
+              const float py = m[0][1] * u + m[1][1] * v + m[2][1] * w;
+              const float pz = m[0][2] * u + m[1][2] * v + m[2][2] * w;
+              const float nz = rec[TR_NZ + 0] * u + rec[TR_NZ + 1] * v + rec[TR_NZ + 2] * w;
+              int32_t s;
+              if (p.do_shell) {  // with its shell sub-pixel (STomoSh_INT3.cpp:1179-1185)
+                const float pz2 = pz + ((nz >= 0.0f) ? -p.shell_off : p.shell_off);
+                s = em.put2(px, py, pz, nz, pz2, -nz, rank);
+              } else {
+                s = em.put(px, py, pz, nz, rank);
+              }
+              if (s >= 0 && s != px_i * G + py_i) em.flags |= F_RANK_TIE;
+            }
+          }
+        }
+        wave_sync();
+        if (qn > 64u) {
+          const uint32_t rest = qn - 64u;
+          uint32_t mm = 0u;
+          if ((uint32_t)lane < rest) mm = L.qmeta[wave][64 + lane];
+          wave_sync();
+          if ((uint32_t)lane < rest) L.qmeta[wave][lane] = mm;
+          wave_sync();
+        }
+        qn -= np;
+      };
       wave_sync();
       // Column-major walk of the box, written without divergent branches: every lane advances
       // every step (a finished lane's position runs on past x1 and is never tested).
@@ -825,60 +885,48 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6)
         if (cand) L.qmeta[wave][qn + lanes_below(m)] = meta;
         qn += (uint32_t)__popcll(m);
         const bool all_done = __ballot(!done) == 0ull;
-        if (qn >= 64u || (all_done && qn > 0u)) {
-          wave_sync();
-          const uint32_t np = qn < 64u ? qn : 64u;
-          const uint32_t meta = (uint32_t)lane < np ? L.qmeta[wave][lane] : 0u;
-          const int32_t owner_lane = (int32_t)(meta & 63u);
-          const uint32_t rank = __shfl(my_rank, owner_lane);
-          if ((uint32_t)lane < np) {
-            const int32_t px_i = (int32_t)((meta >> 6) & 8191u) - 256;
-            const int32_t py_i = (int32_t)(meta >> 19) - 256;
-            const float *rec = tri[owner_lane];
-            // margined vertices and the exact test's numerators, formed as triVoxel and
-            // _getBaryCoord form them (the same operations as the setup above)
-            float m[3][3];
-#pragma unroll
-            for (int q = 0; q < 3; q++) {
-              m[0][q] = rec[TR_A0 + q] + kEps;
-              m[1][q] = rec[TR_A1 + q] + kEps;
-              m[2][q] = rec[TR_A2 + q] + kEps;
-            }
-            const TriSetup ts = tri_setup(m[0][0], m[0][1], m[1][0], m[1][1], m[2][0], m[2][1]);
-            float nv, nw, u, v, w;
-            bary_num(ts, ((float)px_i + kEps) + p.h2, ((float)py_i + kEps) + p.h2, nv, nw);
-            if (bary_exact(ts.den, nv, nw, u, v, w)) {
-              L.hits[wave][owner_lane] = 1;
-              if (px_i >= exlo && px_i <= exhi && py_i >= eylo && py_i <= eyhi) {
-                // _bary_product (Tomo_types.cpp:75-84) of the margined vertices and the normals
-                const float px = m[0][0] * u + m[1][0] * v + m[2][0] * w;
-                const float py = m[0][1] * u + m[1][1] * v + m[2][1] * w;
-                const float pz = m[0][2] * u + m[1][2] * v + m[2][2] * w;
-                const float nz = rec[TR_NZ + 0] * u + rec[TR_NZ + 1] * v + rec[TR_NZ + 2] * w;
-                int32_t s;
-                if (p.do_shell) {  // with its shell sub-pixel (STomoSh_INT3.cpp:1179-1185)
-                  const float pz2 = pz + ((nz >= 0.0f) ? -p.shell_off : p.shell_off);
-                  s = em.put2(px, py, pz, nz, pz2, -nz, rank);
-                } else {
-                  s = em.put(px, py, pz, nz, rank);
-                }
-                if (s >= 0 && s != px_i * G + py_i) em.flags |= F_RANK_TIE;
-              }
-            }
-          }
-          wave_sync();
-          if (qn > 64u) {
-            const uint32_t rest = qn - 64u;
-            uint32_t mm = 0u;
-            if ((uint32_t)lane < rest) mm = L.qmeta[wave][64 + lane];
-            wave_sync();
-            if ((uint32_t)lane < rest) L.qmeta[wave][lane] = mm;
-            wave_sync();
-          }
-          qn -= np;
-        }
-        if (all_done && qn == 0u) break;
+        if (qn >= 64u) flush();
+        if (all_done) break;
       }
+      // The large boxes, one at a time: their triangle's walk state is read from its lane into
+      // wave-uniform values and the wave takes 64 consecutive pixels (column-major) per step,
+      // with the same prefilter arithmetic as the lane-by-lane walk.
+      while (bigm) {
+        const int j = __ffsll((unsigned long long)bigm) - 1;
+        bigm &= bigm - 1ull;
+        const int32_t jx0 = __builtin_amdgcn_readlane(x0, j), jx1 = __builtin_amdgcn_readlane(x1, j);
+        const int32_t jy0 = __builtin_amdgcn_readlane(y0, j), jy1 = __builtin_amdgcn_readlane(y1, j);
+        const float jv0x = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v0x), j));
+        const float jv0y = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v0y), j));
+        TriScan js;
+        js.ax = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sc.ax), j));
+        js.ay = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sc.ay), j));
+        js.bx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sc.bx), j));
+        js.by = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sc.by), j));
+        js.lo = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sc.lo), j));
+        js.hia = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sc.hia), j));
+        js.his = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sc.his), j));
+        const int32_t hgt = jy1 - jy0 + 1, npx = (jx1 - jx0 + 1) * hgt;
+        const int32_t dq = 64 / hgt, dr = 64 - dq * hgt;
+        const int32_t col = lane / hgt;
+        int32_t wx = jx0 + col, wy = jy0 + (lane - col * hgt);
+        for (int32_t k0 = 0; k0 < npx; k0 += 64) {
+          const float e2 = centre_offset(wx, p.h2, jv0x);
+          const bool cand = (k0 + lane < npx) & scan_candidate(js, e2 * js.ax, e2 * js.bx, centre_offset(wy, p.h2, jv0y));
+          const uint64_t m = __ballot(cand);
+          if (cand)
+            L.qmeta[wave][qn + lanes_below(m)] = (uint32_t)j | ((uint32_t)(wx + 256) << 6) | ((uint32_t)(wy + 256) << 19);
+          qn += (uint32_t)__popcll(m);
+          if (qn >= 64u) flush();
+          wy += dr;
+          wx += dq;
+          if (wy > jy1) {
+            wy -= hgt;
+            wx += 1;
+          }
+        }
+      }
+      while (qn > 0u) flush();
       wave_sync();
       if (active && owner && L.hits[wave][lane] == 0) {
         // very small triangle: one pixel at the un-margined centroid (STomoSh_INT3.cpp:1190-1206)
