@@ -1318,7 +1318,11 @@ __device__ __forceinline__ void pair_body(const SweepParams &p) {
 }
 
 __global__ __launch_bounds__(kThreads, 5) void pair_kernel(SweepParams p) { pair_body<0>(p); }
-__global__ __launch_bounds__(kThreads) void pair_big_kernel(SweepParams p) { pair_body<1>(p); }
+// Four waves per SIMD (128 VGPRs, a few spilled) beat three without spills: the kernel waits on
+// memory far more than it computes.
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void pair_big_kernel(SweepParams p) {
+  pair_body<1>(p);
+}
 
 // ---- bed_kernel: kBandRows grid rows of the bed scan region per work-group -----------------
 // GenerateBed / IsBedCandidate (STomoSh_INT3.cpp:1331-1443) with the raster-order effect: a raft
