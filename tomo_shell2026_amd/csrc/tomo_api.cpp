@@ -155,7 +155,7 @@ ScratchLayout make_layout(int32_t G, int32_t inl) {
   };
   L.count = take(4 * G2);
   L.ent = take(8 * G2 * inl);
-  L.bedinfo = take(4 * G2);
+  L.bedinfo = take(2 * G2);
   L.big = take(4 * G2);
   L.off = take(4 * G2);
   L.cursor = take(4 * G2);

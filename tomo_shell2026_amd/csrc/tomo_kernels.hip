@@ -71,7 +71,8 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
 
 struct Tables {
   uint2 *ent;  // [inl][G2] {key, rank}, layer-major
-  uint32_t *count, *bedinfo, *off, *cursor, *pool_slot, *pool_key, *pool_rank,
+  uint16_t *bedinfo;  // [G2] bed bookkeeping (bed_info bits, 11 used)
+  uint32_t *count, *off, *cursor, *pool_slot, *pool_key, *pool_rank,
       *cmp_key, *cmp_rank;
   int32_t *big;
   int32_t *sv;  // [6][G2] per-slot Vo, Vss, Va, Vb, Vnv, Vtc (kept only when totals may round)
@@ -482,7 +483,7 @@ __device__ __forceinline__ Tables tables_for(const SweepParams &p, int32_t b) {
   Tables t;
   t.count = (uint32_t *)(base + p.lay.count);
   t.ent = (uint2 *)(base + p.lay.ent);
-  t.bedinfo = (uint32_t *)(base + p.lay.bedinfo);
+  t.bedinfo = (uint16_t *)(base + p.lay.bedinfo);
   t.big = (int32_t *)(base + p.lay.big);
   t.off = (uint32_t *)(base + p.lay.off);
   t.cursor = (uint32_t *)(base + p.lay.cursor);
@@ -1128,7 +1129,7 @@ __device__ __forceinline__ void pair_body(const SweepParams &p) {
     SlotResult r;
     pair_keys<N>(k, (int)c, p.pp, r, dbg_rows ? dbg_rows + (int64_t)s * 64 : nullptr);
     account(acc, r, s, p);
-    t.bedinfo[s] = bed_info(r, p.pp);
+    t.bedinfo[s] = (uint16_t)bed_info(r, p.pp);
     if (sv) put_sv(sv, G2, s, r);
     if (final_here) {
       acc.nfinal += r.n;
@@ -1165,7 +1166,7 @@ __device__ __forceinline__ void pair_body(const SweepParams &p) {
       uint32_t e = 0u;
       if (i < i1) {
         if (c == 0u) {
-          t.bedinfo[s] = BI_POSS;
+          t.bedinfo[s] = (uint16_t)BI_POSS;
           if (sv) {
             const SlotResult z = {};
             put_sv(sv, G2, s, z);
@@ -1241,7 +1242,7 @@ __device__ __forceinline__ void pair_body(const SweepParams &p) {
     pair_keys<16>(k, (int)c, p.pp, r, p.dbg_rows ? p.dbg_rows + (int64_t)s * 64 : nullptr);
     acc.nslot++;
     account(acc, r, s, p);
-    t.bedinfo[s] = bed_info(r, p.pp);
+    t.bedinfo[s] = (uint16_t)bed_info(r, p.pp);
     if (sv) put_sv(sv, G2, s, r);
     if (final_here) {
       acc.nfinal += r.n;
@@ -1254,7 +1255,7 @@ __device__ __forceinline__ void pair_body(const SweepParams &p) {
   auto finish_slot = [&](int32_t s, const SlotResult &r) {
     acc.nslot++;
     account(acc, r, s, p);
-    t.bedinfo[s] = bed_info(r, p.pp);
+    t.bedinfo[s] = (uint16_t)bed_info(r, p.pp);
     if (sv) put_sv(sv, G2, s, r);
     if (final_here) {
       acc.nfinal += r.n;
