@@ -232,7 +232,7 @@ struct tomo_handle {
 namespace {
 
 void free_stage_events(tomo_handle *h) {
-  for (hipEvent_t e : h->stage_ev) hipEventDestroy(e);
+  for (hipEvent_t e : h->stage_ev) (void)hipEventDestroy(e);
   h->stage_ev.clear();
   h->stage_of.clear();
 }
@@ -269,7 +269,7 @@ hipError_t scratch_alloc(size_t bytes, int device, uint8_t **out, size_t *cap) {
     }
     if (!old) return e;
     (void)hipGetLastError();
-    hipFree(old);
+    (void)hipFree(old);
     e = hipMalloc(&p, bytes);
     if (e != hipSuccess) return e;
   }
@@ -294,7 +294,7 @@ void scratch_release(uint8_t *p, size_t cap, int device) {
     g_scratch_cache.bytes = cap;
     g_scratch_cache.device = device;
   }
-  if (old) hipFree(old);
+  if (old) (void)hipFree(old);
 }
 
 int ensure_scratch(tomo_handle *h, int32_t batch) {
@@ -773,8 +773,8 @@ static int debug_run(tomo_handle *h, const float ypr[3], tomo_record *rec, int32
     rc = enqueue(h, 0, 1, 1, d, g2, drows);
     if (rc) {
       h->phases = was;
-      hipFree(d);
-      if (drows) hipFree(drows);
+      (void)hipFree(d);
+      if (drows) (void)hipFree(drows);
       return rc;
     }
     e = hipStreamSynchronize(h->stream);
@@ -784,8 +784,8 @@ static int debug_run(tomo_handle *h, const float ypr[3], tomo_record *rec, int32
   for (int k = 0; k < 4 && e == hipSuccess; k++)
     if (outs[k]) e = hipMemcpy(outs[k], d + k * g2, sizeof(int32_t) * g2, hipMemcpyDeviceToHost);
   if (e == hipSuccess && rows) e = hipMemcpy(rows, drows, sizeof(uint64_t) * 64 * g2, hipMemcpyDeviceToHost);
-  hipFree(d);
-  if (drows) hipFree(drows);
+  (void)hipFree(d);
+  if (drows) (void)hipFree(drows);
   if (e != hipSuccess) return fail(TOMO_E_HIP, std::string("debug sweep: ") + hipGetErrorString(e));
   h->last_begin = 0;
   h->last_end = 1;
@@ -824,25 +824,25 @@ int tomo_phase_profile(tomo_handle *h, double *ms8) {
 
 int tomo_destroy(tomo_handle *h) {
   if (!h) return TOMO_OK;
-  if (h->stream) hipStreamSynchronize(h->stream);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
   free_stage_events(h);
   if (h->shared) {  // buffers and stream belong to the batch
     h->d_hdr = nullptr;
     h->d_scratch = nullptr;
     h->stream = h->own_stream;
   }
-  if (h->d_hdr) hipFree(h->d_hdr);
+  if (h->d_hdr) (void)hipFree(h->d_hdr);
   if (h->d_scratch) scratch_release(h->d_scratch, h->scratch_cap, h->device);
-  if (h->d_rec) hipFree(h->d_rec);
-  if (h->d_rot) hipFree(h->d_rot);
-  if (h->d_chunk) hipFree(h->d_chunk);
-  if (h->d_soa) hipFree(h->d_soa);
-  if (h->d_tidx) hipFree(h->d_tidx);
-  if (h->d_vaos) hipFree(h->d_vaos);
-  if (h->d_bed_dist) hipFree(h->d_bed_dist);
-  if (h->ev0) hipEventDestroy(h->ev0);
-  if (h->ev1) hipEventDestroy(h->ev1);
-  if (h->stream) hipStreamDestroy(h->stream);
+  if (h->d_rec) (void)hipFree(h->d_rec);
+  if (h->d_rot) (void)hipFree(h->d_rot);
+  if (h->d_chunk) (void)hipFree(h->d_chunk);
+  if (h->d_soa) (void)hipFree(h->d_soa);
+  if (h->d_tidx) (void)hipFree(h->d_tidx);
+  if (h->d_vaos) (void)hipFree(h->d_vaos);
+  if (h->d_bed_dist) (void)hipFree(h->d_bed_dist);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return TOMO_OK;
 }
@@ -875,21 +875,21 @@ struct tomo_batch {
 };
 
 static void batch_drop_graph(tomo_batch *b) {
-  if (b->exec) { hipGraphExecDestroy(b->exec); b->exec = nullptr; }
-  if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
+  if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+  if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
 }
 
 static void release_private_scratch(tomo_handle *h) {
-  hipStreamSynchronize(h->stream);
+  (void)hipStreamSynchronize(h->stream);
   if (h->d_scratch) { scratch_release(h->d_scratch, h->scratch_cap, h->device); h->d_scratch = nullptr; }
-  if (h->d_hdr) { hipFree(h->d_hdr); h->d_hdr = nullptr; }
+  if (h->d_hdr) { (void)hipFree(h->d_hdr); h->d_hdr = nullptr; }
   h->scratch_batch = 0;
   h->scratch_region = 0;
 }
 
 // Detach members from the shared buffers (before re-tuning or destruction).
 static void batch_unshare(tomo_batch *b) {
-  if (b->stream) hipStreamSynchronize(b->stream);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
   batch_drop_graph(b);
   for (tomo_handle *h : b->members)
     if (h->shared) {
@@ -901,7 +901,7 @@ static void batch_unshare(tomo_batch *b) {
       h->scratch_region = 0;
     }
   if (b->d_scratch) { scratch_release(b->d_scratch, b->scratch_cap, b->device); b->d_scratch = nullptr; }
-  if (b->d_hdr) { hipFree(b->d_hdr); b->d_hdr = nullptr; }
+  if (b->d_hdr) { (void)hipFree(b->d_hdr); b->d_hdr = nullptr; }
   b->scratch_bytes = b->hdr_count = 0;
 }
 
@@ -1017,7 +1017,7 @@ int tomo_batch_tune(tomo_batch *b, int32_t sample) {
       consider(t);
     }
     size_t freeb = 0, total = 0;
-    hipMemGetInfo(&freeb, &total);
+    (void)hipMemGetInfo(&freeb, &total);
     for (int32_t bs : {1024, 4096, 8192}) {
       tomo_batch::Tune t = best;
       t.batch = bs < ns ? bs : ns;
@@ -1103,7 +1103,7 @@ int tomo_batch_capture(tomo_batch *b) {
   hipGraph_t g = nullptr;
   const hipError_t ec = hipStreamEndCapture(b->stream, &g);
   if (rc) {
-    if (g) hipGraphDestroy(g);
+    if (g) (void)hipGraphDestroy(g);
     return rc;
   }
   if (ec != hipSuccess) return fail(TOMO_E_HIP, std::string("graph capture: ") + hipGetErrorString(ec));
@@ -1157,9 +1157,9 @@ int tomo_batch_destroy(tomo_batch *b) {
   if (!b) return TOMO_OK;
   batch_unshare(b);
   for (tomo_handle *h : b->members) tomo_destroy(h);
-  if (b->ev0) hipEventDestroy(b->ev0);
-  if (b->ev1) hipEventDestroy(b->ev1);
-  if (b->stream) hipStreamDestroy(b->stream);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return TOMO_OK;
 }
@@ -1325,7 +1325,7 @@ int tomo_cvss_create(const float *f32, int32_t n_tri, const int32_t *tri, int32_
     return fail(TOMO_E_OOM, "device mesh allocation failed");
   }
   if (hipMemcpy(c->d, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice) != hipSuccess) {
-    hipFree(c->d);
+    (void)hipFree(c->d);
     delete c;
     return fail(TOMO_E_HIP, "device mesh upload failed");
   }
@@ -1363,7 +1363,7 @@ int tomo_cvss_backward(tomo_cvss *c, int32_t n, const float *ypr, double *aux, c
 
 int tomo_cvss_destroy(tomo_cvss *c) {
   if (!c) return TOMO_OK;
-  if (c->d) hipFree(c->d);
+  if (c->d) (void)hipFree(c->d);
   delete c;
   return TOMO_OK;
 }

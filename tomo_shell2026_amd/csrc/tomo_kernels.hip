@@ -30,12 +30,6 @@ constexpr int kBedLdsSlots = 13 * 1024;  // bed band + halo slot info ((16 + 2r)
 constexpr int kBedWords = 1024;          // halo source bit words ((16 + 2r) x ceil((width + 2r) / 32))
 constexpr int kBedTabR = 3;              // largest bed radius r served by the bit-window search
 
-// Work-group barrier that also orders the work-group's global-memory slot tables.  A slot
-// table is private to its work-group and all its waves share one CU (and its L1), so the
-// workgroup-scope release/acquire of __syncthreads suffices; an agent-scope fence would write
-// back and invalidate the whole L2 of the XCD.
-__device__ __forceinline__ void wg_sync_global() { __syncthreads(); }
-
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -527,7 +521,7 @@ static_assert(offsetof(DevRecord, Vbed) == 6 * 8 && offsetof(DevRecord, n_final)
                   offsetof(DevRecord, chk_final) == 11 * 8 && offsetof(DevRecord, aVtc) == 17 * 8,
               "DevRecord totals must be consecutive 64-bit fields");
 
-__device__ __forceinline__ int32_t group_or(int32_t f, int32_t *red, int lane, int wave, int tid) {
+__device__ __forceinline__ int32_t group_or(int32_t f, int32_t *red, int lane, int wave) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) f |= __shfl_xor(f, off);
   if (lane == 0) red[wave] = f;
@@ -1314,7 +1308,7 @@ __device__ __forceinline__ void pair_body(const SweepParams &p) {
                    acc.nfinal, (int64_t)acc.chkpre, (int64_t)acc.chkfinal, acc.aVo, acc.aVss,
                    acc.aVa, acc.aVb, acc.aVnv, acc.aVtc};
   group_add<18>(v, red, lane, wave, tid, (unsigned long long *)&R.Va);
-  const int32_t f = group_or(acc.flags, fred, lane, wave, tid);
+  const int32_t f = group_or(acc.flags, fred, lane, wave);
   if (tid == 0 && f) atomicOr(&R.flags, f);
 }
 
@@ -1508,7 +1502,7 @@ __global__ __launch_bounds__(kThreads) void bed_kernel(SweepParams p) {
   // Vbed, n_entries, n_slots, n_final, chk_pre, chk_final
   int64_t v[6] = {nbed, 0, 0, nfinal, 0, (int64_t)chk};
   group_add<6>(v, red, lane, wave, tid, (unsigned long long *)&R.Vbed);
-  const int32_t f = group_or(flags, fred, lane, wave, tid);
+  const int32_t f = group_or(flags, fred, lane, wave);
   if (tid == 0 && f) atomicOr(&R.flags, f);
 }
 
