@@ -68,6 +68,18 @@ def test_bed_radii(oracle_mod, bed, bed_outer, bed_inner):
     assert (r["Vbed"] > 0).any()
 
 
+@pytest.mark.parametrize("bed", [synth.BED_RAFT, synth.BED_BRIM])
+def test_grid_1024(oracle_mod, bed):
+    """A 950-voxel sphere on a 1024 x 1024 grid: 64 rasteriser tiles of 256 x 64 per orientation,
+    bed bands too wide for the LDS staging (the bed kernel reads its bookkeeping from global
+    memory and searches slot by slot), and float totals above 2^24 (the ordered replay)."""
+    v, f = synth.icosphere(5, 50.0)
+    v, _ = synth.fit_to_grid(v * 9.5, 1024)
+    case = C.make(v, f, 1024, synth.ypr_grid(np.arange(0, 360, 120), np.arange(0, 91, 45)), bed=bed)
+    g, r = assert_parity(oracle_mod, case, f"sphere on a 1024 grid bed={bed}")
+    assert (r["n_slots"] > 700000).all() and (r["Vbed"] > 0).all()
+
+
 def test_icosphere_implicit_ss(oracle_mod):
     w = synth.icosphere_workload(bed_type=synth.BED_RAFT)
     w.explicit_ss = False
