@@ -21,7 +21,6 @@ constexpr int kTileWaves = kTileThreads / 64;
 constexpr int kTileSlots = 16384;      // rasteriser tile: slots with a 16-bit LDS counter each
 constexpr int kTileYDefault = 128;     // tile extent along y (power of two); x extent = kTileSlots / y
 constexpr int kChunk = 64;             // triangles per culling chunk (one per lane)
-constexpr int kPairTile = 2048;        // pairing: touched-box slots per work-group
 constexpr int kBandRows = 16;          // bed: grid rows per work-group
 
 // Device record, one per orientation.  Totals are exact 64-bit integers accumulated with
@@ -78,6 +77,7 @@ struct SweepParams {
   int32_t tile_x;          // tile extent along x (slots)
   int32_t tile_y_shift;    // log2 of the tile extent along y
   int32_t pair_tiles;      // pairing work-groups per orientation (touched box)
+  int32_t pair_span;       // touched-box slots per pairing work-group
   int32_t big_groups;      // pairing work-groups per orientation (large-slot list)
   int32_t bed_bands;       // bed work-groups per orientation
   // options

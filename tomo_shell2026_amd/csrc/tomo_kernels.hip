@@ -1077,7 +1077,7 @@ __global__ __launch_bounds__(kThreads) void compact_kernel(SweepParams p) {
   }
 }
 
-// ---- pair_kernel: kPairTile slots of the touched box per work-group ------------------------
+// ---- pair_kernel: pair_span slots of the touched box per work-group -----------------------
 // Pairing (STomoSh_INT3.cpp:366-395) and the per-slot sums of Calculate (:1211-1255).
 // pair_kernel (PART 0) pairs the slots of at most kSmall entries found in the work-group's part
 // of the touched box; pair_big_kernel (PART 1, a second launch with its own register budget)
@@ -1099,7 +1099,7 @@ __device__ __forceinline__ void pair_body(const SweepParams &p) {
   const bool any = bx1 >= bx0;
   const int32_t wy = by1 - by0 + 1;
   const int32_t nbb = any ? (bx1 - bx0 + 1) * wy : 0;
-  const int32_t i0 = tile * kPairTile;
+  const int32_t i0 = tile * p.pair_span;
   const int32_t nwaves = groups * kWaves;
   if (PART == 0 && tile == 0 && tid == 0) {
     // rasteriser flags, and pool or spill list overflow
@@ -1137,7 +1137,7 @@ __device__ __forceinline__ void pair_body(const SweepParams &p) {
   if constexpr (PART == 0) {
   uint32_t n2 = 0u, n4 = 0u, n8 = 0u;  // queue fills (wave-uniform)
   uint32_t *const q2 = pairq[0][wave], *const q4 = pairq[1][wave], *const q8 = pairq[2][wave];
-  const int32_t i1 = min(i0 + kPairTile, nbb);
+  const int32_t i1 = min(i0 + p.pair_span, nbb);
   if (i0 + wave * 64 < i1) {
     // slot i of the box is (bx0 + i / wy, by0 + i % wy); step kThreads slots at a time
     const int32_t first = i0 + wave * 64 + lane;
