@@ -430,7 +430,10 @@ SweepParams base_params(tomo_handle *h) {
   p.pair_span = h->opt.grid <= 256 ? 4096 : 1024;
   p.pair_tiles = (g2 + p.pair_span - 1) / p.pair_span;
   p.big_groups = p.pair_tiles >= 8 ? p.pair_tiles / 8 : 1;
-  p.bed_bands = (h->opt.grid + kBandRows - 1) / kBandRows;
+  // Bed bands of 32 rows while a band and its halo fit the bed kernel's LDS staging (grids up to
+  // 256: 9% faster on the 69k-triangle sphere than 16), 16 rows above.
+  p.band_rows = h->opt.grid <= 256 ? 32 : 16;
+  p.bed_bands = (h->opt.grid + p.band_rows - 1) / p.band_rows;
   p.exact_fsum = h->exact_fsum;
   return p;
 }

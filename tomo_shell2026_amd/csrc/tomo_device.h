@@ -21,7 +21,6 @@ constexpr int kTileWaves = kTileThreads / 64;
 constexpr int kTileSlots = 16384;      // rasteriser tile: slots with a 16-bit LDS counter each
 constexpr int kTileYDefault = 128;     // tile extent along y (power of two); x extent = kTileSlots / y
 constexpr int kChunk = 64;             // triangles per culling chunk (one per lane)
-constexpr int kBandRows = 16;          // bed: grid rows per work-group
 
 // Device record, one per orientation.  Totals are exact 64-bit integers accumulated with
 // atomics by the pairing and bed kernels; the host turns them into the reference's float
@@ -80,6 +79,7 @@ struct SweepParams {
   int32_t pair_span;       // touched-box slots per pairing work-group
   int32_t big_groups;      // pairing work-groups per orientation (large-slot list)
   int32_t bed_bands;       // bed work-groups per orientation
+  int32_t band_rows;       // bed: grid rows per work-group
   // options
   int32_t grid;
   float bed_out, bed_in, h2, shell_off;

@@ -26,8 +26,8 @@ namespace {
 
 constexpr float kNormFactor = 1000.0f;      // g_fNORMALFACTOR (Tomo_types.h:39)
 
-constexpr int kBedLdsSlots = 13 * 1024;  // bed band + halo slot info ((16 + 2r) x (width + 2r))
-constexpr int kBedWords = 1024;          // halo source bit words ((16 + 2r) x ceil((width + 2r) / 32))
+constexpr int kBedLdsSlots = 13 * 1024;  // bed band + halo slot info ((rows + 2r) x (width + 2r))
+constexpr int kBedWords = 1024;          // halo source bit words ((rows + 2r) x ceil((width + 2r) / 32))
 constexpr int kBedTabR = 3;              // largest bed radius r served by the bit-window search
 
 __device__ __forceinline__ void wave_sync() {
@@ -1319,7 +1319,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
   pair_body<1>(p);
 }
 
-// ---- bed_kernel: kBandRows grid rows of the bed scan region per work-group -----------------
+// ---- bed_kernel: band_rows grid rows of the bed scan region per work-group ----------------
 // GenerateBed / IsBedCandidate (STomoSh_INT3.cpp:1331-1443) with the raster-order effect: a raft
 // source slot that takes its own bed row stops being a source for slots scanned after it.
 __global__ __launch_bounds__(kThreads) void bed_kernel(SweepParams p) {
@@ -1342,9 +1342,9 @@ __global__ __launch_bounds__(kThreads) void bed_kernel(SweepParams p) {
   const int32_t m = br + 1;
   const int32_t rx0 = max(bx0 - m, 0), rx1 = min(bx1 + m + 1, G);
   const int32_t ry0 = max(by0 - m, 0), ry1 = min(by1 + m + 1, G);
-  const int32_t X0 = rx0 + band * kBandRows;
+  const int32_t X0 = rx0 + band * p.band_rows;
   if (X0 >= rx1) return;
-  const int32_t X1 = min(X0 + kBandRows, rx1);
+  const int32_t X1 = min(X0 + p.band_rows, rx1);
   const Tables t = tables_for(p, b);
   // neighbour flags of rows [X0 - r, X1 + r) x columns [ry0 - r, ry1 + r), clipped to the grid
   const int32_t hx0 = max(X0 - br, 0), hx1 = min(X1 + br, G);
