@@ -430,9 +430,10 @@ SweepParams base_params(tomo_handle *h) {
   p.pair_span = h->opt.grid <= 256 ? 4096 : 1024;
   p.pair_tiles = (g2 + p.pair_span - 1) / p.pair_span;
   p.big_groups = p.pair_tiles >= 8 ? p.pair_tiles / 8 : 1;
-  // Bed bands of 32 rows while a band and its halo fit the bed kernel's LDS staging (grids up to
-  // 256: 9% faster on the 69k-triangle sphere than 16), 16 rows above.
-  p.band_rows = h->opt.grid <= 256 ? 32 : 16;
+  // Bed bands of 40 rows up to a 256 grid, where a band and its halo fit the bed kernel's LDS
+  // staging for bed radii up to 6 (on the 69k-triangle sphere 15% faster than 16 rows; 32 and
+  // 48 were slower), 16 rows above.
+  p.band_rows = h->opt.grid <= 256 ? 40 : 16;
   p.bed_bands = (h->opt.grid + p.band_rows - 1) / p.band_rows;
   p.exact_fsum = h->exact_fsum;
   return p;
