@@ -1086,6 +1086,7 @@ template <int PART>
 __device__ __forceinline__ void pair_body(const SweepParams &p) {
   __shared__ uint64_t sortbuf[kWaves][kBigCap];
   __shared__ uint32_t pairq[3][kWaves][128];  // queued slots: s | entries << 24
+  __shared__ uint32_t pairk[2][kWaves][128];  // keys of the queued slots of at most 2 entries
   __shared__ int64_t red[kWaves][18];
   __shared__ int32_t fred[kWaves];
   const int tid = threadIdx.x, lane = tid & 63, wave = wave_index();
@@ -1112,14 +1113,20 @@ __device__ __forceinline__ void pair_body(const SweepParams &p) {
   int32_t *const sv = (p.exact_fsum && totals_may_round(h, p.pp)) ? t.sv : nullptr;
   Acc acc = {};
   uint64_t *const dbg_rows = p.dbg_rows;
-  // Pair one slot of at most N entries in registers (one lane).
-  auto pair_small = [&](auto nt, int32_t s, uint32_t c) {
+  // Pair one slot of at most N entries in registers (one lane).  Slots of at most 2 entries
+  // come with their keys (ka, kb: 0 past the count); larger ones read theirs here.
+  auto pair_small = [&](auto nt, int32_t s, uint32_t c, uint32_t ka, uint32_t kb) {
     constexpr int N = decltype(nt)::value;
     t.count[s] = 0u;
     acc.nslot++;
     uint32_t k[N];
+    if constexpr (N == 2) {
+      k[0] = ka;
+      k[1] = kb;
+    } else {
 #pragma unroll
-    for (int q = 0; q < N; q++) k[q] = (q < (int)c) ? t.ent[(int64_t)q * G2 + s].x : 0u;
+      for (int q = 0; q < N; q++) k[q] = (q < (int)c) ? t.ent[(int64_t)q * G2 + s].x : 0u;
+    }
     SlotResult r;
     pair_keys<N>(k, (int)c, p.pp, r, dbg_rows ? dbg_rows + (int64_t)s * 64 : nullptr);
     account(acc, r, s, p);
@@ -1133,22 +1140,27 @@ __device__ __forceinline__ void pair_body(const SweepParams &p) {
   };
   // The touched box is walked 64 slots per wave step; slots of at most kSmall entries are
   // queued by size class (<= 2, <= 4, <= 8) and paired 64 at a time with that class's register
-  // network, so no wave runs a larger network than its slots need.
+  // network, so no wave runs a larger network than its slots need.  Each step first issues the
+  // loads of the next step's counters and of its own slots' keys when they have at most 2
+  // entries (most slots), then pairs the batches earlier steps filled while those loads are in
+  // flight, and only then queues its slots with their keys.  One extra step with no slots
+  // pairs what remains.
   if constexpr (PART == 0) {
   uint32_t n2 = 0u, n4 = 0u, n8 = 0u;  // queue fills (wave-uniform)
   uint32_t *const q2 = pairq[0][wave], *const q4 = pairq[1][wave], *const q8 = pairq[2][wave];
+  uint32_t *const qa = pairk[0][wave], *const qb = pairk[1][wave];
   const int32_t i1 = min(i0 + p.pair_span, nbb);
   if (i0 + wave * 64 < i1) {
     // slot i of the box is (bx0 + i / wy, by0 + i % wy); step kThreads slots at a time
     const int32_t first = i0 + wave * 64 + lane;
     int32_t sx = bx0 + first / wy, sy = by0 + first % wy;
     const int32_t dq = kThreads / wy, dr = kThreads % wy;
-    // Each step's counters are loaded one step ahead, so their latency overlaps the pairing.
     uint32_t c_next = first < i1 ? t.count[sx * G + sy] : 0u;
     for (int32_t base = i0 + wave * 64;; base += kThreads) {
+      const bool tail = base >= i1;  // the extra step
       const int32_t i = base + lane;
       const int32_t s = sx * G + sy;
-      const uint32_t c = c_next;
+      const uint32_t c = i < i1 ? c_next : 0u;
       sx += dq;
       sy += dr;
       if (sy > by1) {
@@ -1156,6 +1168,55 @@ __device__ __forceinline__ void pair_body(const SweepParams &p) {
         sx += 1;
       }
       if (i + kThreads < i1) c_next = t.count[sx * G + sy];
+      uint32_t ka = 0u, kb = 0u;
+      if (c - 1u < 2u) {
+        ka = t.ent[s].x;
+        if (c == 2u) kb = t.ent[(int64_t)G2 + s].x;
+      }
+      // Pair full batches (in the extra step, everything queued), one class at a time: a single
+      // call site per network keeps the code (and the register demand) small.
+      for (;;) {
+        const uint32_t lim = tail ? 1u : 64u;
+        const int fc = n2 >= lim ? 0 : (n4 >= lim ? 1 : (n8 >= lim ? 2 : -1));
+        if (fc < 0) break;
+        uint32_t *const q = fc == 0 ? q2 : (fc == 1 ? q4 : q8);
+        const uint32_t n = fc == 0 ? n2 : (fc == 1 ? n4 : n8);
+        const uint32_t np = n < 64u ? n : 64u;
+        wave_sync();
+        if ((uint32_t)lane < np) {
+          const uint32_t qe = q[lane];
+          const int32_t qs = (int32_t)(qe & 0xffffffu);
+          const uint32_t qc = qe >> 24;
+          if (fc == 0) pair_small(std::integral_constant<int, 2>{}, qs, qc, qa[lane], qb[lane]);
+          else if (fc == 1) pair_small(std::integral_constant<int, 4>{}, qs, qc, 0u, 0u);
+          else pair_small(std::integral_constant<int, kSmall>{}, qs, qc, 0u, 0u);
+        }
+        wave_sync();
+        if (n > 64u) {
+          const uint32_t rest = n - 64u;
+          uint32_t mm = 0u, ma = 0u, mb = 0u;
+          if ((uint32_t)lane < rest) {
+            mm = q[64 + lane];
+            if (fc == 0) {
+              ma = qa[64 + lane];
+              mb = qb[64 + lane];
+            }
+          }
+          wave_sync();
+          if ((uint32_t)lane < rest) {
+            q[lane] = mm;
+            if (fc == 0) {
+              qa[lane] = ma;
+              qb[lane] = mb;
+            }
+          }
+          wave_sync();
+        }
+        if (fc == 0) n2 = n - np;
+        else if (fc == 1) n4 = n - np;
+        else n8 = n - np;
+      }
+      if (tail) break;
       int cls = -1;
       uint32_t e = 0u;
       if (i < i1) {
@@ -1172,7 +1233,12 @@ __device__ __forceinline__ void pair_body(const SweepParams &p) {
       }
       {
         const uint64_t m = __ballot(cls == 0);
-        if (cls == 0) q2[n2 + lanes_below(m)] = e;
+        if (cls == 0) {
+          const uint32_t at = n2 + lanes_below(m);
+          q2[at] = e;
+          qa[at] = ka;
+          qb[at] = kb;
+        }
         n2 += (uint32_t)__popcll(m);
       }
       {
@@ -1185,39 +1251,6 @@ __device__ __forceinline__ void pair_body(const SweepParams &p) {
         if (cls == 2) q8[n8 + lanes_below(m)] = e;
         n8 += (uint32_t)__popcll(m);
       }
-      const bool last = base + kThreads >= i1;
-      // Pair full batches of 64 (and, after the last step, what remains), one class at a time:
-      // a single call site per network keeps the code (and the register demand) small.
-      for (;;) {
-        const uint32_t lim = last ? 1u : 64u;
-        const int fc = n2 >= lim ? 0 : (n4 >= lim ? 1 : (n8 >= lim ? 2 : -1));
-        if (fc < 0) break;
-        uint32_t *const q = fc == 0 ? q2 : (fc == 1 ? q4 : q8);
-        const uint32_t n = fc == 0 ? n2 : (fc == 1 ? n4 : n8);
-        const uint32_t np = n < 64u ? n : 64u;
-        wave_sync();
-        if ((uint32_t)lane < np) {
-          const uint32_t qe = q[lane];
-          const int32_t qs = (int32_t)(qe & 0xffffffu);
-          const uint32_t qc = qe >> 24;
-          if (fc == 0) pair_small(std::integral_constant<int, 2>{}, qs, qc);
-          else if (fc == 1) pair_small(std::integral_constant<int, 4>{}, qs, qc);
-          else pair_small(std::integral_constant<int, kSmall>{}, qs, qc);
-        }
-        wave_sync();
-        if (n > 64u) {
-          const uint32_t rest = n - 64u;
-          uint32_t mm = 0u;
-          if ((uint32_t)lane < rest) mm = q[64 + lane];
-          wave_sync();
-          if ((uint32_t)lane < rest) q[lane] = mm;
-          wave_sync();
-        }
-        if (fc == 0) n2 = n - np;
-        else if (fc == 1) n4 = n - np;
-        else n8 = n - np;
-      }
-      if (last) break;
     }
   }
   } else {
