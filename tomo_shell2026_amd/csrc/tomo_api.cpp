@@ -313,7 +313,7 @@ int ensure_scratch(tomo_handle *h, int32_t batch) {
     h->d_hdr = nullptr;
     return fail(TOMO_E_OOM, "header allocation failed");
   }
-  // Slot counters start at zero; the kernels restore that before a region is reused.
+  // Defined contents from the start (each sweep writes every counter and header it reads).
   HIP_TRY(hipMemsetAsync(h->d_scratch, 0, bytes, h->stream));
   HIP_TRY(hipMemsetAsync(h->d_hdr, 0, sizeof(OrientHdr) * (size_t)batch, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
@@ -859,8 +859,9 @@ int tomo_destroy(tomo_handle *h) {
 // ---------------------------------------------------------------------------------------------
 // Batched sweeps: many meshes, each with its own orientations and options, in one captured
 // hipGraph launch on one stream.  Members keep their own mesh, rotations and records; they share
-// the batch's stream, slot-table scratch and headers, sized for the largest member.  Before each member an acquire pass zeroes the slot counters of the regions it
-// will use, laid out for its grid and inline capacity (the memory held another member's layout).
+// the batch's stream, slot-table scratch and headers, sized for the largest member.  A member
+// lays the regions out for its own grid and inline capacity over the previous member's data:
+// every table a sweep reads is written by that sweep first (the rasteriser writes all counters).
 
 struct tomo_batch {
   int device = 0;
@@ -1102,11 +1103,7 @@ int tomo_batch_capture(tomo_batch *b) {
   for (size_t m = 0; m < b->members.size() && rc == TOMO_OK; m++) {
     tomo_handle *h = b->members[m];
     const int32_t bs = b->tune[m].batch;
-    SweepParams p = base_params(h);
-    p.n_batch = std::min(bs, h->n_orient);
-    const hipError_t e = launch_stage(6, p, b->stream);
-    if (e != hipSuccess) rc = fail(TOMO_E_HIP, std::string("acquire launch: ") + hipGetErrorString(e));
-    if (rc == TOMO_OK) rc = enqueue(h, 0, h->n_orient, bs, nullptr, 0, nullptr);
+    rc = enqueue(h, 0, h->n_orient, bs, nullptr, 0, nullptr);
   }
   hipGraph_t g = nullptr;
   const hipError_t ec = hipStreamEndCapture(b->stream, &g);

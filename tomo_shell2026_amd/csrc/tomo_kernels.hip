@@ -546,11 +546,6 @@ __global__ __launch_bounds__(kThreads) void aabb_kernel(SweepParams p) {
   const float r0 = Rg[0], r1 = Rg[1], r2 = Rg[2], r3 = Rg[3], r4 = Rg[4], r5 = Rg[5], r6 = Rg[6],
               r7 = Rg[7], r8 = Rg[8];
   OrientHdr &h = p.hdr[b];
-  {
-    const Tables t = tables_for(p, b);
-    const uint32_t nbig_prev = h.big_top;
-    for (uint32_t j = tid; j < nbig_prev; j += kThreads) t.count[t.big[j]] = 0u;
-  }
   float mx = 1e5f, my = 1e5f, mz = 1e5f;
   for (int32_t v = tid; v < p.n_vtx; v += kThreads) {
     const float t0 = p.vx[v] - p.cx, t1 = p.vy[v] - p.cy, t2 = p.vz[v] - p.cz;
@@ -941,8 +936,8 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6)
     }
     __syncthreads();  // every listed chunk is done before the list is refilled
   }
-  // Write the tile's non-zero counters (the global table is zero elsewhere) and list the slots
-  // with more than kSmall entries.
+  // Write every counter of the tile, zeros included (so no later kernel has to clear the
+  // global table), and list the slots with more than kSmall entries.
   {
     const Tables &t = em.t;
     const int32_t yshift = p.tile_y_shift, ymask = (1 << yshift) - 1;
@@ -956,7 +951,9 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6)
         const int32_t li = 2 * wi + half;
         const int32_t sx = xa + (li >> yshift), sy = ya + (li & ymask);
         const int32_t s = sx * G + sy;
-        if (c) t.count[s] = c;
+        // only this tile's slots: threads past its last word, and the part of the last tile
+        // row beyond the grid, write nothing
+        if (wi < nwords && sy < yb) t.count[s] = c;
         const bool big = c > (uint32_t)kSmall;
         const uint64_t bm = __ballot(big);
         if (bm) {
@@ -1117,7 +1114,6 @@ __device__ __forceinline__ void pair_body(const SweepParams &p) {
   // come with their keys (ka, kb: 0 past the count); larger ones read theirs here.
   auto pair_small = [&](auto nt, int32_t s, uint32_t c, uint32_t ka, uint32_t kb) {
     constexpr int N = decltype(nt)::value;
-    t.count[s] = 0u;
     acc.nslot++;
     uint32_t k[N];
     if constexpr (N == 2) {
@@ -1254,8 +1250,6 @@ __device__ __forceinline__ void pair_body(const SweepParams &p) {
     }
   }
   } else {
-  // Large slots keep their counters set until the next aabb_kernel on this scratch region
-  // (resetting them here would race with the small-slot scan of other groups).
   // Those with at most 16 entries, all inline: one lane each, 16-key register network.
   const uint32_t lane_cap = t.inl < 16 ? (uint32_t)t.inl : 16u;
   for (uint32_t j = (uint32_t)(tile * kThreads + tid); j < nbig; j += (uint32_t)(groups * kThreads)) {
@@ -1600,21 +1594,6 @@ __global__ __launch_bounds__(64) void fsum_kernel(SweepParams p) {
   if (lane == 0) R.fsum_done = 1;
 }
 
-// ---- acquire_kernel: one work-group per orientation region ---------------------------------
-// The sweep kernels expect every slot counter of a region at zero (they restore that
-// themselves between orientations of one layout).  When regions shared by several meshes are
-// laid out for a new grid or inline capacity, the new counter arrays sit on the previous
-// layout's data: zero them, and the large-slot list that aabb_kernel would otherwise replay.
-__global__ __launch_bounds__(kThreads) void acquire_kernel(SweepParams p) {
-  const int32_t b = blockIdx.x;
-  const Tables t = tables_for(p, b);
-  const int32_t G2 = p.grid * p.grid;
-  uint4 *c4 = reinterpret_cast<uint4 *>(t.count);  // count is 256-byte aligned
-  for (int32_t i = threadIdx.x; i < G2 / 4; i += kThreads) c4[i] = make_uint4(0u, 0u, 0u, 0u);
-  for (int32_t i = (G2 / 4) * 4 + threadIdx.x; i < G2; i += kThreads) t.count[i] = 0u;
-  if (threadIdx.x == 0) p.hdr[b].big_top = 0u;
-}
-
 hipError_t launch_stage(int stage, const SweepParams &p, hipStream_t stream) {
   const int32_t nb = p.n_batch;
   const int32_t nb8 = (nb + 7) / 8 * 8;  // xcd_map pads the orientation count to whole XCD rounds
@@ -1631,9 +1610,6 @@ hipError_t launch_stage(int stage, const SweepParams &p, hipStream_t stream) {
       break;
     case 5:
       if (p.exact_fsum) hipLaunchKernelGGL(fsum_kernel, dim3(nb), dim3(64), 0, stream, p);
-      break;
-    case 6:  // n_batch = regions to clear for this layout
-      hipLaunchKernelGGL(acquire_kernel, dim3(nb), dim3(kThreads), 0, stream, p);
       break;
     default: return hipErrorInvalidValue;
   }
