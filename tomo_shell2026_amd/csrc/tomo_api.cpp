@@ -424,10 +424,10 @@ SweepParams base_params(tomo_handle *h) {
   p.tiles_x = (h->opt.grid + p.tile_x - 1) / p.tile_x;
   p.tiles_y = (h->opt.grid + (1 << ys) - 1) >> ys;
   p.tiles = p.tiles_x * p.tiles_y;
-  // Pairing work-groups: 4,096 touched-box slots each up to a 256 grid, 1,024 above (measured on
-  // the 69k-triangle sphere at 256 and the thin-shell manikin at 512: 9% and 3% faster than
-  // 2,048); an eighth as many for the large-slot list.
-  p.pair_span = h->opt.grid <= 256 ? 4096 : 1024;
+  // Pairing work-groups: 8,192 touched-box slots each up to a 256 grid, 1,024 above (measured on
+  // the 69k-triangle sphere at 256 against 4,096 and 16,384: 3% faster than either; on the
+  // thin-shell manikin at 512, 3% faster than 2,048); an eighth as many for the large-slot list.
+  p.pair_span = h->opt.grid <= 256 ? 8192 : 1024;
   p.pair_tiles = (g2 + p.pair_span - 1) / p.pair_span;
   p.big_groups = p.pair_tiles >= 8 ? p.pair_tiles / 8 : 1;
   // Bed bands of 40 rows up to a 256 grid, where a band and its halo fit the bed kernel's LDS
