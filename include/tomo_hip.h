@@ -43,9 +43,10 @@ extern "C" {
                                 /* reference writes out of bounds: undefined behaviour)       */
 #define TOMO_F_CAP_DROP 8       /* a slot held 63 distinct rows and dropped one               */
 #define TOMO_F_SENTINEL_DROP 16 /* a floor row was skipped because its slot was full          */
-#define TOMO_F_RANK_TIE 32      /* a triangle put two pixels into one slot (insertion order   */
-                                /* between them is not tracked; matters only for CAP_DROP)    */
-#define TOMO_F_UNRESOLVED 64    /* a slot exceeded the exact-resolution capacity              */
+                                /* (bit 32 is unused)                                         */
+#define TOMO_F_UNRESOLVED 64    /* more than 65,535 rows fell on one slot (a slot counter is  */
+                                /* 16 bits) or one triangle put more than 1,024 rows into a   */
+                                /* slot of over 8,192 rows: the record is not exact           */
 
 /* Bed types (Tomo_types.h:64-69). */
 #define TOMO_BED_NONE 0
@@ -183,6 +184,14 @@ int tomo_batch_tune(tomo_batch *b, int32_t sample);
  * microseconds per orientation (negative if not tuned). */
 int tomo_batch_get_tuning(tomo_batch *b, int32_t member, int32_t *batch, int32_t *tile_y,
                           int32_t *inline_capacity, float *us_per_orientation);
+
+/* Set a member's launch shape by hand, as tomo_batch_tune would: orientations in flight
+ * (0 = the default, min(n_ypr, 2048)), rasteriser tile extent along y (0 or a power of two
+ * 32..512) and inline slot capacity (8..32).  Lets a caller reuse shapes tuned in an earlier
+ * process instead of tuning again.  Invalidates a capture.  No reference counterpart (the
+ * reference sweeps parts one call at a time). */
+int tomo_batch_set_tuning(tomo_batch *b, int32_t member, int32_t batch, int32_t tile_y,
+                          int32_t inline_capacity);
 
 /* Allocate the shared scratch and capture every member's sweep into one graph (no launch). */
 int tomo_batch_capture(tomo_batch *b);

@@ -61,6 +61,10 @@ def lib():
         L.orc_vol_to_mass.argtypes = (P, P)
         L.orc_bary_point.argtypes = (P, P, P, P, P)
         L.orc_bary_point.restype = ct.c_int
+        L.orc_bary_parts.argtypes = (P, P, P, P, P)
+        L.orc_bary_parts.restype = ct.c_int
+        L.orc_insert_log.argtypes = (P, ct.c_int32, P, ct.c_int32, P, P, P, ct.c_int64, P)
+        L.orc_insert_log.restype = ct.c_int64
         L.orc_find_first_optimal.argtypes = (ct.c_int32, P)
         L.orc_find_first_optimal.restype = ct.c_int32
         _lib = L
@@ -168,6 +172,30 @@ def bary_point(p, a, b, c):
     uvw = np.zeros(3, np.float32)
     ok = lib().orc_bary_point(*[_ptr(x) for x in arr], _ptr(uvw))
     return bool(ok), uvw
+
+
+def bary_parts(p, a, b, c):
+    """(accepted, den, v, w, u) of _getBaryCoord for one pixel centre (v, w, u = 0 if |den| <= eps)."""
+    arr = [np.ascontiguousarray(x, np.float32) for x in (p, a, b, c)]
+    part = np.zeros(4, np.float32)
+    ok = lib().orc_bary_parts(*[_ptr(x) for x in arr], _ptr(part))
+    return bool(ok), part[0], part[1], part[2], part[3]
+
+
+INSERT_LOG_DTYPE = np.dtype([("tri", "<i4"), ("seq", "<i4"), ("slot", "<i4"), ("key", "<u4")])
+
+
+def insert_log(f32, i32, grid, tri, vtx, vnrm, ypr3, cap=1 << 22):
+    """Every SetBit call of Pixelize for one orientation, in call order: (tri, seq, slot, key)."""
+    tri, vtx, vnrm = _mesh_arrays(tri, vtx, vnrm)
+    y = np.ascontiguousarray(ypr3, dtype=np.float32).reshape(3)
+    out = np.zeros(cap, dtype=INSERT_LOG_DTYPE)
+    o = make_opts(f32, i32, grid)
+    n = lib().orc_insert_log(ct.byref(o), tri.shape[0], _ptr(tri), vtx.shape[0], _ptr(vtx), _ptr(vnrm),
+                             _ptr(y), cap, _ptr(out))
+    if n < 0 or n > cap:
+        raise RuntimeError(f"orc_insert_log failed or overflowed: {n}")
+    return out[:n].copy()
 
 
 def find_first_optimal(mss):

@@ -40,6 +40,10 @@ typedef struct {
   int64_t ndirty, capdirty;
   float *vx, *vy, *vz, *nz; /* placed vertices and rotated normal z */
   int32_t flags;
+  /* optional log of Pixelize's SetBit calls (test diagnostics): {triangle, seq, slot, key} */
+  int32_t *log;
+  int64_t nlog, caplog;
+  int32_t cur_tri, cur_seq;
 } orc_ws;
 
 static inline float orc_min(float a, float b) { return (a <= b) ? a : b; } /* Tomo_types.cpp:29-32 */
@@ -157,6 +161,16 @@ static void orc_setbit(orc_ws *w, float px, float py, float pz, float nzf, int t
   int16_t z = orc_to_i16(w, pz);
   int16_t nz = orc_to_i16(w, nzf * ORC_NF);
   int32_t s = x * w->grid + y;
+  if (w->log && type == 0) {
+    if (w->nlog < w->caplog) {
+      int32_t *e = w->log + 4 * w->nlog;
+      e[0] = w->cur_tri;
+      e[1] = w->cur_seq;
+      e[2] = s;
+      e[3] = (int32_t)(((uint32_t)(z + 32768) << 16) | (uint32_t)(nz + 32768));
+    }
+    w->nlog++;
+  }
   int16_t *slot = orc_slot(w, s);
   int n = slot[0];
   int id = -1;
@@ -173,9 +187,10 @@ static void orc_setbit(orc_ws *w, float px, float py, float pz, float nzf, int t
   slot[(id + 1) * 3 + 2] = nz;
 }
 
-/* _getBaryCoord (Tomo_types.cpp:86-115) with _dot (Tomo_types.cpp:14-17) on z=0 vectors. */
-static inline int orc_bary(float px, float py, const float *a, const float *b, const float *c,
-                           float *uo, float *vo, float *wo) {
+/* _getBaryCoord (Tomo_types.cpp:86-115) with _dot (Tomo_types.cpp:14-17) on z=0 vectors.
+ * part[] (may be NULL) receives den, v, w, u (v, w, u only when |den| > eps, else 0). */
+static inline int orc_bary_full(float px, float py, const float *a, const float *b, const float *c,
+                                float *uo, float *vo, float *wo, float *part) {
   const float zz = 0.0f * 0.0f;
   float e0x = b[0] - a[0], e0y = b[1] - a[1];
   float e1x = c[0] - a[0], e1y = c[1] - a[1];
@@ -187,10 +202,12 @@ static inline int orc_bary(float px, float py, const float *a, const float *b, c
   float d21 = e2x * e1x + e2y * e1y + zz;
   float den = d00 * d11 - d01 * d01;
   float aden = (den >= 0.0f) ? den : -den;
+  if (part) { part[0] = den; part[1] = part[2] = part[3] = 0.0f; }
   if (aden > ORC_MARGIN) {
     float v = (d11 * d20 - d01 * d21) / den;
     float w = (d00 * d21 - d01 * d20) / den;
     float u = (float)((1.0 - (double)v) - (double)w);
+    if (part) { part[1] = v; part[2] = w; part[3] = u; }
     if (u >= -ORC_MARGIN && v >= -ORC_MARGIN && (double)v <= 1.0 + (double)ORC_MARGIN &&
         (double)(u + v) <= 1.0 + (double)ORC_MARGIN) {
       *uo = u; *vo = v; *wo = w;
@@ -198,6 +215,11 @@ static inline int orc_bary(float px, float py, const float *a, const float *b, c
     }
   }
   return 0;
+}
+
+static inline int orc_bary(float px, float py, const float *a, const float *b, const float *c,
+                           float *uo, float *vo, float *wo) {
+  return orc_bary_full(px, py, a, b, c, uo, vo, wo, NULL);
 }
 
 /* STomoSh_INT3::triVoxel (STomoSh_INT3.cpp:1138-1209).  a*: placed vertices, n*z: rotated normal z. */
@@ -223,6 +245,7 @@ static void orc_trivoxel(orc_ws *w, const orc_ctx *c, const float *a0, const flo
       float cy = (float)y + ORC_MARGIN + h2;
       float u, v, wv;
       if (!orc_bary(cx, cy, v0, v1, v2, &u, &v, &wv)) continue;
+      w->cur_seq = 2 * ((x - x0) * (y1 - y0 + 1) + (y - y0)); /* insertion order within the triangle */
       /* _bary_product (Tomo_types.cpp:75-84): p = p0*u + p1*v + p2*w */
       float px = v0[0] * u + v1[0] * v + v2[0] * wv;
       float py = v0[1] * u + v1[1] * v + v2[1] * wv;
@@ -233,6 +256,7 @@ static void orc_trivoxel(orc_ws *w, const orc_ctx *c, const float *a0, const flo
       if (do_shell) { /* shell sub-pixel: z moved against the normal, normal flipped (:1179-1185) */
         pz += (nz >= 0.0f) ? -off : off;
         nz = -nz;
+        w->cur_seq++;
         orc_setbit(w, px, py, pz, nz, 0);
         hits++;
       }
@@ -241,9 +265,11 @@ static void orc_trivoxel(orc_ws *w, const orc_ctx *c, const float *a0, const flo
   if (hits == 0) { /* tiny triangle: one pixel at the un-margined centroid (:1190-1206) */
     float t[3];
     for (int i = 0; i < 3; i++) t[i] = (a0[i] + a1[i] + a2[i]) * 0.333333f + ORC_MARGIN;
+    w->cur_seq = 0;
     orc_setbit(w, t[0], t[1], t[2], n0z, 0);
     if (do_shell) {
       float sz = t[2] + ((n0z >= 0.0f) ? -off : off);
+      w->cur_seq = 1;
       orc_setbit(w, t[0], t[1], sz, -n0z, 0);
     }
   }
@@ -292,6 +318,7 @@ static void orc_pixelize(orc_ws *w, const orc_ctx *c, int32_t nTri, const int32_
   orc_clear(w);
   for (int32_t t = 0; t < nTri; t++) {
     int32_t i0 = tri[t * 3 + 0], i1 = tri[t * 3 + 1], i2 = tri[t * 3 + 2];
+    w->cur_tri = t;
     float a0[3] = {w->vx[i0], w->vy[i0], w->vz[i0]};
     float a1[3] = {w->vx[i1], w->vy[i1], w->vz[i1]};
     float a2[3] = {w->vx[i2], w->vy[i2], w->vz[i2]};
@@ -697,6 +724,33 @@ int orc_bary_point(const float p[2], const float a[2], const float b[2], const f
   int ok = orc_bary(p[0], p[1], a3, b3, c3, &u, &v, &w);
   uvw[0] = u; uvw[1] = v; uvw[2] = w;
   return ok;
+}
+
+int orc_bary_parts(const float p[2], const float a[2], const float b[2], const float c[2], float part[4]) {
+  float a3[3] = {a[0], a[1], 0.0f}, b3[3] = {b[0], b[1], 0.0f}, c3[3] = {c[0], c[1], 0.0f};
+  float u, v, w;
+  return orc_bary_full(p[0], p[1], a3, b3, c3, &u, &v, &w, part);
+}
+
+int64_t orc_insert_log(const orc_opts *opt, int32_t nTri, const int32_t *tri, int32_t nVtx,
+                       const float *vtx, const float *vnrm, const float ypr[3], int64_t cap,
+                       int32_t *out) {
+  if (!opt || nTri < 0 || nVtx < 0 || (nTri > 0 && (!tri || !vtx || !vnrm)) || cap < 0 || (cap > 0 && !out))
+    return -1;
+  orc_ctx c;
+  orc_ctx_init(&c, opt, nVtx, vtx);
+  orc_ws w;
+  if (orc_ws_init(&w, c.grid, nVtx)) { orc_ws_free(&w); return -2; }
+  float R[9], mat[16];
+  orc_ypr_matrix(ypr[0], ypr[1], ypr[2], R);
+  orc_rotate(&w, &c, R, nVtx, vtx, vnrm, mat);
+  w.log = out;
+  w.caplog = cap;
+  w.nlog = 0;
+  orc_pixelize(&w, &c, nTri, tri);
+  int64_t n = w.nlog;
+  orc_ws_free(&w);
+  return n;
 }
 
 int32_t orc_find_first_optimal(int32_t n, const float *mss) {

@@ -94,6 +94,19 @@ void orc_vol_to_mass(const orc_opts *opt, orc_record *r);
 /* _getBaryCoord (Tomo_types.cpp:86-115) for one point; returns 1 and u, v, w when accepted. */
 int orc_bary_point(const float p[2], const float a[2], const float b[2], const float c[2], float uvw[3]);
 
+/* _getBaryCoord's intermediate values for one point: part = {den, v, w, u} (v, w, u are 0 when
+ * |den| <= eps); returns 1 when the point is accepted. */
+int orc_bary_parts(const float p[2], const float a[2], const float b[2], const float c[2], float part[4]);
+
+/* Test diagnostics: every SetBit call Pixelize makes for one orientation, in call order, as
+ * {triangle, seq, slot, key} with seq = 2 * (pixel index in the triangle's box, x outer, y inner)
+ * + sub (0 and 1 for the centroid rows) and key = ((z + 32768) << 16) | (nZ + 32768); entries
+ * outside the grid are not logged.  Writes at most cap entries to out[cap][4]; returns the count
+ * (negative on bad arguments). */
+int64_t orc_insert_log(const orc_opts *opt, int32_t nTri, const int32_t *tri, int32_t nVtx,
+                       const float *vtx, const float *vnrm, const float ypr[3], int64_t cap,
+                       int32_t *out);
+
 /* First strict minimum with initial value 1e5 (Tomo_Shell2026.cpp:89-97). */
 int32_t orc_find_first_optimal(int32_t n, const float *mss);
 

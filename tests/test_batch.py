@@ -72,3 +72,82 @@ def test_batch_graph_matches_single_sweeps_and_oracle(oracle_mod):
         ref = oracle_mod.sweep(w.f32, w.i32, w.grid, w.tri, w.vtx, w.vnrm, w.ypr, nthreads=8)
         bad = C.compare_records(first[m], ref, f"{w.name} graph vs oracle: ")
         assert not bad, "\n".join(bad)
+
+
+@pytest.mark.gpu
+def test_members_reuse_larger_layouts_in_any_order(oracle_mod):
+    """Members share one scratch block and each lays its slot table out over the previous
+    member's data, so every table a sweep reads must be written by that sweep first.  A 512-grid
+    member with 24 inline entries and 256-tall tiles runs first, then a 256-grid member with 8
+    inline entries, a 128-grid member with 16, and the 512-grid one again with 8: graph results
+    must equal single sweeps with default shapes, and the oracle."""
+    from tomo_shell2026_amd import _lib
+    ypr = synth.ypr_grid(np.arange(0, 360, 24), np.arange(0, 61, 20))  # 15 x 4
+    big_v, big_f = synth.displaced_sphere_n(3000, 260.0, seed=11)
+    a = synth.member_workload("dsphere_big", big_v, big_f, shell=False, ypr=ypr)
+    bv, bf = synth.displaced_sphere_n(6000, 60.0, seed=12)
+    b = synth.member_workload("dsphere_mid", bv, bf, shell=False, ypr=ypr)
+    cv, cf = synth.icosphere(3, 25.0)
+    c = synth.member_workload("ico", cv, cf, shell=False, ypr=ypr)
+    ws = [a, b, c, a]
+    shapes = [(len(ypr), 256, 24), (40, 32, 8), (7, 0, 16), (len(ypr), 64, 8)]
+    assert [w.grid for w in ws] == [512, 256, 128, 512]
+    print("[gpu-case] batch member order: grids 512/256/128/512, inline 24/8/16/8", flush=True)
+    singles = []
+    for w in ws[:3]:
+        with _lib.Sweep(w.f32, w.i32, w.grid) as s:
+            s.set_mesh(w.tri, w.vtx, w.vnrm)
+            singles.append(s.sweep(w.ypr))
+    singles.append(singles[0])
+    with _lib.Batch() as B:
+        for w in ws:
+            B.add(w.f32, w.i32, w.grid, w.tri, w.vtx, w.vnrm, w.ypr)
+        for m, (bs, ty, inl) in enumerate(shapes):
+            B.set_tuning(m, batch=bs, tile_y=ty, inline=inl)
+            got = B.tuning(m)
+            assert (got["batch"], got["tile_y"], got["inline"]) == (bs, ty, inl)
+        for _ in range(2):
+            B.launch()
+            B.sync()
+            for m, w in enumerate(ws):
+                bad = C.compare_records(B.records(m), singles[m], f"{w.name}#{m} graph vs single: ")
+                assert not bad, "\n".join(bad)
+    for m, w in enumerate(ws[:3]):
+        ref = oracle_mod.sweep(w.f32, w.i32, w.grid, w.tri, w.vtx, w.vnrm, w.ypr, nthreads=8)
+        bad = C.compare_records(singles[m], ref, f"{w.name} single vs oracle: ")
+        assert not bad, "\n".join(bad)
+
+
+@pytest.mark.gpu
+def test_batch70_workload(oracle_mod):
+    """The 70-mesh workload at its full size: 6 manikin parts and 64 displaced spheres, each on
+    its own 1 degree yaw x 0..60 pitch grid (1,537,200 orientation evaluations), tuned, captured
+    into one graph and launched.  Every member's 21,960 records must equal its own per-mesh
+    sweep bit for bit, and every 499th orientation of every member must equal the oracle's.
+    This is the per-part flow of cfms_tomo/__init__.py:26-30 done in one launch."""
+    import time
+    from tomo_shell2026_amd import _lib
+    t0 = time.time()
+    ws = synth.batch70_workloads()
+    print(f"[gpu-case] batch70: {len(ws)} meshes generated in {time.time() - t0:.1f} s", flush=True)
+    with _lib.Batch() as B:
+        for w in ws:
+            B.add(w.f32, w.i32, w.grid, w.tri, w.vtx, w.vnrm, w.ypr)
+        B.tune(4096)
+        print(f"[gpu-case] batch70: tuned at {time.time() - t0:.1f} s", flush=True)
+        B.launch()
+        B.sync()
+        graph = [B.records(m) for m in range(len(ws))]
+    print(f"[gpu-case] batch70: graph launched at {time.time() - t0:.1f} s", flush=True)
+    for m, w in enumerate(ws):
+        with _lib.Sweep(w.f32, w.i32, w.grid) as s:
+            s.set_mesh(w.tri, w.vtx, w.vnrm)
+            single = s.sweep(w.ypr)
+        bad = C.compare_records(graph[m], single, f"{w.name} graph vs single: ")
+        assert not bad, "\n".join(bad)
+        assert not (graph[m]["flags"] & 64).any()
+        idx = np.arange(0, len(w.ypr), 499)
+        ref = oracle_mod.sweep(w.f32, w.i32, w.grid, w.tri, w.vtx, w.vnrm, w.ypr[idx], nthreads=8)
+        bad = C.compare_records(graph[m][idx], ref, f"{w.name} graph vs oracle: ")
+        assert not bad, "\n".join(bad)
+    print(f"[gpu-case] batch70: compared at {time.time() - t0:.1f} s", flush=True)

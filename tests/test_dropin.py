@@ -131,12 +131,13 @@ def test_manager_batched_group_matches_per_part():
     group_b = _Group([_Part(m, f"p{i}") for i, m in enumerate(meshes)])
     M = TomoManager(so)
     M.get_best_orientation(group_a, batched=True)
-    batched = [(c.tomo_object, M.CppDLL.Mss3D) for c in group_a.cutobjects]
+    batched = [(c.tomo_object, mss) for c, mss in zip(group_a.cutobjects, M.CppDLL.Mss3D_group)]
     per_part = []
     for c in group_b.cutobjects:
         t_o = M.CppDLL.run_dll(c, False)
         per_part.append((t_o, M.CppDLL.Mss3D.copy()))
-    for (a, _), (b, mss_b) in zip(batched, per_part):
+    for (a, mss_a), (b, mss_b) in zip(batched, per_part):
+        np.testing.assert_array_equal(mss_a.view(np.uint32), mss_b.view(np.uint32))
         np.testing.assert_array_equal(a.mat4x4.view(np.uint32), b.mat4x4.view(np.uint32))
         assert a.best_vminfo.GetMss() == b.best_vminfo.GetMss()
         assert (a.best_vminfo.vmData.yaw, a.best_vminfo.vmData.pitch) == (b.best_vminfo.vmData.yaw,
@@ -146,3 +147,39 @@ def test_manager_batched_group_matches_per_part():
         assert ga.keys() == gb.keys()
         for k in ga:
             np.testing.assert_array_equal(ga[k], gb[k])
+
+
+@pytest.mark.gpu
+def test_manager_batched_group_matches_oracle(oracle_mod):
+    """The batched cut-group bridge against the oracle: every part's per-orientation Mss from
+    the one graph launch equals the oracle's sweep of the same packed inputs bit for bit, and
+    the part's optimum (findOptimals) is the oracle's.  Parts: a closed sphere (solid mode) and
+    two open halves (shell mode, which run_dll picks for meshes that are not watertight)."""
+    from tests import _cases as C
+    from tomo_shell2026_amd.cfms_tomo import TomoManager
+    from tomo_shell2026_amd.cfms_tomo.tomo_io import findOptimals
+    print("[gpu-case] TomoManager batched group vs oracle", flush=True)
+    so = SlicingOptions()
+    so.theta_YP = 30
+    so.bVerbose = False
+    v1, f1 = synth.icosphere(3, 30.0)
+    hv, hf = C.slice_plane(v1, f1, (0.0, 0.0, 1.0), cap=False)
+    sv, sf = C.slice_plane(v1, f1, (1.0, 0.0, 0.7), cap=False)
+    meshes = [TomoMesh(v1 + 100.0, f1), TomoMesh(hv + 80.0, hf), TomoMesh(sv + 60.0, sf)]
+    group = _Group([_Part(m, f"p{i}") for i, m in enumerate(meshes)])
+    M = TomoManager(so)
+    M.get_best_orientation(group, batched=True)
+    ref_iface = TomoDLLInterface("Tomo_Shell2026.dll", so, True, True)
+    shells = []
+    for m, c_o, mss in zip(meshes, group.cutobjects, M.CppDLL.Mss3D_group):
+        ref_iface.bShellMesh = so.bShellMesh and not m.is_watertight
+        shells.append(bool(ref_iface.bShellMesh))
+        vtx, tri = ref_iface.pack(m, False)
+        vn = np.asarray(m.vertex_normals, np.float32)
+        ref = oracle_mod.sweep(ref_iface.Float32Info, ref_iface.Int32Info, int(ref_iface.Int32Info[3]), tri, vtx, vn,
+                               ref_iface.YPR, nthreads=8)
+        np.testing.assert_array_equal(mss.view(np.uint32), ref["Mss"].view(np.uint32))
+        best, _ = findOptimals(ref_iface.YPR, ref["Mss"], False)
+        got = c_o.tomo_object.best_vminfo
+        assert (got.vmData.yaw, got.vmData.pitch, got.GetMss()) == (best.vmData.yaw, best.vmData.pitch, best.GetMss())
+    assert shells == [False, True, True]

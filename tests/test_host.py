@@ -270,3 +270,32 @@ def test_marching_tetrahedra_sphere():
     e = np.sort(np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]]), 1)
     _, cnt = np.unique(e, axis=0, return_counts=True)
     assert (cnt == 2).all()
+
+
+def test_mesh_front_end_closed_forms():
+    """Angle-weighted vertex normals, area and watertightness (the quantities run_dll reads,
+    tomo_dll_interface.py:59-70) against closed forms.  The box's faces are split into two
+    triangles with diagonals that meet some corners twice and others once: at every corner the
+    three faces contribute 90 degrees each, so the angle-weighted normal is the corner's diagonal
+    (+-1, +-1, +-1)/sqrt(3); weighting by area (or by triangle count) would not give it at the
+    corners whose faces are split unevenly, which the test also checks."""
+    from tests import _cases as C
+    from tomo_shell2026_amd.mesh import face_normals, mesh_area
+    v, f = C.box((0.0, 0.0, 0.0), (2.0, 3.0, 4.0))
+    assert mesh_area(v, f) == pytest.approx(2 * (2 * 3 + 2 * 4 + 3 * 4), abs=1e-12)
+    n = vertex_normals(v, f)
+    ctr = v.mean(0)
+    diag = np.sign(v - ctr) / np.sqrt(3.0)
+    np.testing.assert_allclose(n, diag, atol=1e-12)
+    fn = face_normals(v, f)
+    area = 0.5 * np.linalg.norm(np.cross(v[f[:, 1]] - v[f[:, 0]], v[f[:, 2]] - v[f[:, 0]]), axis=1)
+    by_area = np.zeros_like(v)
+    for k in range(3):
+        np.add.at(by_area, f[:, k], fn * area[:, None])
+    by_area /= np.linalg.norm(by_area, axis=1, keepdims=True)
+    assert np.abs(by_area - diag).max() > 0.05  # the fixture separates the weightings
+    assert is_watertight(f)
+    flipped = f.copy()
+    flipped[3] = flipped[3, ::-1]
+    assert not is_watertight(flipped)
+    assert not is_watertight(f[1:])

@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(HERE, "libtomo_hip.so")
 
 TOMO_OK = 0
 F_FP32_INEXACT, F_INT16_WRAP, F_FOOTPRINT_OOB, F_CAP_DROP = 1, 2, 4, 8
-F_SENTINEL_DROP, F_RANK_TIE, F_UNRESOLVED = 16, 32, 64
+F_SENTINEL_DROP, F_UNRESOLVED = 16, 64
 
 RECORD_DTYPE = np.dtype(
     [("Va", "<i8"), ("Vb", "<i8"), ("Vo", "<i8"), ("Vss", "<i8"), ("Vnv", "<i8"),
@@ -32,7 +32,8 @@ EXPORTS = (
     "tomo_find_first_optimal", "TomoSh_HIP", "getMss", "getMo", "getVtc", "getVolMassInfo",
     "getMat4x4", "getnData2i", "getpData2i", "OnDestroy",
     "tomo_cvss_create", "tomo_cvss_forward", "tomo_cvss_backward", "tomo_cvss_destroy",
-    "tomo_batch_create", "tomo_batch_add", "tomo_batch_tune", "tomo_batch_get_tuning", "tomo_batch_capture",
+    "tomo_batch_create", "tomo_batch_add", "tomo_batch_tune", "tomo_batch_get_tuning", "tomo_batch_set_tuning",
+    "tomo_batch_capture",
     "tomo_batch_launch", "tomo_batch_sync", "tomo_batch_last_ms", "tomo_batch_size", "tomo_batch_records",
     "tomo_batch_destroy",
 )
@@ -96,6 +97,7 @@ def load():
         "tomo_batch_add": ((P, P, P, I32, I32, P, I32, P, P, I32, P, P), ct.c_int),
         "tomo_batch_tune": ((P, I32), ct.c_int),
         "tomo_batch_get_tuning": ((P, I32, P, P, P, P), ct.c_int),
+        "tomo_batch_set_tuning": ((P, I32, I32, I32, I32), ct.c_int),
         "tomo_batch_capture": ((P,), ct.c_int),
         "tomo_batch_launch": ((P,), ct.c_int),
         "tomo_batch_sync": ((P,), ct.c_int),
@@ -285,6 +287,10 @@ class Batch:
                                            ct.byref(us)), "tomo_batch_get_tuning")
         return {"batch": b.value, "tile_y": t.value, "inline": i.value,
                 "us_per_orientation": round(us.value, 4)}
+
+    def set_tuning(self, member, batch=0, tile_y=0, inline=16):
+        check(load().tomo_batch_set_tuning(self._h, int(member), int(batch), int(tile_y), int(inline)),
+              "tomo_batch_set_tuning")
 
     def capture(self):
         check(load().tomo_batch_capture(self._h), "tomo_batch_capture")

@@ -129,6 +129,7 @@ class TomoDLLInterface:
             mss = [B.records(m)["Mss"].astype(np.float32) for m in range(len(work))]
         finally:
             B.close()
+        self.Mss3D_group = mss  # per part, in cutobjects order
         out = []
         empty_i = np.zeros(3, np.int32)
         empty_f = np.zeros(3, np.float32)

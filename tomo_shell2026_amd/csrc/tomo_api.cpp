@@ -141,12 +141,14 @@ void vol_to_mass(const Options &o, float *vm) {
 }
 
 // One orientation's slot-table region.  inl entries per slot are stored inline; the overflow
-// pool grows as inl shrinks (2 * max(G^2, 65536) entries at inl = 8).
-ScratchLayout make_layout(int32_t G, int32_t inl) {
+// pool grows as inl shrinks (2 * max(G^2, 65536) entries at inl = 8).  scale multiplies the
+// pool and the spill list (orientations that overflow them are swept again with scale 4, 16, 64).
+ScratchLayout make_layout(int32_t G, int32_t inl, int32_t scale = 1) {
   ScratchLayout L{};
   const int64_t G2 = (int64_t)G * G;
   const int64_t base = G2 > 65536 ? G2 : 65536;
-  const int32_t pool = (int32_t)(base * kInlineDefault / (inl < kInlineDefault ? inl : kInlineDefault));
+  const int64_t pool64 = base * kInlineDefault / (inl < kInlineDefault ? inl : kInlineDefault) * scale;
+  const int32_t pool = (int32_t)std::min<int64_t>(pool64, (int64_t)1 << 30);
   int64_t at = 0;
   auto take = [&](int64_t bytes) {
     const int64_t here = at;
@@ -165,14 +167,18 @@ ScratchLayout make_layout(int32_t G, int32_t inl) {
   L.cmp_key = take(4 * (int64_t)pool);
   L.cmp_rank = take(4 * (int64_t)pool);
   L.sv = take(4 * 6 * G2);
-  const int32_t spill = 16 * G + 4096;
+  const int32_t spill = (16 * G + 4096) * scale;
   L.spill_slot = take(4 * (int64_t)spill);
   L.spill_key = take(4 * (int64_t)spill);
   L.spill_rank = take(4 * (int64_t)spill);
+  // slots of more than kBigCap entries each hold over kBigCap - inl pool entries
+  const int32_t nhuge = pool / 256 + 64;
+  L.huge = take(4 * (int64_t)nhuge);
   L.bytes = at;
   L.pool = pool;
   L.inl = inl;
   L.spill = spill;
+  L.nhuge = nhuge;
   return L;
 }
 
@@ -198,6 +204,7 @@ struct tomo_handle {
   int32_t n_chunks = 0;
   float *d_soa = nullptr;  // 3 * n_vtx: x, y, z
   int4 *d_tidx = nullptr;    // n_tri: vertex indices + caller's index (culling order)
+  int32_t *d_tpos = nullptr; // n_tri: position in d_tidx of each caller's triangle
   float4 *d_vaos = nullptr;  // 2 * n_vtx: (x, y, z, nx), (ny, nz, 0, 0)
   float *d_bed_dist = nullptr;
   // orientations
@@ -349,6 +356,7 @@ void finish_record(const tomo_handle *h, const DevRecord &d, const float *R, tom
   r.Vbed = d.Vbed; r.n_entries = d.n_entries; r.n_slots = d.n_slots; r.n_final = d.n_final;
   r.chk_pre = d.chk_pre; r.chk_final = d.chk_final; r.n_bed = (int32_t)d.Vbed;
   int32_t flags = d.flags;
+  if (flags & F_RETRY) flags = (flags & ~F_RETRY) | TOMO_F_UNRESOLVED;  // still overflowing
   const int64_t lim = (int64_t)1 << 24;
   if (d.aVo > lim || d.aVss > lim || d.aVa > lim || d.aVb > lim || d.aVnv > lim || d.aVtc > lim)
     flags |= TOMO_F_FP32_INEXACT;
@@ -392,6 +400,7 @@ SweepParams base_params(tomo_handle *h) {
   p.chunk_box = h->d_chunk;
   p.n_chunks = h->n_chunks;
   p.tidx = h->d_tidx;
+  p.tpos = h->d_tpos;
   p.vaos = h->d_vaos;
   p.vx = h->d_soa;
   p.vy = h->d_soa + h->n_vtx;
@@ -475,6 +484,82 @@ int enqueue(tomo_handle *h, int32_t begin, int32_t end, int32_t batch, int32_t *
   return TOMO_OK;
 }
 
+// Device copy of a host array (hipErrorInvalidValue-free for empty arrays: they hold one
+// element).  The type of *dst may differ from the element type (same bytes).
+template <typename D, typename T>
+hipError_t upload(D **dst, const std::vector<T> &v) {
+  *dst = nullptr;
+  const size_t bytes = v.size() * sizeof(T);
+  hipError_t e = hipMalloc((void **)dst, bytes);
+  if (e != hipSuccess) {
+    *dst = nullptr;
+    return e;
+  }
+  return hipMemcpy(*dst, v.data(), bytes, hipMemcpyHostToDevice);
+}
+
+void drop_mesh(tomo_handle *h) {
+  if (h->d_tidx) (void)hipFree(h->d_tidx);
+  if (h->d_tpos) (void)hipFree(h->d_tpos);
+  if (h->d_vaos) (void)hipFree(h->d_vaos);
+  if (h->d_chunk) (void)hipFree(h->d_chunk);
+  if (h->d_soa) (void)hipFree(h->d_soa);
+  h->d_tidx = nullptr; h->d_tpos = nullptr; h->d_vaos = nullptr; h->d_chunk = nullptr; h->d_soa = nullptr;
+  h->n_tri = h->n_vtx = h->n_chunks = 0;
+}
+
+// Orientations whose overflow pool or spill list filled up (F_RETRY in d, the records of
+// orientations [begin, begin + d.size())) are swept again one at a time in a private slot table
+// whose pool and spill list are 4x larger, then 16x and 64x while some still overflow.  The
+// handle's own scratch (possibly shared with a batch) is not touched.
+int retry_overflowed(tomo_handle *h, int32_t begin, std::vector<DevRecord> &d) {
+  std::vector<int32_t> idx;
+  for (size_t i = 0; i < d.size(); i++)
+    if (d[i].flags & F_RETRY) idx.push_back((int32_t)i);
+  for (int32_t scale = 4; scale <= 64 && !idx.empty(); scale *= 4) {
+    const ScratchLayout lay = make_layout(h->opt.grid, h->lay.inl, scale);
+    uint8_t *scr = nullptr;
+    OrientHdr *hdr = nullptr;
+    if (hipMalloc(&scr, (size_t)lay.bytes) != hipSuccess || hipMalloc(&hdr, sizeof(OrientHdr)) != hipSuccess) {
+      (void)hipGetLastError();
+      (void)hipFree(scr);
+      return fail(TOMO_E_OOM, "slot table for an overflowing orientation");
+    }
+    hipError_t e = hipSuccess;
+    for (int32_t i : idx) {
+      SweepParams p = base_params(h);
+      p.lay = lay;
+      p.scratch = scr;
+      p.hdr = hdr;
+      p.n_batch = 1;
+      p.rot = h->d_rot + (size_t)(begin + i) * 9;
+      p.rec = h->d_rec + begin + i;
+      for (int stage = 0; stage < 6 && e == hipSuccess; stage++) e = launch_stage(stage, p, h->stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(&d[(size_t)i], p.rec, sizeof(DevRecord), hipMemcpyDeviceToHost, h->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(scr);
+    (void)hipFree(hdr);
+    if (e != hipSuccess) return fail(TOMO_E_HIP, std::string("overflow re-sweep: ") + hipGetErrorString(e));
+    std::vector<int32_t> still;
+    for (int32_t i : idx)
+      if (d[(size_t)i].flags & F_RETRY) still.push_back(i);
+    idx.swap(still);
+  }
+  return TOMO_OK;
+}
+
+void drop_orientations(tomo_handle *h) {
+  if (h->d_rec) (void)hipFree(h->d_rec);
+  if (h->d_rot) (void)hipFree(h->d_rot);
+  h->d_rec = nullptr;
+  h->d_rot = nullptr;
+  h->rec_cap = 0;
+  h->n_orient = 0;
+  h->rot.clear();
+}
+
 }  // namespace
 
 extern "C" {
@@ -547,12 +632,7 @@ int tomo_set_mesh(tomo_handle *h, int32_t n_tri, const int32_t *tri, int32_t n_v
     if (tri[i] < 0 || tri[i] >= n_vtx) return fail(TOMO_E_INVALID_ARG, "triangle index out of range");
   if (n_tri >= (1 << 30)) return fail(TOMO_E_INVALID_ARG, "too many triangles");
   HIP_TRY(hipStreamSynchronize(h->stream));
-  if (h->d_tidx) { HIP_TRY(hipFree(h->d_tidx)); h->d_tidx = nullptr; }
-  if (h->d_vaos) { HIP_TRY(hipFree(h->d_vaos)); h->d_vaos = nullptr; }
-  if (h->d_chunk) { HIP_TRY(hipFree(h->d_chunk)); h->d_chunk = nullptr; }
-  if (h->d_soa) { HIP_TRY(hipFree(h->d_soa)); h->d_soa = nullptr; }
-  h->n_tri = n_tri;
-  h->n_vtx = n_vtx;
+  drop_mesh(h);  // a failure below leaves the handle without a mesh (tomo_run: TOMO_E_STATE)
   // raw AABB centre (STomoAABB3Df.cpp:42-72): float add, then *0.5 in double
   float mn[3] = {1e5f, 1e5f, 1e5f}, mx[3] = {-1e5f, -1e5f, -1e5f};
   for (int32_t v = 0; v < n_vtx; v++)
@@ -565,8 +645,6 @@ int tomo_set_mesh(tomo_handle *h, int32_t n_tri, const int32_t *tri, int32_t n_v
   std::vector<float> soa((size_t)3 * (n_vtx > 0 ? n_vtx : 1));
   for (int32_t v = 0; v < n_vtx; v++)
     for (int k = 0; k < 3; k++) soa[(size_t)k * n_vtx + v] = vtx[(size_t)v * 3 + k];
-  HIP_TRY(hipMalloc(&h->d_soa, soa.size() * sizeof(float)));
-  HIP_TRY(hipMemcpy(h->d_soa, soa.data(), soa.size() * sizeof(float), hipMemcpyHostToDevice));
   // Culling order: triangles sorted by the Morton code of their centroid, so each run of kChunk
   // triangles is compact in space and its AABB bounds where the rasteriser can touch it.
   std::vector<uint64_t> code((size_t)n_tri);
@@ -601,9 +679,9 @@ int tomo_set_mesh(tomo_handle *h, int32_t n_tri, const int32_t *tri, int32_t n_v
     ord_id[(size_t)i] = t;
     for (int j = 0; j < 3; j++) ord_tri[(size_t)i * 3 + j] = tri[(size_t)t * 3 + j];
   }
-  h->n_chunks = (n_tri + kChunk - 1) / kChunk;
-  std::vector<float> box((size_t)6 * (h->n_chunks > 0 ? h->n_chunks : 1));
-  for (int32_t c = 0; c < h->n_chunks; c++) {
+  const int32_t n_chunks = (n_tri + kChunk - 1) / kChunk;
+  std::vector<float> box((size_t)6 * (n_chunks > 0 ? n_chunks : 1));
+  for (int32_t c = 0; c < n_chunks; c++) {
     float b[6] = {3.0e38f, 3.0e38f, 3.0e38f, -3.0e38f, -3.0e38f, -3.0e38f};
     for (int32_t i = c * kChunk; i < std::min(n_tri, (c + 1) * kChunk); i++)
       for (int j = 0; j < 3; j++)
@@ -618,21 +696,36 @@ int tomo_set_mesh(tomo_handle *h, int32_t n_tri, const int32_t *tri, int32_t n_v
   // caller's index) and two float4 per vertex (raw position and normal): 32 bytes per vertex plus
   // 16 per triangle stay resident in an XCD's L2 while every orientation reads them.
   std::vector<int32_t> tix((size_t)4 * (n_tri > 0 ? n_tri : 1), 0);
+  std::vector<int32_t> tpos((size_t)(n_tri > 0 ? n_tri : 1), 0);
   for (int32_t i = 0; i < n_tri; i++) {
     for (int j = 0; j < 3; j++) tix[(size_t)i * 4 + j] = ord_tri[(size_t)i * 3 + j];
     tix[(size_t)i * 4 + 3] = ord_id[(size_t)i];
+    tpos[(size_t)ord_id[(size_t)i]] = i;
   }
   std::vector<float> aos((size_t)8 * (n_vtx > 0 ? n_vtx : 1), 0.0f);
   for (int32_t v = 0; v < n_vtx; v++) {
     for (int k = 0; k < 3; k++) aos[(size_t)v * 8 + k] = vtx[(size_t)v * 3 + k];
     for (int k = 0; k < 3; k++) aos[(size_t)v * 8 + 3 + k] = vtx_nrm[(size_t)v * 3 + k];
   }
-  HIP_TRY(hipMalloc(&h->d_tidx, tix.size() * sizeof(int32_t)));
-  HIP_TRY(hipMemcpy(h->d_tidx, tix.data(), tix.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-  HIP_TRY(hipMalloc(&h->d_vaos, aos.size() * sizeof(float)));
-  HIP_TRY(hipMemcpy(h->d_vaos, aos.data(), aos.size() * sizeof(float), hipMemcpyHostToDevice));
-  HIP_TRY(hipMalloc(&h->d_chunk, sizeof(float) * box.size()));
-  HIP_TRY(hipMemcpy(h->d_chunk, box.data(), sizeof(float) * box.size(), hipMemcpyHostToDevice));
+  float *d_soa = nullptr, *d_chunk = nullptr;
+  int4 *d_tidx = nullptr;
+  int32_t *d_tpos = nullptr;
+  float4 *d_vaos = nullptr;
+  hipError_t e = upload(&d_soa, soa);
+  if (e == hipSuccess) e = upload(&d_tidx, tix);
+  if (e == hipSuccess) e = upload(&d_tpos, tpos);
+  if (e == hipSuccess) e = upload(&d_vaos, aos);
+  if (e == hipSuccess) e = upload(&d_chunk, box);
+  if (e != hipSuccess) {
+    (void)hipFree(d_soa); (void)hipFree(d_tidx); (void)hipFree(d_tpos);
+    (void)hipFree(d_vaos); (void)hipFree(d_chunk);
+    return fail(e == hipErrorOutOfMemory ? TOMO_E_OOM : TOMO_E_HIP,
+                std::string("mesh upload: ") + hipGetErrorString(e));
+  }
+  h->d_soa = d_soa; h->d_tidx = d_tidx; h->d_tpos = d_tpos; h->d_vaos = d_vaos; h->d_chunk = d_chunk;
+  h->n_tri = n_tri;
+  h->n_vtx = n_vtx;
+  h->n_chunks = n_chunks;
   return TOMO_OK;
 }
 
@@ -640,20 +733,26 @@ int tomo_set_orientations(tomo_handle *h, int32_t n, const float *ypr) {
   if (!h) return fail(TOMO_E_INVALID_ARG, "null handle");
   if (n < 0 || (n > 0 && !ypr)) return fail(TOMO_E_INVALID_ARG, "bad orientation arguments");
   HIP_TRY(hipStreamSynchronize(h->stream));
-  h->rot.assign((size_t)9 * (n > 0 ? n : 1), 0.0f);
+  h->n_orient = 0;  // a failure below leaves no orientations (tomo_run: range error)
+  std::vector<float> rot((size_t)9 * (n > 0 ? n : 1), 0.0f);
   for (int32_t i = 0; i < n; i++)
-    ypr_matrix(ypr[(size_t)i * 3 + 0], ypr[(size_t)i * 3 + 1], ypr[(size_t)i * 3 + 2], &h->rot[(size_t)i * 9]);
-  if (n > h->rec_cap) {
-    if (h->d_rec) { HIP_TRY(hipFree(h->d_rec)); h->d_rec = nullptr; }
-    if (h->d_rot) { HIP_TRY(hipFree(h->d_rot)); h->d_rot = nullptr; }
-    h->rec_cap = 0;
-    if (hipMalloc(&h->d_rec, sizeof(DevRecord) * (size_t)n) != hipSuccess ||
-        hipMalloc(&h->d_rot, sizeof(float) * 9 * (size_t)n) != hipSuccess)
+    ypr_matrix(ypr[(size_t)i * 3 + 0], ypr[(size_t)i * 3 + 1], ypr[(size_t)i * 3 + 2], &rot[(size_t)i * 9]);
+  const int32_t need = n > 0 ? n : 1;
+  if (need > h->rec_cap || !h->d_rec || !h->d_rot) {
+    drop_orientations(h);
+    if (hipMalloc(&h->d_rec, sizeof(DevRecord) * (size_t)need) != hipSuccess ||
+        hipMalloc(&h->d_rot, sizeof(float) * 9 * (size_t)need) != hipSuccess) {
+      (void)hipGetLastError();
+      drop_orientations(h);
       return fail(TOMO_E_OOM, "orientation buffers");
-    h->rec_cap = n;
+    }
+    h->rec_cap = need;
   }
-  if (n > 0)
-    HIP_TRY(hipMemcpy(h->d_rot, h->rot.data(), sizeof(float) * 9 * (size_t)n, hipMemcpyHostToDevice));
+  if (n > 0 && hipMemcpy(h->d_rot, rot.data(), sizeof(float) * 9 * (size_t)n, hipMemcpyHostToDevice) != hipSuccess) {
+    drop_orientations(h);
+    return fail(TOMO_E_HIP, "rotation upload failed");
+  }
+  h->rot.swap(rot);
   h->n_orient = n;
   return TOMO_OK;
 }
@@ -695,7 +794,8 @@ int tomo_set_tuning(tomo_handle *h, int32_t knob, int32_t value) {
 int tomo_run(tomo_handle *h, int32_t begin, int32_t end) {
   if (!h) return fail(TOMO_E_INVALID_ARG, "null handle");
   if (begin < 0 || end > h->n_orient || begin > end) return fail(TOMO_E_INVALID_ARG, "bad orientation range");
-  if (!h->d_soa || !h->d_tidx) return fail(TOMO_E_STATE, "no mesh");
+  if (!h->d_soa || !h->d_tidx || !h->d_tpos || !h->d_vaos || !h->d_chunk) return fail(TOMO_E_STATE, "no mesh");
+  if (!h->d_rot || !h->d_rec) return fail(TOMO_E_STATE, "no orientations");
   h->last_begin = begin;
   h->last_end = end;
   h->timed = false;
@@ -729,6 +829,8 @@ int tomo_get_records(tomo_handle *h, int32_t begin, int32_t end, tomo_record *ou
   std::vector<DevRecord> d((size_t)n);
   HIP_TRY(hipMemcpyAsync(d.data(), h->d_rec + begin, sizeof(DevRecord) * (size_t)n, hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
+  const int rc = retry_overflowed(h, begin, d);
+  if (rc) return rc;
   for (int32_t i = 0; i < n; i++) finish_record(h, d[(size_t)i], &h->rot[(size_t)(begin + i) * 9], out[i]);
   return TOMO_OK;
 }
@@ -764,36 +866,40 @@ static int debug_run(tomo_handle *h, const float ypr[3], tomo_record *rec, int32
   if (!h->d_soa || !h->d_tidx) return fail(TOMO_E_STATE, "no mesh");
   int rc = tomo_set_orientations(h, 1, ypr);
   if (rc) return rc;
-  rc = ensure_scratch(h, 1);
-  if (rc) return rc;
   const size_t g2 = (size_t)h->opt.grid * h->opt.grid;
   int32_t *d = nullptr;
   uint64_t *drows = nullptr;
   HIP_TRY(hipMalloc(&d, sizeof(int32_t) * 4 * g2));
-  hipError_t e = hipMemsetAsync(d, 0, sizeof(int32_t) * 4 * g2, h->stream);
-  if (e == hipSuccess && rows) {
-    e = hipMalloc(&drows, sizeof(uint64_t) * 64 * g2);
-    if (e == hipSuccess) e = hipMemsetAsync(drows, 0, sizeof(uint64_t) * 64 * g2, h->stream);
-  }
+  hipError_t e = rows ? hipMalloc(&drows, sizeof(uint64_t) * 64 * g2) : hipSuccess;
   const bool was = h->phases;
   h->phases = false;
-  if (e == hipSuccess) {
+  // The dumps must come from the pass whose record is reported: an orientation that overflows
+  // the pool or the spill list is dumped again with larger ones.
+  const ScratchLayout lay0 = h->lay;
+  for (int32_t scale = 1; scale <= 64 && e == hipSuccess && rc == TOMO_OK; scale *= 4) {
+    if (scale > 1) h->lay = make_layout(h->opt.grid, lay0.inl, scale);
+    rc = ensure_scratch(h, 1);
+    if (rc) break;
+    e = hipMemsetAsync(d, 0, sizeof(int32_t) * 4 * g2, h->stream);
+    if (e == hipSuccess && drows) e = hipMemsetAsync(drows, 0, sizeof(uint64_t) * 64 * g2, h->stream);
+    if (e != hipSuccess) break;
     rc = enqueue(h, 0, 1, 1, d, g2, drows);
-    if (rc) {
-      h->phases = was;
-      (void)hipFree(d);
-      if (drows) (void)hipFree(drows);
-      return rc;
-    }
-    e = hipStreamSynchronize(h->stream);
+    if (rc) break;
+    DevRecord dr;
+    e = hipMemcpyAsync(&dr, h->d_rec, sizeof(DevRecord), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess || !(dr.flags & F_RETRY)) break;
   }
+  h->lay = lay0;
   h->phases = was;
   int32_t *outs[4] = {cnt_pre, cnt_final, slot_vo, slot_vss};
-  for (int k = 0; k < 4 && e == hipSuccess; k++)
+  for (int k = 0; k < 4 && e == hipSuccess && rc == TOMO_OK; k++)
     if (outs[k]) e = hipMemcpy(outs[k], d + k * g2, sizeof(int32_t) * g2, hipMemcpyDeviceToHost);
-  if (e == hipSuccess && rows) e = hipMemcpy(rows, drows, sizeof(uint64_t) * 64 * g2, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && rc == TOMO_OK && rows)
+    e = hipMemcpy(rows, drows, sizeof(uint64_t) * 64 * g2, hipMemcpyDeviceToHost);
   (void)hipFree(d);
   if (drows) (void)hipFree(drows);
+  if (rc) return rc;
   if (e != hipSuccess) return fail(TOMO_E_HIP, std::string("debug sweep: ") + hipGetErrorString(e));
   h->last_begin = 0;
   h->last_end = 1;
@@ -843,10 +949,7 @@ int tomo_destroy(tomo_handle *h) {
   if (h->d_scratch) scratch_release(h->d_scratch, h->scratch_cap, h->device);
   if (h->d_rec) (void)hipFree(h->d_rec);
   if (h->d_rot) (void)hipFree(h->d_rot);
-  if (h->d_chunk) (void)hipFree(h->d_chunk);
-  if (h->d_soa) (void)hipFree(h->d_soa);
-  if (h->d_tidx) (void)hipFree(h->d_tidx);
-  if (h->d_vaos) (void)hipFree(h->d_vaos);
+  drop_mesh(h);
   if (h->d_bed_dist) (void)hipFree(h->d_bed_dist);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -972,7 +1075,7 @@ static float time_shape(tomo_handle *h, const tomo_batch::Tune &t, int32_t b0, i
   if (hipMemcpy(d.data(), h->d_rec + b0, sizeof(DevRecord) * (size_t)n, hipMemcpyDeviceToHost) != hipSuccess)
     return -1.0f;
   for (const DevRecord &r : d)
-    if (r.flags & TOMO_F_UNRESOLVED) return -1.0f;
+    if (r.flags & (TOMO_F_UNRESOLVED | F_RETRY)) return -1.0f;
   return ms < 0.0f ? -1.0f : ms * 1000.0f / (float)n;
 }
 
@@ -1057,6 +1160,22 @@ int tomo_batch_get_tuning(tomo_batch *b, int32_t member, int32_t *batch, int32_t
   if (tile_y) *tile_y = t.tile_y;
   if (inl) *inl = t.inl;
   if (us_per_orientation) *us_per_orientation = t.us_per_orientation;
+  return TOMO_OK;
+}
+
+int tomo_batch_set_tuning(tomo_batch *b, int32_t member, int32_t batch, int32_t tile_y, int32_t inl) {
+  if (!b || member < 0 || member >= (int32_t)b->members.size()) return fail(TOMO_E_INVALID_ARG, "bad member");
+  if (batch < 0) return fail(TOMO_E_INVALID_ARG, "batch must be >= 0");
+  if (tile_y != 0 && (tile_y < 32 || tile_y > 512 || (tile_y & (tile_y - 1))))
+    return fail(TOMO_E_INVALID_ARG, "tile extent along y must be 0 or a power of two in 32..512");
+  if (inl < kSmall || inl > kInlineMax) return fail(TOMO_E_INVALID_ARG, "inline slot capacity must be 8..32");
+  batch_unshare(b);
+  const int32_t n = b->members[(size_t)member]->n_orient;
+  tomo_batch::Tune t;
+  t.batch = batch > 0 ? std::min(batch, n) : std::min(n, 2048);
+  t.tile_y = tile_y;
+  t.inl = inl;
+  b->tune[(size_t)member] = t;
   return TOMO_OK;
 }
 

@@ -14,7 +14,10 @@ constexpr int kWaves = kThreads / 64;
 constexpr int kSmall = 8;              // slots with <= 8 entries are paired in registers
 constexpr int kInlineDefault = 16;     // entries stored inline per slot (layer-major); tunable
 constexpr int kInlineMax = 32;         // in [kSmall, kInlineMax]
-constexpr int kBigCap = 512;           // entries a single slot may hold for exact resolution
+constexpr int kBigCap = 512;           // entries of a slot sorted by one wave in pair_big_kernel
+constexpr int kHugeLds = 8192;         // entries of a slot sorted in LDS by pair_huge_kernel; more
+                                       // are selected in passes over global memory
+constexpr int kHugeList = 1024;        // one triangle's entries of one slot (pair_huge_kernel)
 constexpr int kMaxBedR = 16;           // int(BedOuterBound) supported by the bed kernel
 constexpr int kTileThreads = 768;      // rasteriser work-group: 12 waves
 constexpr int kTileWaves = kTileThreads / 64;
@@ -46,18 +49,19 @@ struct OrientHdr {
   float kx, ky, kz;
   unsigned long long zabs;  // sum of |z| over emitted entries: bounds every sum of |slot value|
   uint32_t spill_top;       // entries whose slot lies outside the tile that produced them
-  int32_t pad[3];
+  uint32_t huge_top;        // slots with more than kBigCap entries (pair_huge_kernel's list)
+  int32_t pad[2];
 };
 
 // Per in-flight orientation scratch, carved from one device allocation.  G2 = grid * grid.
 struct ScratchLayout {
   int64_t count, ent, bedinfo, big, off, cursor, pool_slot, pool_key, pool_rank, cmp_key,
-      cmp_rank, sv, spill_slot, spill_key, spill_rank;  // byte offsets inside one orientation's region
+      cmp_rank, sv, spill_slot, spill_key, spill_rank, huge;  // byte offsets inside one orientation's region
   int64_t bytes; // bytes per region
   int32_t pool;  // overflow pool capacity (entries)
   int32_t inl;   // inline entries per slot
   int32_t spill; // spill list capacity (entries)
-  int32_t pad_;
+  int32_t nhuge; // huge-slot list capacity (slots)
 };
 
 struct SweepParams {
@@ -65,6 +69,7 @@ struct SweepParams {
   const float *chunk_box;   // [n_chunks][6] raw-coordinate AABB (min xyz, max xyz) of kChunk triangles
   const float *vx, *vy, *vz;  // [n_vtx] raw vertex positions (structure of arrays)
   const int4 *tidx;         // [n_tri] vertex indices i0, i1, i2 and the caller's triangle index
+  const int32_t *tpos;      // [n_tri] position in tidx of the caller's triangle t
   const float4 *vaos;       // [n_vtx][2] raw (x, y, z, nx), (ny, nz, 0, 0)
   int32_t n_tri, n_vtx, n_chunks;
   float cx, cy, cz;        // raw AABB centre (STomoSh_Base.cpp:75-82)

@@ -71,9 +71,11 @@ struct Tables {
   int32_t *big;
   int32_t *sv;  // [6][G2] per-slot Vo, Vss, Va, Vb, Vnv, Vtc (kept only when totals may round)
   uint32_t *spill_slot, *spill_key, *spill_rank;
+  int32_t *huge;  // slots with more than kBigCap entries
   int32_t pool;
   int32_t inl;    // inline entries per slot (>= kSmall)
   int32_t spill;  // spill list capacity
+  int32_t nhuge;  // huge list capacity
 };
 
 struct Acc {
@@ -348,37 +350,196 @@ __device__ void wave_pair(uint32_t key, uint64_t V, int lane, const PairParams &
   }
 }
 
-// One slot with more than kSmall entries, handled by a whole wave.  Restores SetBit's exact
-// kept set: the first 63 distinct keys in insertion order, insertion order being (triangle,
-// main-before-sub) = ascending rank (STomoVoxelSpaceInfo.cpp:200-220).
-__device__ void big_slot(int32_t s, uint32_t c, const Tables &t, int32_t G2, uint64_t *buf,
-                         int lane, const PairParams &pp, SlotResult &r, uint64_t *rows) {
-  int32_t fl = 0;
-  if (c > (uint32_t)kBigCap) {
-    fl |= F_UNRESOLVED;
-    c = kBigCap;
+constexpr uint32_t kNoSeq = 0xffffffffu;
+
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint32_t lo = __shfl_xor((uint32_t)v, o), hi = __shfl_xor((uint32_t)(v >> 32), o);
+    const uint64_t w = ((uint64_t)hi << 32) | lo;
+    v = w < v ? w : v;
   }
+  return v;
+}
+
+// Entry i of slot s (inline layer or its compacted pool run starting at off).
+__device__ __forceinline__ void slot_entry(const Tables &t, int32_t G2, int32_t s, uint32_t off, uint32_t i,
+                                           uint32_t &key, uint32_t &rank, int32_t &fl) {
+  if (i < (uint32_t)t.inl) {
+    const uint2 e = t.ent[(int64_t)i * G2 + s];
+    key = e.x;
+    rank = e.y;
+  } else {
+    const uint32_t j = off + (i - (uint32_t)t.inl);
+    if (j < (uint32_t)t.pool) {
+      key = t.cmp_key[j];
+      rank = t.cmp_rank[j];
+    } else {  // the pool overflowed: the host sweeps this orientation again with a larger one
+      key = 0u;
+      rank = 0xffffffffu;
+      fl |= F_RETRY;
+    }
+  }
+}
+
+// Insertion order of one triangle's rows in one slot.  triVoxel walks a triangle's pixel box x
+// outer, y inner and appends each accepted pixel's row and then its shell sub-pixel row
+// (STomoSh_INT3.cpp:1163-1186); SetBit keeps a slot's first 63 distinct keys in that order
+// (STomoVoxelSpaceInfo.cpp:200-220).  Entries carry the rank 2 * triangle + sub, which orders
+// the rows of different triangles.  Two pixels of one triangle can land in one slot (the
+// interpolated point of a sliver can leave the pixel it was tested at); their order is found
+// here by walking the triangle's box again with the rasteriser's arithmetic, the wave testing
+// 64 consecutive pixels per step.  For this lane's key gk (if gv) returns the first sequence
+// number 2 * (pixel index in the box, x-major) + sub at which the caller's triangle t appends
+// gk to slot s of orientation b, or kNoSeq if it never does (its rows are then its zero-hit
+// centroid rows, which the rank already orders main before sub).
+__device__ uint32_t tri_key_order(const SweepParams &p, int32_t b, uint32_t t, int32_t s, uint32_t gk, bool gv,
+                                  int lane) {
+  const OrientHdr &h = p.hdr[b];
+  const float kx = h.kx, ky = h.ky, kz = h.kz;
+  const float *Rg = p.rot + (int64_t)b * 9;
+  const float R0 = Rg[0], R1 = Rg[1], R2 = Rg[2], R3 = Rg[3], R4 = Rg[4], R5 = Rg[5], R6 = Rg[6],
+              R7 = Rg[7], R8 = Rg[8];
+  const int4 ix = p.tidx[p.tpos[t]];
+  const int32_t ids[3] = {ix.x, ix.y, ix.z};
+  float m[3][3], nzv[3];
+#pragma unroll
+  for (int q = 0; q < 3; q++) {  // as raster_kernel: rotate, place, then the margin
+    const float4 va = p.vaos[2 * ids[q]], vb = p.vaos[2 * ids[q] + 1];
+    const float t0 = va.x - p.cx, t1 = va.y - p.cy, t2 = va.z - p.cz;
+    m[q][0] = (dot_row(R0, R1, R2, t0, t1, t2) - kx) + kEps;
+    m[q][1] = (dot_row(R3, R4, R5, t0, t1, t2) - ky) + kEps;
+    m[q][2] = (dot_row(R6, R7, R8, t0, t1, t2) - kz) + kEps;
+    nzv[q] = dot_row(R6, R7, R8, va.w, vb.x, vb.y);
+  }
+  const float v0x = m[0][0], v0y = m[0][1], v1x = m[1][0], v1y = m[1][1], v2x = m[2][0], v2y = m[2][1];
+  float mnx = (v0x <= v1x) ? v0x : v1x; mnx = (mnx <= v2x) ? mnx : v2x;
+  float mny = (v0y <= v1y) ? v0y : v1y; mny = (mny <= v2y) ? mny : v2y;
+  float mxx = (v0x >= v1x) ? v0x : v1x; mxx = (mxx >= v2x) ? mxx : v2x;
+  float mxy = (v0y >= v1y) ? v0y : v1y; mxy = (mxy >= v2y) ? mxy : v2y;
+  const int32_t G = p.grid, lo = -64, hi = G + 64;
+  const int32_t x0 = max((int32_t)mnx, lo), y0 = max((int32_t)mny, lo);
+  const int32_t x1 = min((int32_t)mxx, hi), y1 = min((int32_t)mxy, hi);
+  const TriSetup ts = tri_setup(v0x, v0y, v1x, v1y, v2x, v2y);
+  uint32_t best = kNoSeq;
+  if (tri_degenerate(ts) || x1 < x0 || y1 < y0) return best;
+  const int32_t hgt = y1 - y0 + 1;
+  const int64_t npx = (int64_t)(x1 - x0 + 1) * hgt;
+  for (int64_t i0 = 0; i0 < npx; i0 += 64) {
+    const int64_t i = i0 + lane;
+    bool hit = false;
+    uint32_t k1 = 0u, k2 = 0u;
+    if (i < npx) {
+      const int32_t x = x0 + (int32_t)(i / hgt), y = y0 + (int32_t)(i % hgt);
+      float nv, nw, u, v, w;
+      bary_num(ts, ((float)x + kEps) + p.h2, ((float)y + kEps) + p.h2, nv, nw);
+      if (bary_exact(ts.den, nv, nw, u, v, w)) {
+        const float px = m[0][0] * u + m[1][0] * v + m[2][0] * w;
+        const float py = m[0][1] * u + m[1][1] * v + m[2][1] * w;
+        const float pz = m[0][2] * u + m[1][2] * v + m[2][2] * w;
+        const float nz = nzv[0] * u + nzv[1] * v + nzv[2] * w;
+        const int32_t sx = (int32_t)px, sy = (int32_t)py;
+        const bool in = px > -2147483648.0f && px < 2147483648.0f && py > -2147483648.0f &&
+                        py < 2147483648.0f && sx >= 0 && sy >= 0 && sx < G && sy < G;
+        if (in && sx * G + sy == s) {
+          int32_t fl = 0;
+          hit = true;
+          k1 = make_key(to_i16(pz, fl), to_i16(nz * kNormFactor, fl));
+          if (p.do_shell) {
+            const float pz2 = pz + ((nz >= 0.0f) ? -p.shell_off : p.shell_off);
+            const float nz2 = -nz;
+            k2 = make_key(to_i16(pz2, fl), to_i16(nz2 * kNormFactor, fl));
+          }
+        }
+      }
+    }
+    uint64_t mh = __ballot(hit);
+    while (mh) {
+      const int j = __ffsll((unsigned long long)mh) - 1;
+      mh &= mh - 1ull;
+      const uint32_t ka = __shfl(k1, j), kb = __shfl(k2, j);
+      const uint32_t seq = 2u * (uint32_t)(i0 + j);
+      if (gv && gk == ka) best = min(best, seq);
+      if (p.do_shell && gv && gk == kb) best = min(best, seq + 1u);
+    }
+  }
+  return best;
+}
+
+// Of the distinct keys buf[a..e) (low words), all first appended to slot s by the caller's
+// triangle T and listed in ascending rank order, the `need` that T appends first: lane
+// lane0 + k receives the k-th of them in mykey.  Orders by (tri_key_order, list position), so
+// keys T appends only as centroid rows keep their rank order.  Overwrites the high words.
+__device__ void pick_first_of_triangle(const SweepParams &p, int32_t b, int32_t s, uint32_t T, uint64_t *buf,
+                                       int a, int e, int need, int lane0, int lane, uint32_t &mykey) {
+  for (int g0 = a; g0 < e; g0 += 64) {
+    const int gi = g0 + lane;
+    const bool gv = gi < e;
+    const uint32_t gk = gv ? (uint32_t)buf[gi] : 0u;
+    const uint32_t seq = tri_key_order(p, b, T, s, gk, gv, lane);
+    wave_sync();
+    if (gv) buf[gi] = ((uint64_t)seq << 32) | gk;
+  }
+  wave_sync();
+  uint64_t last = 0ull;
+  for (int it = 0; it < need; it++) {
+    uint64_t best = ~0ull;
+    for (int g0 = a; g0 < e; g0 += 64) {
+      const int gi = g0 + lane;
+      if (gi < e) {
+        const uint64_t c = (buf[gi] & 0xffffffff00000000ull) | (uint32_t)gi;
+        if ((it == 0 || c > last) && c < best) best = c;
+      }
+    }
+    best = wave_min_u64(best);
+    if (best == ~0ull) break;  // callers ask for fewer keys than the list holds
+    last = best;
+    const uint32_t wk = (uint32_t)buf[(uint32_t)best];
+    if (lane == lane0 + it) mykey = wk;
+  }
+  wave_sync();
+}
+
+// The 63 keys SetBit keeps of a slot with D > 63 distinct keys, from buf[0..D) holding each
+// key's smallest rank as (rank << 32) | key in ascending order: the first 63, except that when
+// the 63rd and 64th belong to one triangle its keys are ordered by pick_first_of_triangle.
+// Returns the kept key of lanes 0..62 (0 in lane 63); may overwrite buf.
+__device__ uint32_t cap_select(const SweepParams &p, int32_t b, int32_t s, uint64_t *buf, int D, int lane) {
+  uint32_t mykey = lane < kMaxPix ? (uint32_t)buf[lane] : 0u;
+  const uint32_t t62 = (uint32_t)(buf[kMaxPix - 1] >> 33), t63 = (uint32_t)(buf[kMaxPix] >> 33);
+  if (t62 != t63) return mykey;
+  // the triangle's keys occupy positions [a, e), a <= 62 < 63 < e
+  const int a = __ffsll((unsigned long long)__ballot((uint32_t)(buf[lane] >> 33) == t62)) - 1;
+  int e = 64;
+  for (;;) {
+    const int i = e + lane;
+    const uint64_t outside = __ballot(!(i < D && (uint32_t)(buf[i < D ? i : 0] >> 33) == t62));
+    if (outside) {
+      e += __ffsll((unsigned long long)outside) - 1;
+      break;
+    }
+    e += 64;
+  }
+  wave_sync();
+  pick_first_of_triangle(p, b, s, t62, buf, a, e, kMaxPix - a, a, lane, mykey);
+  return mykey;
+}
+
+// One slot with kSmall < c <= kBigCap entries, handled by a whole wave.  Restores SetBit's kept
+// set: the first 63 distinct keys in insertion order (triangle, then pixel, then main before
+// sub; STomoVoxelSpaceInfo.cpp:200-220).  Returns false, leaving r unset, when the 63rd and
+// 64th keys come from one triangle: their order needs cap_select, and the slot goes to
+// pair_huge_kernel (this kernel's register budget stays that of its common path).
+__device__ bool big_slot(const SweepParams &p, int32_t b, int32_t s, uint32_t c, const Tables &t, int32_t G2,
+                         uint64_t *buf, int lane, SlotResult &r, uint64_t *rows) {
+  int32_t fl = 0;
   const uint32_t off = (c > (uint32_t)t.inl) ? t.off[s] : 0u;
   const int P = pow2_at_least((int)c);
   for (int i = lane; i < P; i += 64) {
     uint64_t v = 0;
     if (i < (int)c) {
       uint32_t key, rank;
-      if (i < t.inl) {
-        const uint2 e = t.ent[(int64_t)i * G2 + s];
-        key = e.x;
-        rank = e.y;
-      } else {
-        uint32_t j = off + (uint32_t)(i - t.inl);
-        if (j < (uint32_t)t.pool) {
-          key = t.cmp_key[j];
-          rank = t.cmp_rank[j];
-        } else {
-          key = 0u;
-          rank = 0xffffffffu;
-          fl |= F_UNRESOLVED;
-        }
-      }
+      slot_entry(t, G2, s, off, (uint32_t)i, key, rank, fl);
       // descending (key, then ascending rank)
       v = ((uint64_t)key << 32) | (uint64_t)(~rank);
     }
@@ -408,17 +569,20 @@ __device__ void big_slot(int32_t s, uint32_t c, const Tables &t, int32_t G2, uin
     D += __popcll(m);
   }
   uint32_t mykey = 0;
+  bool tie = false;
   if (D > kMaxPix) {
     // More than 63 distinct keys: SetBit keeps the 63 that arrived first.
     fl |= F_CAP_DROP;
     const int P2 = pow2_at_least(D);
     for (int i = D + lane; i < P2; i += 64) buf[i] = ~0ull;
     wave_sync();
-    // ascending by rank == descending by complement
+    // ascending by (rank, key) == descending by complement
     for (int i = lane; i < P2; i += 64) buf[i] = ~buf[i];
     wave_sync();
     wave_sort_desc(buf, P2, lane);
-    uint64_t v = (lane < kMaxPix) ? ~buf[lane] : 0ull;
+    // the 63rd and 64th by rank are ~buf[62] and ~buf[63]
+    tie = (uint32_t)(~buf[kMaxPix - 1] >> 33) == (uint32_t)(~buf[kMaxPix] >> 33) && !(fl & F_RETRY);
+    const uint64_t v = (lane < kMaxPix) ? ~buf[lane] : 0ull;
     wave_sync();
     buf[lane] = (lane < kMaxPix) ? (uint64_t)(uint32_t)v : 0ull;  // keys only, pad 0
     wave_sync();
@@ -429,8 +593,9 @@ __device__ void big_slot(int32_t s, uint32_t c, const Tables &t, int32_t G2, uin
     mykey = (lane < D) ? (uint32_t)buf[lane] : 0u;
   }
   wave_sync();
-  wave_pair(mykey, (D >= 64) ? ~0ull : ((1ull << D) - 1ull), lane, pp, r, rows);
+  wave_pair(mykey, (D >= 64) ? ~0ull : ((1ull << D) - 1ull), lane, p.pp, r, rows);
   r.flags |= fl;
+  return !tie;
 }
 
 // A slot with kSmall < c < W entries in a W-lane segment (W = 64: the whole wave; W = 32: two
@@ -490,7 +655,9 @@ __device__ __forceinline__ Tables tables_for(const SweepParams &p, int32_t b) {
   t.spill_slot = (uint32_t *)(base + p.lay.spill_slot);
   t.spill_key = (uint32_t *)(base + p.lay.spill_key);
   t.spill_rank = (uint32_t *)(base + p.lay.spill_rank);
+  t.huge = (int32_t *)(base + p.lay.huge);
   t.pool = p.lay.pool;
+  t.nhuge = p.lay.nhuge;
   t.inl = p.lay.inl;
   t.spill = p.lay.spill;
   return t;
@@ -535,9 +702,9 @@ __device__ __forceinline__ int32_t group_or(int32_t f, int32_t *red, int lane, i
 }  // namespace
 
 // ---- aabb_kernel: one work-group per orientation ------------------------------------------
-// Rotates every vertex and reduces the rotated AABB minimum
-// (STomoSh_Base.cpp:77-109); resets the orientation's bookkeeping, and the
-// counters of the previous orientation's large slots that pair_kernel left set.
+// Rotates every vertex and reduces the rotated AABB minimum (STomoSh_Base.cpp:77-109) and
+// resets the orientation's header and record.  The slot counters need no reset: the rasteriser
+// writes every counter of its tile, zeros included.
 __global__ __launch_bounds__(kThreads) void aabb_kernel(SweepParams p) {
   __shared__ float red[kWaves][3];
   const int tid = threadIdx.x, lane = tid & 63, wave = wave_index();
@@ -576,7 +743,7 @@ __global__ __launch_bounds__(kThreads) void aabb_kernel(SweepParams p) {
     }
     const float kx = mx - p.bed_out, ky = my - p.bed_out, kz = mz;
     h.bbox[0] = p.grid; h.bbox[1] = -1; h.bbox[2] = p.grid; h.bbox[3] = -1;
-    h.big_top = 0u; h.pool_top = 0u; h.flags = 0; h.spill_top = 0u;
+    h.big_top = 0u; h.pool_top = 0u; h.flags = 0; h.spill_top = 0u; h.huge_top = 0u;
     h.kx = kx; h.ky = ky; h.kz = kz;
     h.zabs = 0ull;
     DevRecord &R = p.rec[b];
@@ -834,14 +1001,12 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6)
               const float py = m[0][1] * u + m[1][1] * v + m[2][1] * w;
               const float pz = m[0][2] * u + m[1][2] * v + m[2][2] * w;
               const float nz = rec[TR_NZ + 0] * u + rec[TR_NZ + 1] * v + rec[TR_NZ + 2] * w;
-              int32_t s;
               if (p.do_shell) {  // with its shell sub-pixel (STomoSh_INT3.cpp:1179-1185)
                 const float pz2 = pz + ((nz >= 0.0f) ? -p.shell_off : p.shell_off);
-                s = em.put2(px, py, pz, nz, pz2, -nz, rank);
+                em.put2(px, py, pz, nz, pz2, -nz, rank);
               } else {
-                s = em.put(px, py, pz, nz, rank);
+                em.put(px, py, pz, nz, rank);
               }
-              if (s >= 0 && s != px_i * G + py_i) em.flags |= F_RANK_TIE;
             }
           }
         }
@@ -1101,7 +1266,7 @@ __device__ __forceinline__ void pair_body(const SweepParams &p) {
   const int32_t nwaves = groups * kWaves;
   if (PART == 0 && tile == 0 && tid == 0) {
     // rasteriser flags, and pool or spill list overflow
-    const int32_t f0 = h.flags | (npool > (uint32_t)p.lay.pool || h.spill_top > (uint32_t)p.lay.spill ? F_UNRESOLVED : 0);
+    const int32_t f0 = h.flags | (npool > (uint32_t)p.lay.pool || h.spill_top > (uint32_t)p.lay.spill ? F_RETRY : 0);
     if (f0) atomicOr(&p.rec[b].flags, f0);
   }
   if (PART == 0 ? i0 >= nbb : (uint32_t)(tile * kWaves) >= nbig) return;  // nothing for this group
@@ -1284,6 +1449,12 @@ __device__ __forceinline__ void pair_body(const SweepParams &p) {
       if (p.dbg_final) p.dbg_final[s] = r.n;
     }
   };
+  // A slot left to pair_huge_kernel (see big_slot).
+  auto defer = [&](int32_t s) {
+    const uint32_t q = atomicAdd(&p.hdr[b].huge_top, 1u);
+    if (q < (uint32_t)t.nhuge) t.huge[q] = s;
+    else acc.flags |= F_RETRY;  // only when the pool overflowed as well
+  };
   const bool pool_ok = npool <= (uint32_t)t.pool;
   for (uint32_t j0 = (uint32_t)(tile * kWaves + wave); j0 < nbig; j0 += (uint32_t)nwaves * 64u) {
     const uint32_t j = j0 + (uint32_t)nwaves * (uint32_t)lane;
@@ -1319,12 +1490,18 @@ __device__ __forceinline__ void pair_body(const SweepParams &p) {
       m64 &= m64 - 1ull;
       const int32_t s = __shfl(ls, la);
       const uint32_t c = __shfl(lc, la);
+      if (c > (uint32_t)kBigCap) {
+        if (lane == 0) defer(s);
+        continue;
+      }
       uint64_t *const rows = p.dbg_rows ? p.dbg_rows + (int64_t)s * 64 : nullptr;
       SlotResult r;
-      if (c < 64u && (c <= (uint32_t)t.inl || pool_ok))
+      if (c < 64u && (c <= (uint32_t)t.inl || pool_ok)) {
         mid_slot<64>(s, c, t, G2, lane, p.pp, r, rows);
-      else
-        big_slot(s, c, t, G2, sortbuf[wave], lane, p.pp, r, rows);
+      } else if (!big_slot(p, b, s, c, t, G2, sortbuf[wave], lane, r, rows)) {
+        if (lane == 0) defer(s);
+        continue;
+      }
       if (lane == 0) finish_slot(s, r);
     }
   }
@@ -1344,6 +1521,292 @@ __global__ __launch_bounds__(kThreads, 5) void pair_kernel(SweepParams p) { pair
 // memory far more than it computes.
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void pair_big_kernel(SweepParams p) {
   pair_body<1>(p);
+}
+
+// ---- pair_huge_kernel: one work-group per orientation -------------------------------------
+// Pairs the slots pair_big_kernel listed: those with more than kBigCap entries (rows from many
+// triangles over one pixel: vertical walls of finely meshed parts put thousands there) and those
+// whose 63rd and 64th distinct keys come from one triangle (cap_select orders them).  Up to
+// kHugeLds entries are sorted in LDS by the whole work-group and the first entry of each key
+// keeps the key's smallest rank, as in big_slot; beyond that the kept keys are chosen in passes
+// over the entries in global memory, one triangle (the one with the smallest rank not yet kept)
+// per pass.  Either way the 63-row cap keeps SetBit's first 63 distinct keys
+// (STomoVoxelSpaceInfo.cpp:200-220), with one triangle's keys ordered by pick_first_of_triangle.
+// Most orientations have no such slot and return at once.
+__device__ void group_sort_desc(uint64_t *buf, int P, int tid) {
+  for (int size = 2; size <= P; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int i = tid; i < (P >> 1); i += kThreads) {
+        const int lo = 2 * i - (i & (stride - 1));
+        const int hi = lo + stride;
+        const uint64_t a = buf[lo], c = buf[hi];
+        const bool sw = ((lo & size) == 0) ? (a < c) : (a > c);
+        if (sw) {
+          buf[lo] = c;
+          buf[hi] = a;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Exclusive count of `flag` over the work-group's threads before this one; *total gets the sum.
+__device__ __forceinline__ uint32_t group_scan(bool flag, uint32_t *wcnt, int lane, int wave, uint32_t &total) {
+  const uint64_t m = __ballot(flag);
+  if (lane == 0) wcnt[wave] = (uint32_t)__popcll(m);
+  __syncthreads();
+  uint32_t before = 0u;
+  total = 0u;
+#pragma unroll
+  for (int w = 0; w < kWaves; w++) {
+    const uint32_t c = wcnt[w];
+    before += w < wave ? c : 0u;
+    total += c;
+  }
+  __syncthreads();
+  return before + lanes_below(m);
+}
+
+__device__ __forceinline__ uint32_t group_min(uint32_t v, uint32_t *wv, int lane, int wave) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor(v, o));
+  if (lane == 0) wv[wave] = v;
+  __syncthreads();
+  uint32_t r = wv[0];
+#pragma unroll
+  for (int w = 1; w < kWaves; w++) r = min(r, wv[w]);
+  __syncthreads();
+  return r;
+}
+
+// key in the ascending list sel[0..n)
+__device__ __forceinline__ bool in_sorted(const uint32_t *sel, uint32_t n, uint32_t key) {
+  uint32_t lo = 0u, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (sel[mid] < key) lo = mid + 1u;
+    else hi = mid;
+  }
+  return lo < n && sel[lo] == key;
+}
+
+// Wave: buf[0..n) = (key << 32 | ~rank) entries (n <= P, P a power of two >= 64): distinct keys
+// with their smallest rank, as (rank << 32) | key in ascending (rank, key) order, in buf[0..D).
+__device__ int wave_distinct_by_rank(uint64_t *buf, int n, int lane) {
+  const int P = pow2_at_least(n);
+  for (int i = n + lane; i < P; i += 64) buf[i] = 0ull;
+  wave_sync();
+  wave_sort_desc(buf, P, lane);
+  int D = 0;
+  uint32_t carry = 0u;
+  for (int base = 0; base < n; base += 64) {
+    const int i = base + lane;
+    const uint64_t v = (i < n) ? buf[i] : 0ull;
+    const uint32_t vkey = (uint32_t)(v >> 32);
+    uint32_t pk = __shfl_up(vkey, 1);
+    if (lane == 0) pk = carry;
+    carry = __shfl(vkey, 63);
+    const bool first = (i < n) && (i == 0 || vkey != pk);
+    const uint64_t m = __ballot(first);
+    const int pos = D + (int)lanes_below(m);
+    wave_sync();
+    if (first) buf[pos] = ((uint64_t)(~(uint32_t)v) << 32) | vkey;
+    wave_sync();
+    D += __popcll(m);
+  }
+  const int P2 = pow2_at_least(D);
+  for (int i = D + lane; i < P2; i += 64) buf[i] = ~0ull;
+  wave_sync();
+  for (int i = lane; i < P2; i += 64) buf[i] = ~buf[i];
+  wave_sync();
+  wave_sort_desc(buf, P2, lane);
+  for (int i = lane; i < P2; i += 64) buf[i] = ~buf[i];
+  wave_sync();
+  return D;
+}
+
+__global__ __launch_bounds__(kThreads) void pair_huge_kernel(SweepParams p) {
+  __shared__ uint64_t hbuf[kHugeLds];
+  __shared__ uint32_t wcnt[kWaves];
+  __shared__ uint32_t sel[64];  // kept keys, ascending (selection in passes)
+  __shared__ uint32_t lcount, nsel;
+  __shared__ int32_t capdrop;
+  __shared__ int64_t red[kWaves][18];
+  __shared__ int32_t fred[kWaves];
+  const int tid = threadIdx.x, lane = tid & 63, wave = wave_index();
+  const int32_t b = blockIdx.x;
+  if (b >= p.n_batch) return;
+  const OrientHdr &h = p.hdr[b];
+  const uint32_t nh = min(h.huge_top, (uint32_t)p.lay.nhuge);
+  if (nh == 0u) return;
+  const Tables t = tables_for(p, b);
+  const int32_t G2 = p.grid * p.grid;
+  const bool final_here = !p.bed_on;
+  int32_t *const sv = (p.exact_fsum && totals_may_round(h, p.pp)) ? t.sv : nullptr;
+  Acc acc = {};
+  for (uint32_t k = 0; k < nh; k++) {
+    const int32_t s = t.huge[k];
+    const uint32_t c = t.count[s];
+    const uint32_t off = t.off[s];  // c > 63 > inline capacity: the slot has a pool run
+    int32_t fl = 0;
+    uint32_t mykey = 0u;  // wave 0: the kept keys, descending from lane 0
+    int D = 0;
+    if (c <= (uint32_t)kHugeLds) {
+      const int P = pow2_at_least((int)c);
+      for (int i = tid; i < P; i += kThreads) {
+        uint64_t v = 0ull;
+        if (i < (int)c) {
+          uint32_t key, rank;
+          slot_entry(t, G2, s, off, (uint32_t)i, key, rank, fl);
+          v = ((uint64_t)key << 32) | (uint64_t)(~rank);
+        }
+        hbuf[i] = v;
+      }
+      __syncthreads();
+      group_sort_desc(hbuf, P, tid);
+      // first entry of each key (its smallest rank), compacted in place as (rank << 32) | key
+      uint32_t nd = 0u, carry = 0u;
+      for (int base = 0; base < (int)c; base += kThreads) {
+        const int i = base + tid;
+        const uint64_t v = i < (int)c ? hbuf[i] : 0ull;
+        const uint32_t vkey = (uint32_t)(v >> 32);
+        const uint32_t pk = tid == 0 ? carry : (uint32_t)(hbuf[i - 1] >> 32);
+        const uint32_t next_carry = (uint32_t)(hbuf[min(base + kThreads, (int)c) - 1] >> 32);
+        const bool first = i < (int)c && (i == 0 || vkey != pk);
+        uint32_t total;
+        const uint32_t pos = nd + group_scan(first, wcnt, lane, wave, total);
+        if (first) hbuf[pos] = ((uint64_t)(~(uint32_t)v) << 32) | vkey;
+        __syncthreads();
+        carry = next_carry;
+        nd += total;
+      }
+      D = (int)nd;
+      if (D > kMaxPix) {
+        fl |= F_CAP_DROP;
+        const int P2 = pow2_at_least(D);
+        for (int i = D + tid; i < P2; i += kThreads) hbuf[i] = ~0ull;
+        __syncthreads();
+        for (int i = tid; i < P2; i += kThreads) hbuf[i] = ~hbuf[i];
+        __syncthreads();
+        group_sort_desc(hbuf, P2, tid);
+        for (int i = tid; i < P2; i += kThreads) hbuf[i] = ~hbuf[i];
+        __syncthreads();
+        if (wave == 0) {
+          const uint32_t kept = cap_select(p, b, s, hbuf, D, lane);
+          wave_sync();
+          hbuf[lane] = kept;
+          wave_sync();
+          wave_sort_desc(hbuf, 64, lane);
+          mykey = (uint32_t)hbuf[lane];
+        }
+        D = kMaxPix;
+      } else if (wave == 0) {
+        mykey = lane < D ? (uint32_t)hbuf[lane] : 0u;
+      }
+    } else {
+      // Passes over global memory: each adds the not yet kept keys of the triangle holding the
+      // smallest rank among entries whose key is not kept.
+      if (tid == 0) {
+        nsel = 0u;
+        capdrop = 0;
+      }
+      __syncthreads();
+      for (;;) {
+        const uint32_t ns = nsel;
+        uint32_t rmin = 0xffffffffu;
+        for (uint32_t i = tid; i < c; i += kThreads) {
+          uint32_t key, rank;
+          slot_entry(t, G2, s, off, i, key, rank, fl);
+          if (!in_sorted(sel, ns, key)) rmin = min(rmin, rank);
+        }
+        rmin = group_min(rmin, wcnt, lane, wave);
+        if (rmin == 0xffffffffu) break;
+        if (ns == (uint32_t)kMaxPix) {
+          if (tid == 0) capdrop = 1;
+          break;
+        }
+        const uint32_t T = rmin >> 1;
+        if (tid == 0) lcount = 0u;
+        __syncthreads();
+        for (uint32_t i = tid; i < c; i += kThreads) {
+          uint32_t key, rank;
+          slot_entry(t, G2, s, off, i, key, rank, fl);
+          if ((rank >> 1) == T && !in_sorted(sel, ns, key)) {
+            const uint32_t q = atomicAdd(&lcount, 1u);
+            if (q < (uint32_t)kHugeList) hbuf[q] = ((uint64_t)key << 32) | (uint64_t)(~rank);
+          }
+        }
+        __syncthreads();
+        if (wave == 0) {
+          uint32_t nl = lcount;
+          if (nl > (uint32_t)kHugeList) {
+            fl |= F_UNRESOLVED;  // one triangle with more than kHugeList rows in one slot
+            nl = kHugeList;
+          }
+          const int u = wave_distinct_by_rank(hbuf, (int)nl, lane);
+          const int room = kMaxPix - (int)ns;
+          uint32_t add = 0u;
+          if (u <= room) {
+            add = lane < u ? (uint32_t)hbuf[lane] : 0u;
+          } else {
+            pick_first_of_triangle(p, b, s, T, hbuf, 0, u, room, 0, lane, add);
+            if (lane == 0) capdrop = 1;
+          }
+          const int na = u < room ? u : room;
+          if (lane < na) sel[ns + lane] = add;
+          wave_sync();
+          // keep sel ascending: one lane inserts the new keys (at most 63 in all)
+          if (lane == 0) {
+            for (int q = (int)ns; q < (int)ns + na; q++) {
+              const uint32_t v = sel[q];
+              int j = q - 1;
+              while (j >= 0 && sel[j] > v) {
+                sel[j + 1] = sel[j];
+                j--;
+              }
+              sel[j + 1] = v;
+            }
+            nsel = ns + (uint32_t)na;
+          }
+          wave_sync();
+        }
+        __syncthreads();
+        if (capdrop) break;
+      }
+      __syncthreads();
+      const uint32_t ns = nsel;
+      D = (int)ns;
+      if (capdrop) fl |= F_CAP_DROP;
+      if (wave == 0) mykey = lane < D ? sel[D - 1 - lane] : 0u;
+    }
+    fl = group_or(fl, fred, lane, wave);
+    if (wave == 0) {
+      uint64_t *const rows = p.dbg_rows ? p.dbg_rows + (int64_t)s * 64 : nullptr;
+      SlotResult r;
+      wave_pair(mykey, (D >= 64) ? ~0ull : ((1ull << D) - 1ull), lane, p.pp, r, rows);
+      r.flags |= fl;
+      if (lane == 0) {
+        acc.nslot++;
+        account(acc, r, s, p);
+        t.bedinfo[s] = (uint16_t)bed_info(r, p.pp);
+        if (sv) put_sv(sv, G2, s, r);
+        if (final_here) {
+          acc.nfinal += r.n;
+          acc.chkfinal += (uint64_t)(s + 1) * (uint64_t)r.n;
+          if (p.dbg_final) p.dbg_final[s] = r.n;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  DevRecord &R = p.rec[b];
+  int64_t v[18] = {acc.Va, acc.Vb, acc.Vo, acc.Vss, acc.Vnv, acc.Vtc, 0, acc.nent, acc.nslot,
+                   acc.nfinal, (int64_t)acc.chkpre, (int64_t)acc.chkfinal, acc.aVo, acc.aVss,
+                   acc.aVa, acc.aVb, acc.aVnv, acc.aVtc};
+  group_add<18>(v, red, lane, wave, tid, (unsigned long long *)&R.Va);
+  const int32_t f = group_or(acc.flags, fred, lane, wave);
+  if (tid == 0 && f) atomicOr(&R.flags, f);
 }
 
 // ---- bed_kernel: band_rows grid rows of the bed scan region per work-group ----------------
@@ -1604,6 +2067,7 @@ hipError_t launch_stage(int stage, const SweepParams &p, hipStream_t stream) {
     case 3:
       hipLaunchKernelGGL(pair_kernel, dim3(nb8 * p.pair_tiles), dim3(kThreads), 0, stream, p);
       hipLaunchKernelGGL(pair_big_kernel, dim3(nb8 * p.big_groups), dim3(kThreads), 0, stream, p);
+      hipLaunchKernelGGL(pair_huge_kernel, dim3(nb), dim3(kThreads), 0, stream, p);
       break;
     case 4:
       if (p.bed_on) hipLaunchKernelGGL(bed_kernel, dim3(nb8 * p.bed_bands), dim3(kThreads), 0, stream, p);

@@ -32,9 +32,12 @@ enum : int32_t {
   T_TC = 256, T_NVB = 512, T_NVA = 1024
 };
 
+// Per-orientation flags (the public TOMO_F_* values).  F_RETRY is internal: the overflow pool
+// or the spill list of the orientation's slot table filled up, and the host sweeps the
+// orientation again with larger ones before it reports the record.
 enum : int32_t {
   F_FP32_INEXACT = 1, F_INT16_WRAP = 2, F_FOOTPRINT_OOB = 4, F_CAP_DROP = 8,
-  F_SENTINEL_DROP = 16, F_RANK_TIE = 32, F_UNRESOLVED = 64
+  F_SENTINEL_DROP = 16, F_UNRESOLVED = 64, F_RETRY = 128
 };
 
 constexpr int kMaxPix = 63;  // pixel rows per slot (nSlotCapacityHeight - 1)
