@@ -104,3 +104,54 @@ def is_watertight(f):
         return False
     dkey = e[:, 0] * (int(f.max()) + 1) + e[:, 1]
     return np.unique(dkey).size == dkey.size
+
+
+def load_mesh(filename):
+    """Read a triangle mesh from an STL (binary or ASCII) or OBJ file without trimesh, merging
+    vertices with equal coordinates (rounded to 8 decimals, as trimesh merges STL corners).
+    Polygons of OBJ files are split into fans.  Returns a TomoMesh."""
+    path = str(filename)
+    with open(path, "rb") as fh:
+        data = fh.read()
+    low = path.lower()
+    if low.endswith(".obj"):
+        vs, fs = [], []
+        for line in data.decode(errors="replace").splitlines():
+            parts = line.split()
+            if not parts:
+                continue
+            if parts[0] == "v":
+                vs.append([float(x) for x in parts[1:4]])
+            elif parts[0] == "f":
+                idx = [int(p.split("/")[0]) for p in parts[1:]]
+                idx = [i - 1 if i > 0 else len(vs) + i for i in idx]
+                fs += [[idx[0], idx[k], idx[k + 1]] for k in range(1, len(idx) - 1)]
+        return TomoMesh(np.array(vs, np.float64).reshape(-1, 3), np.array(fs, np.int64).reshape(-1, 3),
+                        name=path)
+    if not low.endswith(".stl"):
+        raise ValueError(f"unsupported mesh file (STL or OBJ expected): {path}")
+    corners = None
+    if len(data) >= 84:
+        n = int(np.frombuffer(data, "<u4", 1, 80)[0])
+        if len(data) == 84 + 50 * n:
+            rec = np.frombuffer(data, np.dtype([("n", "<f4", 3), ("v", "<f4", (3, 3)), ("a", "<u2")]), n, 84)
+            corners = rec["v"].astype(np.float64).reshape(-1, 3)
+    if corners is None:
+        pts = [[float(x) for x in line.split()[1:4]] for line in data.decode(errors="replace").splitlines()
+               if line.strip().startswith("vertex")]
+        corners = np.array(pts, np.float64).reshape(-1, 3)
+    uniq, inv = np.unique(np.round(corners, 8), axis=0, return_inverse=True)
+    return TomoMesh(uniq, inv.reshape(-1, 3).astype(np.int64), name=path)
+
+
+def save_stl(filename, vertices, faces):
+    """Write a binary STL (the format validate_primitives.py:17-20 exports its cases in)."""
+    v = np.asarray(vertices, np.float64)
+    f = np.asarray(faces, np.int64)
+    rec = np.zeros(len(f), np.dtype([("n", "<f4", 3), ("v", "<f4", (3, 3)), ("a", "<u2")]))
+    rec["n"] = face_normals(v, f).astype(np.float32)
+    rec["v"] = v[f].astype(np.float32)
+    with open(filename, "wb") as fh:
+        fh.write(b"\0" * 80)
+        fh.write(np.array([len(f)], "<u4").tobytes())
+        fh.write(rec.tobytes())
