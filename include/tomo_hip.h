@@ -94,6 +94,23 @@ int tomo_set_orientations(tomo_handle *h, int32_t n, const float *ypr);
 /* Enqueue the sweep of orientations [begin, end) on the handle's stream (asynchronous). */
 int tomo_run(tomo_handle *h, int32_t begin, int32_t end);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): tomo_run for
+ * orientations [begin, end), with the slot tables and orientation headers in a static device
+ * array of the library, TOMO_STATIC_SCRATCH_BYTES on every device it runs on, held until the
+ * process ends, instead of allocated scratch.  It calls no allocation routine.  The mesh and
+ * orientations come from tomo_set_mesh / tomo_set_orientations (set-up calls, which allocate)
+ * and the records are read with tomo_get_records as after tomo_run.  Orientations run in batches
+ * of as many as the array holds at the handle's grid and inline capacity (about 20 at a 256
+ * grid with 16 inline entries, 5 at 512; tomo_get_info reports it).  Fails with
+ * TOMO_E_INVALID_ARG when one orientation's slot table exceeds the array (grids above 1,024 at
+ * 16 inline entries).  All calls on one device share the array, which keeps the last call's
+ * contents (nothing in it needs clearing: a sweep writes every header, counter and table before
+ * it reads them); calls that overlap on one device race.  An orientation whose overflow pool
+ * fills up is reported with TOMO_F_UNRESOLVED instead of being swept again. */
+#define TOMO_STATIC_SCRATCH_BYTES (256u << 20)
+int tomo_run_static(tomo_handle *h, int32_t begin, int32_t end);
+
 /* Wait for the handle's stream. */
 int tomo_sync(tomo_handle *h);
 

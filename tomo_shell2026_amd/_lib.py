@@ -26,7 +26,7 @@ assert RECORD_DTYPE.itemsize == 208
 
 # Every symbol include/tomo_hip.h declares.
 EXPORTS = (
-    "tomo_create", "tomo_set_mesh", "tomo_set_orientations", "tomo_run", "tomo_sync",
+    "tomo_create", "tomo_set_mesh", "tomo_set_orientations", "tomo_run", "tomo_run_static", "tomo_sync",
     "tomo_get_records", "tomo_sweep", "tomo_debug_slots", "tomo_debug_rows", "tomo_set_launch", "tomo_set_tuning",
     "tomo_last_kernel_ms", "tomo_get_info", "tomo_set_phase_profile", "tomo_phase_profile", "tomo_destroy", "tomo_last_error",
     "tomo_find_first_optimal", "TomoSh_HIP", "getMss", "getMo", "getVtc", "getVolMassInfo",
@@ -66,6 +66,7 @@ def load():
         "tomo_set_mesh": ((P, I32, P, I32, P, P), ct.c_int),
         "tomo_set_orientations": ((P, I32, P), ct.c_int),
         "tomo_run": ((P, I32, I32), ct.c_int),
+        "tomo_run_static": ((P, I32, I32), ct.c_int),
         "tomo_sync": ((P,), ct.c_int),
         "tomo_get_records": ((P, I32, I32, P), ct.c_int),
         "tomo_sweep": ((P, I32, P, P), ct.c_int),
@@ -183,6 +184,11 @@ class Sweep:
     def run(self, begin=0, end=None):
         end = self.n if end is None else end
         check(load().tomo_run(self._h, int(begin), int(end)), "tomo_run")
+
+    def run_static(self, begin=0, end=None):
+        """tomo_run with the slot tables in the library's static device array (allocates nothing)."""
+        end = self.n if end is None else end
+        check(load().tomo_run_static(self._h, int(begin), int(end)), "tomo_run_static")
 
     def sync(self):
         check(load().tomo_sync(self._h), "tomo_sync")

@@ -16,6 +16,7 @@
 
 namespace tomo {
 hipError_t launch_stage(int stage, const SweepParams &p, hipStream_t stream);
+hipError_t static_scratch_ptr(int device, uint8_t **out);
 struct CvssMeshDev {
   const float *nx, *ny, *nz, *area, *cx, *cy, *cz, *vx, *vy, *vz;
   int32_t n_tri, n_vtx;
@@ -229,6 +230,11 @@ struct tomo_handle {
   int32_t exact_fsum = 1;
   int32_t last_batch = 0;
   int32_t last_begin = 0, last_end = 0;
+  // tomo_run_static: this device's static slot-table array (nullptr if the lookup failed) and
+  // whether the last run used it (its records are not swept again on overflow)
+  uint8_t *d_static = nullptr;
+  hipError_t static_err = hipSuccess;
+  bool last_static = false;
   std::vector<float> vbed_seq;
   // per-stage timing (events around every stage of every batch)
   bool phases = false;
@@ -449,10 +455,15 @@ SweepParams base_params(tomo_handle *h) {
 }
 
 // Enqueue the five stages for orientations [begin, end) in batches of `batch`.
+// scr / hdr, when given, replace the handle's slot-table scratch and orientation headers.
 int enqueue(tomo_handle *h, int32_t begin, int32_t end, int32_t batch, int32_t *dbg4, size_t g2,
-            uint64_t *dbg_rows = nullptr) {
+            uint64_t *dbg_rows = nullptr, uint8_t *scr = nullptr, OrientHdr *hdr = nullptr) {
   for (int32_t b0 = begin; b0 < end; b0 += batch) {
     SweepParams p = base_params(h);
+    if (scr) {
+      p.scratch = scr;
+      p.hdr = hdr;
+    }
     p.n_batch = (end - b0) < batch ? (end - b0) : batch;
     p.rot = h->d_rot + (size_t)b0 * 9;
     p.rec = h->d_rec + b0;
@@ -596,6 +607,12 @@ int tomo_create(const float *f32, const int32_t *i32, int32_t grid, tomo_handle 
     return fail(TOMO_E_HIP, "stream/event creation failed");
   }
   h->lay = make_layout(o.grid, kInlineDefault);
+  // tomo_run_static's array on this (current) device; a failed lookup only disables that entry.
+  h->static_err = static_scratch_ptr(h->device, &h->d_static);
+  if (h->static_err != hipSuccess) {
+    (void)hipGetLastError();
+    h->d_static = nullptr;
+  }
   // IsBedCandidate distances dist2D (STomoSh_INT3.cpp:1326-1329) for dx, dy in [-r, r).
   const int r = o.bed_r > 0 ? o.bed_r : 1;
   std::vector<float> dist((size_t)(2 * r) * (2 * r));
@@ -799,6 +816,7 @@ int tomo_run(tomo_handle *h, int32_t begin, int32_t end) {
   h->last_begin = begin;
   h->last_end = end;
   h->timed = false;
+  h->last_static = false;
   const int32_t n = end - begin;
   if (n == 0) return TOMO_OK;
   const int32_t batch = auto_batch(h, n);
@@ -812,6 +830,47 @@ int tomo_run(tomo_handle *h, int32_t begin, int32_t end) {
   HIP_TRY(hipEventRecord(h->ev1, h->stream));
   h->timed = true;
   h->last_batch = batch;
+  return TOMO_OK;
+}
+
+int tomo_run_static(tomo_handle *h, int32_t begin, int32_t end) {
+  if (!h) return fail(TOMO_E_INVALID_ARG, "null handle");
+  if (begin < 0 || end > h->n_orient || begin > end) return fail(TOMO_E_INVALID_ARG, "bad orientation range");
+  if (!h->d_soa || !h->d_tidx || !h->d_tpos || !h->d_vaos || !h->d_chunk) return fail(TOMO_E_STATE, "no mesh");
+  if (!h->d_rot || !h->d_rec) return fail(TOMO_E_STATE, "no orientations");
+  if (h->shared) return fail(TOMO_E_STATE, "a batch member runs through tomo_batch_launch");
+  if (!h->d_static)
+    return fail(TOMO_E_STATE, std::string("static slot-table array unavailable: ") + hipGetErrorString(h->static_err));
+  // Headers first (256-byte aligned), then one slot-table region per orientation in flight.
+  const size_t region = (size_t)h->lay.bytes, cap = kStaticScratchBytes;
+  auto fits = [&](size_t b) { return ((b * sizeof(OrientHdr) + 255) & ~(size_t)255) + b * region <= cap; };
+  size_t batch = cap / (region + sizeof(OrientHdr));
+  while (batch > 0 && !fits(batch)) batch--;
+  if (batch == 0)
+    return fail(TOMO_E_INVALID_ARG, "one orientation's slot table (" + std::to_string(region) + " bytes at grid " +
+                                        std::to_string(h->opt.grid) + ", " + std::to_string(h->lay.inl) +
+                                        " inline entries) exceeds the static array (" + std::to_string(cap) +
+                                        " bytes)");
+  h->last_begin = begin;
+  h->last_end = end;
+  h->timed = false;
+  h->last_static = true;
+  const int32_t n = end - begin;
+  if (n == 0) return TOMO_OK;
+  const int32_t b = (int32_t)std::min<size_t>(batch, (size_t)n);
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  const bool was = h->phases;  // per-stage timing creates events: off here
+  h->phases = false;
+  OrientHdr *hdr = (OrientHdr *)h->d_static;
+  uint8_t *scr = h->d_static + ((b * sizeof(OrientHdr) + 255) & ~(size_t)255);
+  hipError_t e = hipEventRecord(h->ev0, h->stream);
+  int rc = e == hipSuccess ? enqueue(h, begin, end, b, nullptr, 0, nullptr, scr, hdr) : TOMO_OK;
+  if (e == hipSuccess && rc == TOMO_OK) e = hipEventRecord(h->ev1, h->stream);
+  h->phases = was;
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(TOMO_E_HIP, std::string("tomo_run_static: ") + hipGetErrorString(e));
+  h->timed = true;
+  h->last_batch = b;
   return TOMO_OK;
 }
 
@@ -829,8 +888,10 @@ int tomo_get_records(tomo_handle *h, int32_t begin, int32_t end, tomo_record *ou
   std::vector<DevRecord> d((size_t)n);
   HIP_TRY(hipMemcpyAsync(d.data(), h->d_rec + begin, sizeof(DevRecord) * (size_t)n, hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
-  const int rc = retry_overflowed(h, begin, d);
-  if (rc) return rc;
+  if (!h->last_static) {  // tomo_run_static allocates nothing, so its overflows stay unresolved
+    const int rc = retry_overflowed(h, begin, d);
+    if (rc) return rc;
+  }
   for (int32_t i = 0; i < n; i++) finish_record(h, d[(size_t)i], &h->rot[(size_t)(begin + i) * 9], out[i]);
   return TOMO_OK;
 }

@@ -24,6 +24,8 @@ constexpr int kTileWaves = kTileThreads / 64;
 constexpr int kTileSlots = 16384;      // rasteriser tile: slots with a 16-bit LDS counter each
 constexpr int kTileYDefault = 128;     // tile extent along y (power of two); x extent = kTileSlots / y
 constexpr int kChunk = 64;             // triangles per culling chunk (one per lane)
+constexpr size_t kStaticScratchBytes = (size_t)256 << 20;  // tomo_run_static's array, per device
+constexpr int kMaxDevices = 64;
 
 // Device record, one per orientation.  Totals are exact 64-bit integers accumulated with
 // atomics by the pairing and bed kernels; the host turns them into the reference's float

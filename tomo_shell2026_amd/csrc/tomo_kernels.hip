@@ -15,10 +15,17 @@
 // follows the reference's operation order without contraction (built with -ffp-contract=off).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <type_traits>
 
 #include "tomo_device.h"
 #include "tomo_raster.h"
+
+// Slot tables and orientation headers of tomo_run_static, the entry point that allocates
+// nothing: kStaticScratchBytes on every device the library runs on, held until the process
+// ends.  Host code never passes this name to a kernel (on the host it is a stand-in, not a
+// device address); it uses the address static_scratch_ptr looks up per device.
+static __device__ __align__(256) uint8_t scratch[tomo::kStaticScratchBytes];
 
 namespace tomo {
 
@@ -2055,6 +2062,25 @@ __global__ __launch_bounds__(64) void fsum_kernel(SweepParams p) {
   }
   if (lane < 6) R.fsum[lane] = acc;
   if (lane == 0) R.fsum_done = 1;
+}
+
+// Device address of `scratch` on `device`, which must be the current device the first time it
+// is asked for: looked up once per device with hipGetSymbolAddress and kept (every GPU holds its
+// own copy of the array at its own address).
+hipError_t static_scratch_ptr(int device, uint8_t **out) {
+  static std::mutex mu;
+  static void *addr[kMaxDevices] = {};
+  *out = nullptr;
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!addr[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[device] = p;
+  }
+  *out = (uint8_t *)addr[device];
+  return hipSuccess;
 }
 
 hipError_t launch_stage(int stage, const SweepParams &p, hipStream_t stream) {
