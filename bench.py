@@ -77,6 +77,47 @@ def summarize_pmc(fetch_dir, write_dir, cfetch_dir, cwrite_dir, kernel="raster_k
             "bytes_per_launch": fk * 1024.0 * f_scale + wk * 1024.0 * w_scale}
 
 
+def summarize_counters(lds_dir, wr_dir):
+    """Per kernel, averaged over its launches: LDS bank-conflict cycles over conflict-free LDS
+    cycles (SQ_LDS_BANK_CONFLICT / (SQ_LDS_IDX_ACTIVE - SQ_LDS_BANK_CONFLICT), the rate rocprof's
+    LdsBankConflict metric reports), LDS and VALU instructions per wave, and the bytes the L2
+    wrote to memory from its 64-byte and 32-byte write requests (TCC_EA0_WRREQ counts both,
+    TCC_EA0_WRREQ_64B the 64-byte ones)."""
+    import collections
+    import csv
+    import glob
+
+    def load(d):
+        out = collections.defaultdict(lambda: collections.defaultdict(list))
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(f) as fh:
+                for row in csv.DictReader(fh):
+                    k = row["Kernel_Name"].split("(")[0].replace("tomo::", "")
+                    out[k][row["Counter_Name"]].append(float(row["Counter_Value"]))
+        return {k: {c: sum(v) / len(v) for c, v in cs.items()} for k, cs in out.items()}
+
+    lds, wr = load(lds_dir), load(wr_dir)
+    summ = {}
+    for k in sorted(set(lds) | set(wr)):
+        if k.startswith("__amd"):
+            continue
+        a, b = lds.get(k, {}), wr.get(k, {})
+        e = {}
+        if a:
+            bc, act, waves = a["SQ_LDS_BANK_CONFLICT"], a["SQ_LDS_IDX_ACTIVE"], max(a["SQ_WAVES"], 1.0)
+            e.update({"lds_bank_conflict_cycles": bc, "lds_active_cycles": act,
+                      "lds_bank_conflict_rate": round(bc / (act - bc), 4) if act > bc else None,
+                      "lds_addr_conflict_cycles": a["SQ_LDS_ADDR_CONFLICT"],
+                      "lds_insts_per_wave": round(a["SQ_INSTS_LDS"] / waves, 1),
+                      "valu_insts_per_wave": round(a["SQ_INSTS_VALU"] / waves, 1), "waves": waves})
+        if b:
+            req, req64 = b["TCC_EA0_WRREQ_sum"], b["TCC_EA0_WRREQ_64B_sum"]
+            e.update({"write_requests": req, "write_requests_64B": req64,
+                      "write_bytes_per_launch": 64.0 * req64 + 32.0 * (req - req64)})
+        summ[k] = e
+    return summ
+
+
 _T0 = time.perf_counter()
 
 
@@ -85,17 +126,52 @@ def progress(msg):
     print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(w, records, stride, threads):
+def cpu_info():
+    """Host CPU model, logical CPUs and the CPUs this process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return model, os.cpu_count() or 1, avail
+
+
+def cpu_baseline(w, records, stride, runs=3):
+    """The C oracle (orientation-parallel pthreads, as Tomo_Shell2026.cpp:155-214 fans out) on a
+    bounded sample of the same sweep: the median of `runs` runs on all available cores (at most
+    16, this GPU's share of the host), and one run on one thread over a 20x sparser sample."""
     from oracle import oracle as O
     O.build()
+    model, nproc, avail = cpu_info()
+    threads = max(1, min(16, avail))
     idx = np.arange(0, len(w.ypr), stride)
+    times = []
+    same = True
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        ref = O.sweep(w.f32, w.i32, w.grid, w.tri, w.vtx, w.vnrm, w.ypr[idx], nthreads=threads)
+        times.append(time.perf_counter() - t0)
+        same = same and bool(np.array_equal(ref["Vss"], records["Vss"][idx]) and
+                             np.array_equal(ref["Vo"], records["Vo"][idx]) and
+                             np.array_equal(ref["Mss"].view(np.uint32), records["Mss"][idx].view(np.uint32)))
+    med = float(np.median(times))
+    idx1 = np.arange(0, len(w.ypr), stride * 20)
     t0 = time.perf_counter()
-    ref = O.sweep(w.f32, w.i32, w.grid, w.tri, w.vtx, w.vnrm, w.ypr[idx], nthreads=threads)
-    dt = time.perf_counter() - t0
-    same = bool(np.array_equal(ref["Vss"], records["Vss"][idx]) and np.array_equal(ref["Vo"], records["Vo"][idx]))
-    return {"value": round(len(idx) / dt, 2), "unit": "orientations/s", "cores": threads, "kind": "port",
-            "sample": f"every {stride}th orientation of the same sweep ({len(idx)} orientations, "
-                      f"{dt:.1f} s wall, oracle in C, orientation-parallel pthreads)",
+    ref1 = O.sweep(w.f32, w.i32, w.grid, w.tri, w.vtx, w.vnrm, w.ypr[idx1], nthreads=1)
+    dt1 = time.perf_counter() - t0
+    same = same and bool(np.array_equal(ref1["Vss"], records["Vss"][idx1]))
+    return {"value": round(len(idx) / med, 2), "unit": "orientations/s", "cores": threads, "kind": "port",
+            "sample": f"every {stride}th orientation of the same sweep ({len(idx)} orientations), median of "
+                      f"{runs} runs ({', '.join(f'{t:.2f}' for t in times)} s wall), oracle in C, "
+                      f"orientation-parallel pthreads",
+            "one_thread": {"value": round(len(idx1) / dt1, 2), "unit": "orientations/s",
+                           "sample": f"every {stride * 20}th orientation ({len(idx1)}, {dt1:.1f} s wall)"},
+            "cpu_model": model, "nproc": nproc, "cpus_available": avail,
             "matches_gpu_on_sample": same}
 
 
@@ -139,6 +215,18 @@ def bench_batch70(args):
         flags["fp32_inexact"] += int(((r["flags"] & 1) != 0).sum())
         flags["footprint_oob"] += int(((r["flags"] & 4) != 0).sum())
         flags["unresolved"] += int(((r["flags"] & 64) != 0).sum())
+    # every 499th orientation of every member against the oracle
+    from oracle import oracle as O
+    O.build()
+    sample_ok, sample_n = True, 0
+    for m, w in enumerate(ws):
+        r = B.records(m)
+        idx = np.arange(0, len(w.ypr), 499)
+        ref = O.sweep(w.f32, w.i32, w.grid, w.tri, w.vtx, w.vnrm, w.ypr[idx], nthreads=16)
+        sample_n += len(idx)
+        for k in ("Vo", "Vss", "Va", "Vb", "Vbed", "n_entries", "chk_final", "flags"):
+            sample_ok = sample_ok and bool(np.array_equal(r[k][idx], ref[k]))
+        sample_ok = sample_ok and bool(np.array_equal(r["Mss"][idx].view(np.uint32), ref["Mss"].view(np.uint32)))
     B.close()
     # the same sweeps, mesh by mesh, default launch shape, no graph
     plain_ms = 0.0
@@ -165,6 +253,7 @@ def bench_batch70(args):
         "graph_ms": round(graph_ms, 2), "plain_per_mesh_ms": round(plain_ms, 2),
         "graph_vs_plain_speedup": round(plain_ms / graph_ms, 3),
         "tune_s": round(tune_s, 2), "capture_s": round(capture_s, 2), "tuned_shapes": shapes, "flags": flags,
+        "matches_oracle_on_sample": sample_ok, "oracle_sample": f"every 499th orientation of every member ({sample_n})",
     }
 
 
@@ -205,9 +294,18 @@ def main():
                     help="run a 1 GiB device copy (for scaling rocprof FETCH_SIZE / WRITE_SIZE) and exit")
     ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
                     help="rocprofv3 --pmc output folders: write the traffic summary under profiles/ and exit")
+    ap.add_argument("--summarize-counters", nargs=2, metavar=("LDS_PMC", "WRITE_PMC"),
+                    help="rocprofv3 --pmc output folders of the LDS and write-request counter passes: write "
+                         "profiles/sweep_dsphere69120_counters.json and exit")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.summarize_counters:
+        summ = summarize_counters(*args.summarize_counters)
+        with open(os.path.join(ROOT, "profiles", "sweep_dsphere69120_counters.json"), "w") as fh:
+            json.dump(summ, fh, indent=1)
+        print(json.dumps(summ))
         return
     if args.summarize_pmc:
         summ = summarize_pmc(*args.summarize_pmc)
@@ -259,11 +357,15 @@ def main():
     raster_ms_launch = stages["raster"] / launches
     rb = raster_bytes(len(w.vtx), len(w.tri), p_bar, s_bar)
     achieved = rb * batch / (raster_ms_launch * 1e-3) / 1e9
-    traffic = None
+    traffic, counters = None, None
     tpath = os.path.join(ROOT, "profiles", "raster_kernel_dsphere69120_traffic.json")
+    cpath = os.path.join(ROOT, "profiles", "sweep_dsphere69120_counters.json")
     if args.workload == "dsphere" and os.path.exists(tpath):
         with open(tpath) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
+    if args.workload == "dsphere" and os.path.exists(cpath):
+        with open(cpath) as fh:
+            counters = json.load(fh).get("raster_kernel")
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -284,6 +386,8 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "kernel": "raster_kernel", "launch_orientations": batch,
                      "kernel_ms_per_launch": round(raster_ms_launch, 3),
+                     "lds_bank_conflict_rate": None if counters is None else counters["lds_bank_conflict_rate"],
+                     "write_bytes_from_requests": None if counters is None else counters["write_bytes_per_launch"],
                      "algorithmic_bytes_per_launch": round(rb * batch),
                      "algorithmic_bytes_per_orientation": round(rb),
                      "mean_entries": round(p_bar, 1), "mean_slots": round(s_bar, 1),
@@ -302,15 +406,20 @@ def main():
     if not args.no_refine:
         out["refine"] = refine_comparison(w)
         progress(f"refine: {out['refine']['wall_speedup']}x less wall time")
+        if args.workload == "dsphere":
+            # the differentiation workload's mesh: the thin-shell manikin (512 grid, shell mode)
+            mw = synth.manikin_workload()
+            out["refine_manikin"] = dict(refine_comparison(mw), workload=mw.name)
+            progress(f"refine manikin: {out['refine_manikin']['wall_speedup']}x less wall time, "
+                     f"gap {out['refine_manikin']['vss_gap_rel']}")
     if not args.no_batch and args.workload == "dsphere":
         b = bench_batch70(argparse.Namespace(steps=2, warmup=1))
         progress(f"batch70: {b['value']:.0f} orientations/s")
         out["batch70"] = {k: b[k] for k in ("value", "unit", "config", "graph_ms", "plain_per_mesh_ms",
                                             "graph_vs_plain_speedup", "tune_s", "capture_s", "tuned_shapes",
-                                            "flags")}
+                                            "flags", "matches_oracle_on_sample", "oracle_sample")}
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(w, rec, args.cpu_stride, threads)
+        out["cpu_baseline"] = cpu_baseline(w, rec, args.cpu_stride)
         progress(f"cpu baseline: {out['cpu_baseline']['value']} orientations/s")
     print(json.dumps(out), flush=True)
 

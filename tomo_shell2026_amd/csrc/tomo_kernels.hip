@@ -61,6 +61,23 @@ __device__ __forceinline__ T wave_sum(T v) {
 
 __device__ __forceinline__ int msb64(uint64_t m) { return 63 - __clzll((long long)m); }
 
+// Mask of the active lanes for which c holds, taken straight from the compare that forms c
+// (HIP's __ballot takes an int, which costs a select and a second compare per call).
+__device__ __forceinline__ uint64_t ballot(bool c) { return __builtin_amdgcn_ballot_w64(c); }
+
+// Compare predicates of __builtin_amdgcn_fcmpf / __builtin_amdgcn_uicmp (LLVM's CmpInst codes):
+// each call is one vector compare whose result is the wave's lane mask.
+constexpr int kFcmpUge = 11, kFcmpUle = 13, kIcmpSgt = 38;
+
+// scan_candidate (tomo_raster.h) as a lane mask: a >= lo, b >= lo and a + b <= his, each also
+// true when unordered (a NaN leaves the pixel to the exact test), with a and b formed as there.
+__device__ __forceinline__ uint64_t scan_mask(const TriScan &t, float rax, float rbx, float e2y) {
+  const float a = __builtin_fmaf(e2y, t.ay, rax);
+  const float b = __builtin_fmaf(e2y, t.by, rbx);
+  return __builtin_amdgcn_fcmpf(a, t.lo, kFcmpUge) & __builtin_amdgcn_fcmpf(b, t.lo, kFcmpUge) &
+         __builtin_amdgcn_fcmpf(a + b, t.his, kFcmpUle);
+}
+
 // Index of the calling wave in its work-group, as a wave-uniform (scalar) value: state derived
 // from it (turn counters, LDS row bases) then stays in scalar registers.
 __device__ __forceinline__ int wave_index() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
@@ -287,10 +304,10 @@ __device__ void wave_pair(uint32_t key, uint64_t V, int lane, const PairParams &
   const uint32_t k10 = make_key(1, 0);
   r.npre = D;
   r.sabs = seg_sum<W>(valid ? (z < 0 ? -z : z) : 0);
-  r.has10 = (__ballot(valid && key == k10) & seg) != 0ull;
+  r.has10 = (ballot(valid && key == k10) & seg) != 0ull;
   r.flags = 0;
   r.vo = r.vss = r.va = r.vb = r.vnv = r.vtc = 0;
-  if (__ballot(D > 1) == 0ull) {  // every segment has at most one row
+  if (ballot(D > 1) == 0ull) {  // every segment has at most one row
     const int32_t zl = __shfl(z, V ? msb64(V) : base);
     r.n = D;
     r.last_z = D ? zl : 0;
@@ -299,12 +316,12 @@ __device__ void wave_pair(uint32_t key, uint64_t V, int lane, const PairParams &
     return;
   }
   const bool isA = valid && nz >= 1, isB = valid && nz <= -1;
-  const uint64_t mNZ = __ballot(isA || isB), mA = __ballot(isA);
+  const uint64_t mNZ = ballot(isA || isB), mA = ballot(isA);
   const uint64_t pv = mNZ & lt;
   const bool prevA = pv ? (((mA >> msb64(pv)) & 1ull) != 0) : false;
   const bool kA = isA && !prevA, kB = isB && prevA;
   const bool bs = kB && nz <= pp.cut_ssb, bn = kB && nz >= pp.cut_nvb;
-  const uint64_t mKA = __ballot(kA) & seg, mBS = __ballot(bs) & seg, mBN = __ballot(bn) & seg;
+  const uint64_t mKA = ballot(kA) & seg, mBS = ballot(bs) & seg, mBN = ballot(bn) & seg;
   const uint64_t evS = (mBS | mKA), evN = (mBN | mKA);
   const bool prevS = (evS & lt) ? (((mBS >> msb64(evS & lt)) & 1ull) != 0) : false;
   const bool prevN = (evN & lt) ? (((mBN >> msb64(evN & lt)) & 1ull) != 0) : false;
@@ -460,7 +477,7 @@ __device__ uint32_t tri_key_order(const SweepParams &p, int32_t b, uint32_t t, i
         }
       }
     }
-    uint64_t mh = __ballot(hit);
+    uint64_t mh = ballot(hit);
     while (mh) {
       const int j = __ffsll((unsigned long long)mh) - 1;
       mh &= mh - 1ull;
@@ -516,11 +533,11 @@ __device__ uint32_t cap_select(const SweepParams &p, int32_t b, int32_t s, uint6
   const uint32_t t62 = (uint32_t)(buf[kMaxPix - 1] >> 33), t63 = (uint32_t)(buf[kMaxPix] >> 33);
   if (t62 != t63) return mykey;
   // the triangle's keys occupy positions [a, e), a <= 62 < 63 < e
-  const int a = __ffsll((unsigned long long)__ballot((uint32_t)(buf[lane] >> 33) == t62)) - 1;
+  const int a = __ffsll((unsigned long long)ballot((uint32_t)(buf[lane] >> 33) == t62)) - 1;
   int e = 64;
   for (;;) {
     const int i = e + lane;
-    const uint64_t outside = __ballot(!(i < D && (uint32_t)(buf[i < D ? i : 0] >> 33) == t62));
+    const uint64_t outside = ballot(!(i < D && (uint32_t)(buf[i < D ? i : 0] >> 33) == t62));
     if (outside) {
       e += __ffsll((unsigned long long)outside) - 1;
       break;
@@ -565,7 +582,7 @@ __device__ bool big_slot(const SweepParams &p, int32_t b, int32_t s, uint32_t c,
     if (lane == 0) pk = carry;
     carry = __shfl(vkey, 63);
     const bool first = (i < (int)c) && (i == 0 || vkey != pk);
-    const uint64_t m = __ballot(first);
+    const uint64_t m = ballot(first);
     const int pos = D + __popcll(m & ((lane == 0) ? 0ull : ((~0ull) >> (64 - lane))));
     wave_sync();
     if (first) {
@@ -630,7 +647,7 @@ __device__ void mid_slot(int32_t s, uint32_t c, const Tables &t, int32_t G2, int
   }
   const uint32_t prev = __shfl_up(key, 1);
   const bool first = i < (int)c && (i == 0 || key != prev);
-  wave_pair<W>(key, __ballot(first), lane, pp, r, rows);
+  wave_pair<W>(key, ballot(first), lane, pp, r, rows);
 }
 
 // Work-groups are dispatched to the 8 XCDs round-robin by block index.  Map block i to
@@ -867,7 +884,7 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6)
         chunk_span(bmin, bmax, ctr, R3, R4, R5, ky, ly, hy);
         keep = hx >= exlo && lx <= exhi && hy >= eylo && ly <= eyhi;
       }
-      const uint64_t km = __ballot(keep);
+      const uint64_t km = ballot(keep);
       if (lane == 0) L.wcnt[wave] = (uint32_t)__popcll(km);
       __syncthreads();
       uint32_t before = 0u, total = 0u;
@@ -966,16 +983,14 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6)
         }
       }
       L.hits[wave][lane] = 0;
-      int32_t x = x0, y = y0;
+      int32_t x = x0;
       bool done = !active || x1 < x0 || y1 < y0;
       // Boxes of more than kLaneBox pixels are left out of the lane-by-lane walk and walked
       // afterwards by the whole wave, 64 pixels per step, so that one large box does not hold
       // the wave for its whole length.
       const bool big = !done && (int64_t)(x1 - x0 + 1) * (int64_t)(y1 - y0 + 1) > kLaneBox;
       done = done || big;
-      uint64_t bigm = __ballot(big);
-      float e2x = centre_offset(x, p.h2, v0x);
-      float rax = e2x * sc.ax, rbx = e2x * sc.bx;
+      uint64_t bigm = ballot(big);
       uint32_t qn = 0;
       // Exact test, interpolation and slot append of the first min(qn, 64) queued candidates.
       auto flush = [&]() {
@@ -1030,25 +1045,36 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6)
       };
       wave_sync();
       // Column-major walk of the box, written without divergent branches: every lane advances
-      // every step (a finished lane's position runs on past x1 and is never tested).
-      uint32_t xmeta = (uint32_t)lane | ((uint32_t)(x + 256) << 6);
-      for (;;) {
-        const bool cand = (!done) & scan_candidate(sc, rax, rbx, centre_offset(y, p.h2, v0y));
-        const uint32_t meta = xmeta | ((uint32_t)(y + 256) << 19);
-        const bool wrap = y >= y1;
-        y = wrap ? y0 : y + 1;
-        x += wrap ? 1 : 0;
-        done = done | (x > x1);
-        e2x = centre_offset(x, p.h2, v0x);
-        rax = e2x * sc.ax;
-        rbx = e2x * sc.bx;
-        xmeta = (uint32_t)lane | ((uint32_t)(x + 256) << 6);
-        const uint64_t m = __ballot(cand);
-        if (cand) L.qmeta[wave][qn + lanes_below(m)] = meta;
-        qn += (uint32_t)__popcll(m);
-        const bool all_done = __ballot(!done) == 0ull;
-        if (qn >= 64u) flush();
-        if (all_done) break;
+      // every step (a finished lane's position runs on past x1 and is never tested).  The row
+      // is carried as yf == (float)y (exact: |y| < 2^24), so its centre offset is formed with
+      // the same float operations as centre_offset(y, ...); meta is the queue word of (x, y),
+      // lane | (x + 256) << 6 | (y + 256) << 19, advanced in place.
+      {
+        const float y0f = (float)y0, y1f = (float)y1;
+        const uint32_t mcol = (uint32_t)lane + ((uint32_t)(y0 + 256) << 19) + (256u << 6);
+        const uint64_t lanes = __builtin_amdgcn_read_exec();
+        uint64_t dm = ballot(done);  // lanes whose walk has ended
+        float yf = y0f;
+        uint32_t meta = mcol + ((uint32_t)x << 6);
+        float e2x = centre_offset(x, p.h2, v0x);
+        float rax = e2x * sc.ax, rbx = e2x * sc.bx;
+        for (;;) {
+          const float e2y = ((yf + kEps) + p.h2) - v0y;
+          const uint64_t m = scan_mask(sc, rax, rbx, e2y) & ~dm;
+          const bool wrap = yf >= y1f;
+          x += wrap ? 1 : 0;
+          yf = wrap ? y0f : yf + 1.0f;
+          const uint32_t mnext = wrap ? mcol + ((uint32_t)x << 6) : meta + (1u << 19);
+          dm |= __builtin_amdgcn_uicmp((uint32_t)x, (uint32_t)x1, kIcmpSgt);
+          e2x = centre_offset(x, p.h2, v0x);
+          rax = e2x * sc.ax;
+          rbx = e2x * sc.bx;
+          if (__builtin_amdgcn_inverse_ballot_w64(m)) L.qmeta[wave][qn + lanes_below(m)] = meta;
+          meta = mnext;
+          qn += (uint32_t)__popcll(m);
+          if (qn >= 64u) flush();
+          if ((lanes & ~dm) == 0ull) break;
+        }
       }
       // The large boxes, one at a time: their triangle's walk state is read from its lane into
       // wave-uniform values and the wave takes 64 consecutive pixels (column-major) per step,
@@ -1066,7 +1092,6 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6)
         js.bx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sc.bx), j));
         js.by = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sc.by), j));
         js.lo = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sc.lo), j));
-        js.hia = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sc.hia), j));
         js.his = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sc.his), j));
         const int32_t hgt = jy1 - jy0 + 1, npx = (jx1 - jx0 + 1) * hgt;
         const int32_t dq = 64 / hgt, dr = 64 - dq * hgt;
@@ -1074,9 +1099,9 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6)
         int32_t wx = jx0 + col, wy = jy0 + (lane - col * hgt);
         for (int32_t k0 = 0; k0 < npx; k0 += 64) {
           const float e2 = centre_offset(wx, p.h2, jv0x);
-          const bool cand = (k0 + lane < npx) & scan_candidate(js, e2 * js.ax, e2 * js.bx, centre_offset(wy, p.h2, jv0y));
-          const uint64_t m = __ballot(cand);
-          if (cand)
+          const uint64_t m = scan_mask(js, e2 * js.ax, e2 * js.bx, centre_offset(wy, p.h2, jv0y)) &
+                             ballot(k0 + lane < npx);
+          if (__builtin_amdgcn_inverse_ballot_w64(m))
             L.qmeta[wave][qn + lanes_below(m)] = (uint32_t)j | ((uint32_t)(wx + 256) << 6) | ((uint32_t)(wy + 256) << 19);
           qn += (uint32_t)__popcll(m);
           if (qn >= 64u) flush();
@@ -1127,7 +1152,7 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6)
         // row beyond the grid, write nothing
         if (wi < nwords && sy < yb) t.count[s] = c;
         const bool big = c > (uint32_t)kSmall;
-        const uint64_t bm = __ballot(big);
+        const uint64_t bm = ballot(big);
         if (bm) {
           uint32_t base = 0u;
           if (lane == 0) base = atomicAdd(&h.big_top, (uint32_t)__popcll(bm));
@@ -1400,7 +1425,7 @@ __device__ __forceinline__ void pair_body(const SweepParams &p) {
         }
       }
       {
-        const uint64_t m = __ballot(cls == 0);
+        const uint64_t m = ballot(cls == 0);
         if (cls == 0) {
           const uint32_t at = n2 + lanes_below(m);
           q2[at] = e;
@@ -1410,12 +1435,12 @@ __device__ __forceinline__ void pair_body(const SweepParams &p) {
         n2 += (uint32_t)__popcll(m);
       }
       {
-        const uint64_t m = __ballot(cls == 1);
+        const uint64_t m = ballot(cls == 1);
         if (cls == 1) q4[n4 + lanes_below(m)] = e;
         n4 += (uint32_t)__popcll(m);
       }
       {
-        const uint64_t m = __ballot(cls == 2);
+        const uint64_t m = ballot(cls == 2);
         if (cls == 2) q8[n8 + lanes_below(m)] = e;
         n8 += (uint32_t)__popcll(m);
       }
@@ -1473,8 +1498,8 @@ __device__ __forceinline__ void pair_body(const SweepParams &p) {
     }
     const bool fits = lc <= (uint32_t)t.inl || pool_ok;
     const bool half = lc > lane_cap && lc <= 32u && fits;
-    uint64_t m32 = __ballot(half);
-    uint64_t m64 = __ballot(lc > lane_cap && !half);
+    uint64_t m32 = ballot(half);
+    uint64_t m64 = ballot(lc > lane_cap && !half);
     while (m32) {
       const int la = __ffsll((unsigned long long)m32) - 1;
       m32 &= m32 - 1ull;
@@ -1560,7 +1585,7 @@ __device__ void group_sort_desc(uint64_t *buf, int P, int tid) {
 
 // Exclusive count of `flag` over the work-group's threads before this one; *total gets the sum.
 __device__ __forceinline__ uint32_t group_scan(bool flag, uint32_t *wcnt, int lane, int wave, uint32_t &total) {
-  const uint64_t m = __ballot(flag);
+  const uint64_t m = ballot(flag);
   if (lane == 0) wcnt[wave] = (uint32_t)__popcll(m);
   __syncthreads();
   uint32_t before = 0u;
@@ -1615,7 +1640,7 @@ __device__ int wave_distinct_by_rank(uint64_t *buf, int n, int lane) {
     if (lane == 0) pk = carry;
     carry = __shfl(vkey, 63);
     const bool first = (i < n) && (i == 0 || vkey != pk);
-    const uint64_t m = __ballot(first);
+    const uint64_t m = ballot(first);
     const int pos = D + (int)lanes_below(m);
     wave_sync();
     if (first) buf[pos] = ((uint64_t)(~(uint32_t)v) << 32) | vkey;
@@ -1882,8 +1907,8 @@ __global__ __launch_bounds__(kThreads) void bed_kernel(SweepParams p) {
           if (c0 >= hw) break;
           const int32_t yc = c0 + lane;
           if (yc < hw) nb[r0 * hw + yc] = (uint16_t)v[q];
-          const uint64_t sm = __ballot(yc < hw && (v[q] & BI_SRC) != 0u);
-          const uint64_t cm = __ballot(yc < hw && (v[q] & BI_CONV) != 0u);
+          const uint64_t sm = ballot(yc < hw && (v[q] & BI_SRC) != 0u);
+          const uint64_t cm = ballot(yc < hw && (v[q] & BI_CONV) != 0u);
           if (bits && lane == 0) {
             const int32_t w0 = c0 >> 5;
             srcw[r0 * nw + w0] = (uint32_t)sm;
@@ -2007,11 +2032,15 @@ __global__ __launch_bounds__(kThreads) void bed_kernel(SweepParams p) {
 // ---- fsum_kernel: one 64-lane work-group per orientation ----------------------------------
 // The reference accumulates per-slot integer totals into float running sums in ascending slot
 // order: Vo and Vss in Pairing (STomoSh_INT3.cpp:386-392), Va, Vb, Vnv and Vtc in Calculate
-// (:1233-1240).  Whenever the sum of |slot value| stays within 2^24 every partial sum is an
-// exactly representable integer and float(total) is that result; otherwise this kernel replays
-// the sums in the same order from the per-slot values pair_kernel kept.  Lane q < 6 owns total q.
+// (:1233-1240).  A running sum is exact, and so equal to the integer prefix sum, as long as the
+// sum of |slot value| so far stays within 2^24.  A total whose sum of |slot value| stays within
+// 2^24 is therefore float(total).  For the others, a first pass sums whole blocks of 512 slots
+// exactly in integers, lane-parallel, while the bound holds for all of them, and the serial
+// replay (lane q owns total q) starts at the first block where it fails for one, from the exact
+// sums before it.
 __global__ __launch_bounds__(64) void fsum_kernel(SweepParams p) {
   constexpr int kAhead = 8;                 // chunks of 64 slots in flight per lane
+  constexpr int kBlock = 8;                 // chunks per block of the exact first pass
   __shared__ int32_t vals[6][65];           // padded: lanes 0..5 read one column each
   const int lane = threadIdx.x;
   const int32_t b = blockIdx.x;
@@ -2019,14 +2048,15 @@ __global__ __launch_bounds__(64) void fsum_kernel(SweepParams p) {
   DevRecord &R = p.rec[b];
   const int64_t lim = (int64_t)1 << 24;
   if (!totals_may_round(h, p.pp)) return;
-  if (!(R.aVo > lim || R.aVss > lim || R.aVa > lim || R.aVb > lim || R.aVnv > lim || R.aVtc > lim)) return;
+  const int64_t tot[6] = {R.Vo, R.Vss, R.Va, R.Vb, R.Vnv, R.Vtc};
+  const int64_t absv[6] = {R.aVo, R.aVss, R.aVa, R.aVb, R.aVnv, R.aVtc};
+  if (!(absv[0] > lim || absv[1] > lim || absv[2] > lim || absv[3] > lim || absv[4] > lim || absv[5] > lim)) return;
   const Tables t = tables_for(p, b);
   const int32_t G = p.grid, G2 = G * G;
   const int32_t bx0 = h.bbox[0], bx1 = h.bbox[1], by0 = h.bbox[2], by1 = h.bbox[3];
   const int32_t wy = by1 - by0 + 1;
   const int32_t nbb = bx1 >= bx0 ? (bx1 - bx0 + 1) * wy : 0;
   const int32_t nchunk = (nbb + 63) >> 6;
-  const int q = lane < 6 ? lane : 0;
   // Slots in ascending id order: row-major over the bounding box (id = x * G + y).
   auto load = [&](int32_t c, int32_t (&v)[6]) {
     const int32_t i = (c << 6) + lane;
@@ -2039,11 +2069,67 @@ __global__ __launch_bounds__(64) void fsum_kernel(SweepParams p) {
       for (int k = 0; k < 6; k++) v[k] = 0;
     }
   };
+  // First pass (wave-uniform): exact integer sums S[k] of whole blocks while, for every total
+  // that needs the replay, the sum of |value| so far stays within 2^24 (then every partial sum
+  // before the block is exact); cmin is the first block where that fails for some total.
+  bool need[6];
+  int64_t S[6], A[6];
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    need[k] = absv[k] > lim;
+    S[k] = 0;
+    A[k] = 0;
+  }
+  int32_t cmin = nchunk;
+  for (int32_t c0 = 0; c0 < nchunk; c0 += kBlock) {
+    int64_t ba[6] = {0, 0, 0, 0, 0, 0}, bs[6] = {0, 0, 0, 0, 0, 0};
+    for (int j = 0; j < kBlock; j++) {
+      int32_t v[6];
+      load(c0 + j, v);
+#pragma unroll
+      for (int k = 0; k < 6; k++) {
+        ba[k] += v[k] < 0 ? -(int64_t)v[k] : (int64_t)v[k];
+        bs[k] += v[k];
+      }
+    }
+    bool fits = true;
+    int64_t BS[6], BA[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      BS[k] = BA[k] = 0;
+      if (!need[k]) continue;
+      BS[k] = wave_sum(bs[k]);
+      BA[k] = wave_sum(ba[k]);
+      fits = fits && A[k] + BA[k] <= lim;
+    }
+    if (!fits) {
+      cmin = c0;
+      break;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      A[k] += BA[k];
+      S[k] += BS[k];
+    }
+  }
+  int32_t mystart = nchunk;
+  int64_t myS = 0, mytot = 0;
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    if (lane == k) {
+      mystart = need[k] ? cmin : nchunk;
+      myS = S[k];
+      mytot = tot[k];
+    }
+  }
+  // |myS| <= A <= 2^24: the float running sum before chunk mystart is exactly myS
+  float acc = mystart < nchunk ? (float)myS : (float)mytot;
+  // Second pass: the serial replay from chunk cmin, each lane adding from its own start.
+  const int q = lane < 6 ? lane : 0;
   int32_t buf[kAhead][6];
 #pragma unroll
-  for (int j = 0; j < kAhead; j++) load(j, buf[j]);
-  float acc = 0.0f;
-  for (int32_t c0 = 0; c0 < nchunk; c0 += kAhead) {
+  for (int j = 0; j < kAhead; j++) load(cmin + j, buf[j]);
+  for (int32_t c0 = cmin; c0 < nchunk; c0 += kAhead) {
 #pragma unroll
     for (int j = 0; j < kAhead; j++) {
       const int32_t c = c0 + j;
@@ -2052,7 +2138,7 @@ __global__ __launch_bounds__(64) void fsum_kernel(SweepParams p) {
         for (int k = 0; k < 6; k++) vals[k][lane] = buf[j][k];
         __syncthreads();
         load(c + kAhead, buf[j]);
-        if (lane < 6) {
+        if (lane < 6 && c >= mystart) {
           const int32_t n = min(64, nbb - (c << 6));
           for (int32_t i = 0; i < n; i++) acc = acc + (float)vals[q][i];
         }

@@ -18,9 +18,10 @@
 // With D = |den|, Sa = Ex (|d11 e0x| + |d01 e1x|) + Ey (|d11 e0y| + |d01 e1y|), Sb likewise for b,
 // Ex, Ey the largest |e2x|, |e2y| in the box, and the slack M = 2^-12 D + 2^-16 (Sa + Sb), a
 // pixel is rejected when
-//   a < -eps*D - M   (v < -eps)          a > L*D + M       (v > 1 + eps)
-//   b < -eps*D - M   (u + v > 1 + eps)   a + b > (1+eps)*D + M  (u < -eps)
-// where L = 1.0009999 is the largest float not above 1 + (double)eps.  |e2x| <= Ex holds exactly
+//   a < -eps*D - M   (v < -eps)          b < -eps*D - M   (u + v > 1 + eps)
+//   a + b > (1+eps)*D + M  (u < -eps)
+// The exact test's v <= 1 + eps is left to it: with b >= -eps*D - M and a + b <= (1+eps)*D + M
+// it only removes pixels with v within about 2 eps above 1.  |e2x| <= Ex holds exactly
 // (the float pixel-centre offset is monotone in the pixel index, so its extremes sit at the box
 // ends).  The affine evaluation and the exact test's float numerators each lie within
 // 4 * 2^-24 Sa (resp. Sb) of the real-number numerator of the same float inputs; the correctly
@@ -79,7 +80,7 @@ TOMO_RHD void bary_num(const TriSetup &s, float cx, float cy, float &nv, float &
 // Per-triangle constants of the prefilter for the pixel-centre offsets |e2x| <= ex, |e2y| <= ey.
 struct TriScan {
   float ax, ay, bx, by;  // gradients of a and b (sign of den folded in)
-  float lo, hia, his;    // a, b >= lo;  a <= hia;  a + b <= his
+  float lo, his;         // a, b >= lo;  a + b <= his
 };
 
 TOMO_RHD TriScan tri_scan(const TriSetup &s, float ex, float ey) {
@@ -97,7 +98,6 @@ TOMO_RHD TriScan tri_scan(const TriSetup &s, float ex, float ey) {
                    ey * (__builtin_fabsf(p1y) + __builtin_fabsf(q0y));
   const float M = 0.000244140625f * D + 0.0000152587890625f * (sa + sb);
   t.lo = -kEps * D - M;
-  t.hia = kOnePlusEpsF * D + M;
   t.his = (1.0f + kEps) * D + M;
   return t;
 }
@@ -110,7 +110,7 @@ TOMO_RHD float centre_offset(int32_t i, float h2, float v0) { return (((float)i 
 TOMO_RHD bool scan_candidate(const TriScan &t, float rax, float rbx, float e2y) {
   const float a = __builtin_fmaf(e2y, t.ay, rax);
   const float b = __builtin_fmaf(e2y, t.by, rbx);
-  return !(a < t.lo || b < t.lo || a > t.hia || a + b > t.his);
+  return !(a < t.lo || b < t.lo || a + b > t.his);
 }
 
 // Upper bound on 1 / x for finite x > 0 (at least 2^-10 above it).
