@@ -710,8 +710,9 @@ int tomo_set_mesh(tomo_handle *h, int32_t n_tri, const int32_t *tri, int32_t n_v
     for (int k = 0; k < 6; k++) box[(size_t)c * 6 + k] = b[k];
   }
   // One 16-byte index record per triangle in culling order (its three vertex indices and the
-  // caller's index) and two float4 per vertex (raw position and normal): 32 bytes per vertex plus
-  // 16 per triangle stay resident in an XCD's L2 while every orientation reads them.
+  // caller's index) and two float4 per vertex (position relative to the rotation centre, formed
+  // by the float subtraction the rotation starts with, and normal): 32 bytes per vertex plus 16
+  // per triangle stay resident in an XCD's L2 while every orientation reads them.
   std::vector<int32_t> tix((size_t)4 * (n_tri > 0 ? n_tri : 1), 0);
   std::vector<int32_t> tpos((size_t)(n_tri > 0 ? n_tri : 1), 0);
   for (int32_t i = 0; i < n_tri; i++) {
@@ -721,7 +722,7 @@ int tomo_set_mesh(tomo_handle *h, int32_t n_tri, const int32_t *tri, int32_t n_v
   }
   std::vector<float> aos((size_t)8 * (n_vtx > 0 ? n_vtx : 1), 0.0f);
   for (int32_t v = 0; v < n_vtx; v++) {
-    for (int k = 0; k < 3; k++) aos[(size_t)v * 8 + k] = vtx[(size_t)v * 3 + k];
+    for (int k = 0; k < 3; k++) aos[(size_t)v * 8 + k] = vtx[(size_t)v * 3 + k] - h->center[k];
     for (int k = 0; k < 3; k++) aos[(size_t)v * 8 + 3 + k] = vtx_nrm[(size_t)v * 3 + k];
   }
   float *d_soa = nullptr, *d_chunk = nullptr;

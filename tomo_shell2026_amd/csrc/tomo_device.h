@@ -72,7 +72,7 @@ struct SweepParams {
   const float *vx, *vy, *vz;  // [n_vtx] raw vertex positions (structure of arrays)
   const int4 *tidx;         // [n_tri] vertex indices i0, i1, i2 and the caller's triangle index
   const int32_t *tpos;      // [n_tri] position in tidx of the caller's triangle t
-  const float4 *vaos;       // [n_vtx][2] raw (x, y, z, nx), (ny, nz, 0, 0)
+  const float4 *vaos;       // [n_vtx][2] (x - cx, y - cy, z - cz, nx), (ny, nz, 0, 0)
   int32_t n_tri, n_vtx, n_chunks;
   float cx, cy, cz;        // raw AABB centre (STomoSh_Base.cpp:75-82)
   // orientations of this batch: rot/rec already offset to the batch's first orientation

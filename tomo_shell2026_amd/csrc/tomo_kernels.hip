@@ -52,6 +52,17 @@ __device__ __forceinline__ float dot_row(float r0, float r1, float r2, float a0,
   return s;
 }
 
+// dot_row without its trailing "+ 0": the two differ only in the sign of a zero result.  The
+// rasteriser offsets every rotated coordinate (by the placement, then by the margin) before any
+// other use, and reads the rotated normal only through the sign test n.z >= 0 and the 16-bit
+// conversion, both of which treat -0 as +0; its outputs are the same either way.
+__device__ __forceinline__ float dot_row_nz(float r0, float r1, float r2, float a0, float a1, float a2) {
+  float s = r0 * a0;
+  s = s + r1 * a1;
+  s = s + r2 * a2;
+  return s;
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll
@@ -430,11 +441,10 @@ __device__ uint32_t tri_key_order(const SweepParams &p, int32_t b, uint32_t t, i
 #pragma unroll
   for (int q = 0; q < 3; q++) {  // as raster_kernel: rotate, place, then the margin
     const float4 va = p.vaos[2 * ids[q]], vb = p.vaos[2 * ids[q] + 1];
-    const float t0 = va.x - p.cx, t1 = va.y - p.cy, t2 = va.z - p.cz;
-    m[q][0] = (dot_row(R0, R1, R2, t0, t1, t2) - kx) + kEps;
-    m[q][1] = (dot_row(R3, R4, R5, t0, t1, t2) - ky) + kEps;
-    m[q][2] = (dot_row(R6, R7, R8, t0, t1, t2) - kz) + kEps;
-    nzv[q] = dot_row(R6, R7, R8, va.w, vb.x, vb.y);
+    m[q][0] = (dot_row_nz(R0, R1, R2, va.x, va.y, va.z) - kx) + kEps;
+    m[q][1] = (dot_row_nz(R3, R4, R5, va.x, va.y, va.z) - ky) + kEps;
+    m[q][2] = (dot_row_nz(R6, R7, R8, va.x, va.y, va.z) - kz) + kEps;
+    nzv[q] = dot_row_nz(R6, R7, R8, va.w, vb.x, vb.y);
   }
   const float v0x = m[0][0], v0y = m[0][1], v1x = m[1][0], v1y = m[1][1], v2x = m[2][0], v2y = m[2][1];
   float mnx = (v0x <= v1x) ? v0x : v1x; mnx = (mnx <= v2x) ? mnx : v2x;
@@ -795,11 +805,12 @@ constexpr int kLaneBox = 48;  // pixel boxes walked lane by lane; larger ones by
 constexpr int kCullCap = 896;  // chunk list entries per window (LDS)
 
 struct RasterLds {
-  uint32_t cnt[kTileSlots / 2];  // the tile's slot counters, 16 bits each
   // placed (un-margined) vertices a0, a1, a2 (x, y, z) and n0z, n1z, n2z; everything else the
   // exact test needs is recomputed from them bit for bit.  Odd stride: lanes reading different
-  // owners hit different banks.
+  // owners hit different banks.  First in the block, so a record's words are within reach of
+  // the immediate offsets of LDS loads and stores.
   float tri[kTileWaves][64][TR_STRIDE];
+  uint32_t cnt[kTileSlots / 2];  // the tile's slot counters, 16 bits each
   uint32_t qmeta[kTileWaves][128];  // owner lane | (x + 256) << 6 | (y + 256) << 19
   uint8_t hits[kTileWaves][64];
   int32_t red[kTileWaves][5];
@@ -909,7 +920,7 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6)
     for (;;) {
       uint32_t idx = 0u;
       if (lane == 0) idx = atomicAdd(&L.take, 1u);
-      idx = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl(idx, 0));
+      idx = (uint32_t)__builtin_amdgcn_readlane((int)idx, 0);
       if (idx >= n) break;
       const int32_t ch = wbase + __builtin_amdgcn_readfirstlane((int)L.clist[idx]);
       const int32_t tr = ch * kChunk + lane;
@@ -924,8 +935,9 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6)
         const int4 ix = p.tidx[tr];
         my_rank = 2u * (uint32_t)ix.w;
         const int32_t ids[3] = {ix.x, ix.y, ix.z};
-        // Rotate the three vertices from the raw mesh (shared by every orientation, so it stays
-        // in L2) exactly as aabb_kernel rotates them (STomoSh_Base.cpp:85-101), then place them.
+        // Rotate the three vertices (stored relative to the rotation centre, so shared by every
+        // orientation and resident in L2) as aabb_kernel rotates them (STomoSh_Base.cpp:85-101),
+        // then place them.
         float4 va[3], vb[3];
 #pragma unroll
         for (int q = 0; q < 3; q++) {
@@ -934,11 +946,10 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6)
         }
 #pragma unroll
         for (int q = 0; q < 3; q++) {
-          const float t0 = va[q].x - ctr[0], t1 = va[q].y - ctr[1], t2 = va[q].z - ctr[2];
-          a[q][0] = dot_row(R0, R1, R2, t0, t1, t2) - kx;
-          a[q][1] = dot_row(R3, R4, R5, t0, t1, t2) - ky;
-          a[q][2] = dot_row(R6, R7, R8, t0, t1, t2) - kz;
-          nzv[q] = dot_row(R6, R7, R8, va[q].w, vb[q].x, vb[q].y);
+          a[q][0] = dot_row_nz(R0, R1, R2, va[q].x, va[q].y, va[q].z) - kx;
+          a[q][1] = dot_row_nz(R3, R4, R5, va[q].x, va[q].y, va[q].z) - ky;
+          a[q][2] = dot_row_nz(R6, R7, R8, va[q].x, va[q].y, va[q].z) - kz;
+          nzv[q] = dot_row_nz(R6, R7, R8, va[q].w, vb[q].x, vb[q].y);
         }
         v0x = a[0][0] + kEps; v0y = a[0][1] + kEps;
         const float v1x = a[1][0] + kEps, v1y = a[1][1] + kEps;

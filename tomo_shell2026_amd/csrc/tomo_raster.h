@@ -56,9 +56,11 @@ TOMO_RHD TriSetup tri_setup(float v0x, float v0y, float v1x, float v1y, float v2
   s.e0y = v1y - v0y;
   s.e1x = v2x - v0x;
   s.e1y = v2y - v0y;
-  s.d00 = s.e0x * s.e0x + s.e0y * s.e0y + 0.0f;
+  // _getBaryCoord's dot products include the z term 0 * 0 = +0.  Adding +0 changes nothing in
+  // a sum of squares (never -0), so it is kept only in d01.
+  s.d00 = s.e0x * s.e0x + s.e0y * s.e0y;
   s.d01 = s.e0x * s.e1x + s.e0y * s.e1y + 0.0f;
-  s.d11 = s.e1x * s.e1x + s.e1y * s.e1y + 0.0f;
+  s.d11 = s.e1x * s.e1x + s.e1y * s.e1y;
   s.den = s.d00 * s.d11 - s.d01 * s.d01;
   return s;
 }
