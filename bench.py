@@ -177,8 +177,9 @@ def cpu_baseline(w, records, stride, runs=3):
 
 def bench_batch70(args):
     """70 meshes (6 thin-shell manikin parts, 64 displaced spheres of 1k-100k triangles), each on
-    its own 1 degree x 0..60 grid: autotune, capture one graph, time `steps` launches; compare
-    with the same sweeps launched mesh by mesh with the default shape and no graph."""
+    its own 1 degree x 0..60 grid: autotune, capture one graph of persistent work-queue launches,
+    time `steps` launches; compare with the same graph in the chained form (each member's own
+    launches) and with the sweeps launched mesh by mesh with the default shape and no graph."""
     from tomo_shell2026_amd import _lib, synth
     ws = synth.batch70_workloads()
     total = sum(len(w.ypr) for w in ws)
@@ -203,6 +204,19 @@ def bench_batch70(args):
         B.sync()
         ms.append(B.last_ms())
     dt = time.perf_counter() - t0
+    rounds, units = B.plan()
+    queue_recs = [B.records(m) for m in range(len(ws))]
+    progress("batch70: the chained form")
+    B.set_mode(B.CHAIN)
+    B.capture()
+    B.launch()
+    B.sync()
+    chain_ms = []
+    for _ in range(args.steps):
+        B.launch()
+        B.sync()
+        chain_ms.append(B.last_ms())
+    chain_same = all(B.records(m).tobytes() == queue_recs[m].tobytes() for m in range(len(ws)))
     shapes = {}
     for m in range(len(ws)):
         t = B.tuning(m)
@@ -210,7 +224,7 @@ def bench_batch70(args):
         shapes[str(k)] = shapes.get(str(k), 0) + 1
     flags = {"cap_drop": 0, "fp32_inexact": 0, "footprint_oob": 0, "unresolved": 0}
     for m in range(len(ws)):
-        r = B.records(m)
+        r = queue_recs[m]
         flags["cap_drop"] += int(((r["flags"] & 8) != 0).sum())
         flags["fp32_inexact"] += int(((r["flags"] & 1) != 0).sum())
         flags["footprint_oob"] += int(((r["flags"] & 4) != 0).sum())
@@ -220,7 +234,7 @@ def bench_batch70(args):
     O.build()
     sample_ok, sample_n = True, 0
     for m, w in enumerate(ws):
-        r = B.records(m)
+        r = queue_recs[m]
         idx = np.arange(0, len(w.ypr), 499)
         ref = O.sweep(w.f32, w.i32, w.grid, w.tri, w.vtx, w.vnrm, w.ypr[idx], nthreads=16)
         sample_n += len(idx)
@@ -243,14 +257,17 @@ def bench_batch70(args):
             plain_ms += s.kernel_ms()
     graph_ms = float(np.mean(ms))
     return {
-        "metric": "orientation evaluations/s, 70 uneven meshes in one graph launch",
+        "metric": "orientation evaluations/s, 70 uneven meshes in one persistent graph launch",
         "value": round(total * args.steps / dt, 1), "unit": "orientations/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+i32",
         "data": "synthetic",
         "config": {"workload": "batch70", "meshes": len(ws), "triangles_total": int(sum(len(w.tri) for w in ws)),
                    "orientations_total": total, "grids": sorted({w.grid for w in ws})},
-        "graph_ms": round(graph_ms, 2), "plain_per_mesh_ms": round(plain_ms, 2),
+        "graph_ms": round(graph_ms, 2), "rounds": rounds, "units": units,
+        "chain_graph_ms": round(float(np.mean(chain_ms)), 2), "chain_matches_queue": chain_same,
+        "queue_vs_chain_speedup": round(float(np.mean(chain_ms)) / graph_ms, 3),
+        "plain_per_mesh_ms": round(plain_ms, 2),
         "graph_vs_plain_speedup": round(plain_ms / graph_ms, 3),
         "tune_s": round(tune_s, 2), "capture_s": round(capture_s, 2), "tuned_shapes": shapes, "flags": flags,
         "matches_oracle_on_sample": sample_ok, "oracle_sample": f"every 499th orientation of every member ({sample_n})",

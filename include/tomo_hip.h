@@ -210,6 +210,20 @@ int tomo_batch_get_tuning(tomo_batch *b, int32_t member, int32_t *batch, int32_t
 int tomo_batch_set_tuning(tomo_batch *b, int32_t member, int32_t batch, int32_t tile_y,
                           int32_t inline_capacity);
 
+/* How tomo_batch_capture builds the graph.  TOMO_BATCH_QUEUE (the default): the members'
+ * orientations are cut into units (a member's orientations in flight), consecutive units are
+ * packed into rounds whose slot tables fit the shared block side by side, and each round runs as
+ * one persistent work-queue launch per stage kernel over all its units' (mesh, orientation-tile)
+ * items.  TOMO_BATCH_CHAIN: every member's own chain of stage launches, member after member.
+ * Results are the same.  Invalidates a capture.  No reference counterpart (the reference sweeps
+ * parts one call at a time). */
+#define TOMO_BATCH_QUEUE 0
+#define TOMO_BATCH_CHAIN 1
+int tomo_batch_set_mode(tomo_batch *b, int32_t mode);
+
+/* Rounds and units of the last persistent capture (0 and 0 for the chained form). */
+int tomo_batch_get_plan(tomo_batch *b, int32_t *rounds, int32_t *units);
+
 /* Allocate the shared scratch and capture every member's sweep into one graph (no launch). */
 int tomo_batch_capture(tomo_batch *b);
 

@@ -40,10 +40,13 @@ def _small_members():
 
 
 @pytest.mark.gpu
-def test_batch_graph_matches_single_sweeps_and_oracle(oracle_mod):
+@pytest.mark.parametrize("mode", ["queue", "chain"])
+def test_batch_graph_matches_single_sweeps_and_oracle(oracle_mod, mode):
+    """Both capture forms: persistent work-queue launches over rounds of units, and each member's
+    own chain of launches."""
     from tomo_shell2026_amd import _lib
     ws = _small_members()
-    print("[gpu-case] batch graph, members:", [(w.name, w.grid, len(w.tri)) for w in ws], flush=True)
+    print(f"[gpu-case] batch graph ({mode}), members:", [(w.name, w.grid, len(w.tri)) for w in ws], flush=True)
     assert len({w.grid for w in ws}) > 1  # layouts change between members
     singles = []
     for w in ws:
@@ -53,8 +56,14 @@ def test_batch_graph_matches_single_sweeps_and_oracle(oracle_mod):
     with _lib.Batch() as B:
         for w in ws:
             B.add(w.f32, w.i32, w.grid, w.tri, w.vtx, w.vnrm, w.ypr)
+        B.set_mode(B.QUEUE if mode == "queue" else B.CHAIN)
         B.launch()
         B.sync()
+        rounds, units = B.plan()
+        if mode == "queue":
+            assert 1 <= rounds <= units and units >= len(ws)
+        else:
+            assert (rounds, units) == (0, 0)
         first = [B.records(m) for m in range(len(ws))]
         B.launch()  # a second launch of the same graph gives the same results
         B.sync()
@@ -75,12 +84,14 @@ def test_batch_graph_matches_single_sweeps_and_oracle(oracle_mod):
 
 
 @pytest.mark.gpu
-def test_members_reuse_larger_layouts_in_any_order(oracle_mod):
+@pytest.mark.parametrize("mode", ["queue", "chain"])
+def test_members_reuse_larger_layouts_in_any_order(oracle_mod, mode):
     """Members share one scratch block and each lays its slot table out over the previous
     member's data, so every table a sweep reads must be written by that sweep first.  A 512-grid
     member with 24 inline entries and 256-tall tiles runs first, then a 256-grid member with 8
     inline entries, a 128-grid member with 16, and the 512-grid one again with 8: graph results
-    must equal single sweeps with default shapes, and the oracle."""
+    must equal single sweeps with default shapes, and the oracle.  In the persistent form the
+    units of a round lie side by side, and every round's tables lie over the previous round's."""
     from tomo_shell2026_amd import _lib
     ypr = synth.ypr_grid(np.arange(0, 360, 24), np.arange(0, 61, 20))  # 15 x 4
     big_v, big_f = synth.displaced_sphere_n(3000, 260.0, seed=11)
@@ -92,7 +103,7 @@ def test_members_reuse_larger_layouts_in_any_order(oracle_mod):
     ws = [a, b, c, a]
     shapes = [(len(ypr), 256, 24), (40, 32, 8), (7, 0, 16), (len(ypr), 64, 8)]
     assert [w.grid for w in ws] == [512, 256, 128, 512]
-    print("[gpu-case] batch member order: grids 512/256/128/512, inline 24/8/16/8", flush=True)
+    print(f"[gpu-case] batch member order ({mode}): grids 512/256/128/512, inline 24/8/16/8", flush=True)
     singles = []
     for w in ws[:3]:
         with _lib.Sweep(w.f32, w.i32, w.grid) as s:
@@ -102,6 +113,7 @@ def test_members_reuse_larger_layouts_in_any_order(oracle_mod):
     with _lib.Batch() as B:
         for w in ws:
             B.add(w.f32, w.i32, w.grid, w.tri, w.vtx, w.vnrm, w.ypr)
+        B.set_mode(B.QUEUE if mode == "queue" else B.CHAIN)
         for m, (bs, ty, inl) in enumerate(shapes):
             B.set_tuning(m, batch=bs, tile_y=ty, inline=inl)
             got = B.tuning(m)
@@ -138,7 +150,10 @@ def test_batch70_workload(oracle_mod):
         B.launch()
         B.sync()
         graph = [B.records(m) for m in range(len(ws))]
-    print(f"[gpu-case] batch70: graph launched at {time.time() - t0:.1f} s", flush=True)
+        rounds, units = B.plan()
+    print(f"[gpu-case] batch70: graph launched at {time.time() - t0:.1f} s ({rounds} rounds of {units} units)",
+          flush=True)
+    assert 1 <= rounds < units
     for m, w in enumerate(ws):
         with _lib.Sweep(w.f32, w.i32, w.grid) as s:
             s.set_mesh(w.tri, w.vtx, w.vnrm)

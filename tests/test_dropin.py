@@ -72,9 +72,17 @@ def test_find_optimals_rounds_to_tenth_gram():
     ypr = np.arange(12, dtype=np.float32).reshape(4, 3)
     mss = np.array([5.04, 5.01, 4.99, 7.0], np.float32)
     best, worst = findOptimals(ypr, mss)
-    # 5.04 and 5.01 and 4.99 all round to 5.0: argsort picks among ties, 7.0 is the worst
-    assert best.GetMss() in (mss[0], mss[1], mss[2])
+    # The reference ranks np.round(Mss * 10) * 0.1 with numpy's default argsort (tomo_io.py:50-58).
+    # 5.04, 5.01 and 4.99 all round to 5.0, so the first of the tied entries is numpy's choice on
+    # this machine (its sort kernels differ by version and CPU): the drop-in must make the same
+    # choice, which an argmin of the raw masses (index 2) or a stable sort would not guarantee.
+    rounded = np.round(mss * 10.0) * 0.1
+    assert best.GetMss() == mss[rounded.argsort()[0]]
+    assert best.vmData.yaw == ypr[rounded.argsort()[0], 0]
     assert worst.vmData.yaw == ypr[3, 0]
+    # without ties the order is the masses' own: 4.2 is the best, 7.0 the worst
+    best2, worst2 = findOptimals(ypr, np.array([5.04, 4.2, 7.0, 5.3], np.float32))
+    assert best2.vmData.yaw == ypr[1, 0] and worst2.vmData.yaw == ypr[2, 0]
     b, w = smallestN_indices(np.array([3.0, 1.0, 2.0]), 2)
     assert list(b) == [1, 2] and list(w) == [0, 2]
 

@@ -33,7 +33,7 @@ EXPORTS = (
     "getMat4x4", "getnData2i", "getpData2i", "OnDestroy",
     "tomo_cvss_create", "tomo_cvss_forward", "tomo_cvss_backward", "tomo_cvss_destroy",
     "tomo_batch_create", "tomo_batch_add", "tomo_batch_tune", "tomo_batch_get_tuning", "tomo_batch_set_tuning",
-    "tomo_batch_capture",
+    "tomo_batch_set_mode", "tomo_batch_get_plan", "tomo_batch_capture",
     "tomo_batch_launch", "tomo_batch_sync", "tomo_batch_last_ms", "tomo_batch_size", "tomo_batch_records",
     "tomo_batch_destroy",
 )
@@ -99,6 +99,8 @@ def load():
         "tomo_batch_tune": ((P, I32), ct.c_int),
         "tomo_batch_get_tuning": ((P, I32, P, P, P, P), ct.c_int),
         "tomo_batch_set_tuning": ((P, I32, I32, I32, I32), ct.c_int),
+        "tomo_batch_set_mode": ((P, I32), ct.c_int),
+        "tomo_batch_get_plan": ((P, P, P), ct.c_int),
         "tomo_batch_capture": ((P,), ct.c_int),
         "tomo_batch_launch": ((P,), ct.c_int),
         "tomo_batch_sync": ((P,), ct.c_int),
@@ -297,6 +299,19 @@ class Batch:
     def set_tuning(self, member, batch=0, tile_y=0, inline=16):
         check(load().tomo_batch_set_tuning(self._h, int(member), int(batch), int(tile_y), int(inline)),
               "tomo_batch_set_tuning")
+
+    QUEUE, CHAIN = 0, 1  # TOMO_BATCH_QUEUE, TOMO_BATCH_CHAIN
+
+    def set_mode(self, mode):
+        """Batch.QUEUE: persistent work-queue launches per round of units (the default);
+        Batch.CHAIN: each member's own chain of stage launches."""
+        check(load().tomo_batch_set_mode(self._h, int(mode)), "tomo_batch_set_mode")
+
+    def plan(self):
+        """(rounds, units) of the last persistent capture."""
+        r, u = ct.c_int32(), ct.c_int32()
+        check(load().tomo_batch_get_plan(self._h, ct.byref(r), ct.byref(u)), "tomo_batch_get_plan")
+        return r.value, u.value
 
     def capture(self):
         check(load().tomo_batch_capture(self._h), "tomo_batch_capture")

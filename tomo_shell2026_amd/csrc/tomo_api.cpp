@@ -17,6 +17,10 @@
 namespace tomo {
 hipError_t launch_stage(int stage, const SweepParams &p, hipStream_t stream);
 hipError_t static_scratch_ptr(int device, uint8_t **out);
+int32_t queue_items(int k, const SweepParams &p);
+hipError_t queue_occupancy(int k, int *per_cu);
+hipError_t launch_queue(int k, const UnitQueue &q, int32_t groups, hipStream_t stream);
+hipError_t launch_clear_heads(uint32_t *heads, int32_t n, hipStream_t stream);
 struct CvssMeshDev {
   const float *nx, *ny, *nz, *area, *cx, *cy, *cz, *vx, *vy, *vz;
   int32_t n_tri, n_vtx;
@@ -1046,6 +1050,13 @@ struct tomo_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool timed = false;
+  int32_t mode = TOMO_BATCH_QUEUE;
+  // persistent launch: every unit's parameters, the per-round per-kernel item tables and the
+  // queue counters (device), and the number of rounds captured
+  SweepParams *d_units = nullptr;
+  int32_t *d_first = nullptr;
+  uint32_t *d_heads = nullptr;
+  int32_t rounds = 0, units = 0;
 };
 
 static void batch_drop_graph(tomo_batch *b) {
@@ -1076,7 +1087,11 @@ static void batch_unshare(tomo_batch *b) {
     }
   if (b->d_scratch) { scratch_release(b->d_scratch, b->scratch_cap, b->device); b->d_scratch = nullptr; }
   if (b->d_hdr) { (void)hipFree(b->d_hdr); b->d_hdr = nullptr; }
+  if (b->d_units) { (void)hipFree(b->d_units); b->d_units = nullptr; }
+  if (b->d_first) { (void)hipFree(b->d_first); b->d_first = nullptr; }
+  if (b->d_heads) { (void)hipFree(b->d_heads); b->d_heads = nullptr; }
   b->scratch_bytes = b->hdr_count = 0;
+  b->rounds = b->units = 0;
 }
 
 int tomo_batch_create(tomo_batch **out) {
@@ -1242,15 +1257,27 @@ int tomo_batch_set_tuning(tomo_batch *b, int32_t member, int32_t batch, int32_t 
 }
 
 // Record every member's sweep (launch shapes from the tuner, or the defaults) into one graph.
-int tomo_batch_capture(tomo_batch *b) {
-  if (!b) return fail(TOMO_E_INVALID_ARG, "null batch");
-  if (b->members.empty()) return fail(TOMO_E_STATE, "empty batch");
-  batch_unshare(b);
+// Attach the members to the batch's stream and shared buffers.
+static void batch_share(tomo_batch *b) {
+  for (size_t m = 0; m < b->members.size(); m++) {
+    tomo_handle *h = b->members[m];
+    h->shared = true;
+    h->stream = b->stream;
+    h->d_scratch = b->d_scratch;
+    h->d_hdr = b->d_hdr;
+    h->scratch_batch = b->tune[m].batch;
+    h->scratch_region = h->lay.bytes;
+    h->phases = false;
+    free_stage_events(h);
+  }
+}
+
+// Per-member capture: each member's sweep as its own chain of stage launches, member after
+// member, every member's tables laid over the shared block from its start.
+static int batch_capture_chain(tomo_batch *b) {
   size_t sbytes = 0, hcount = 0;
   for (size_t m = 0; m < b->members.size(); m++) {
     tomo_handle *h = b->members[m];
-    release_private_scratch(h);
-    apply_tune(h, b->tune[m]);
     const size_t bs = (size_t)b->tune[m].batch;
     sbytes = std::max(sbytes, (size_t)h->lay.bytes * bs);
     hcount = std::max(hcount, bs);
@@ -1268,17 +1295,7 @@ int tomo_batch_capture(tomo_batch *b) {
   HIP_TRY(hipMemsetAsync(b->d_scratch, 0, sbytes, b->stream));
   HIP_TRY(hipMemsetAsync(b->d_hdr, 0, sizeof(OrientHdr) * hcount, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
-  for (size_t m = 0; m < b->members.size(); m++) {
-    tomo_handle *h = b->members[m];
-    h->shared = true;
-    h->stream = b->stream;
-    h->d_scratch = b->d_scratch;
-    h->d_hdr = b->d_hdr;
-    h->scratch_batch = b->tune[m].batch;
-    h->scratch_region = h->lay.bytes;
-    h->phases = false;
-    free_stage_events(h);
-  }
+  batch_share(b);
   HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   int rc = TOMO_OK;
   for (size_t m = 0; m < b->members.size() && rc == TOMO_OK; m++) {
@@ -1294,12 +1311,175 @@ int tomo_batch_capture(tomo_batch *b) {
   }
   if (ec != hipSuccess) return fail(TOMO_E_HIP, std::string("graph capture: ") + hipGetErrorString(ec));
   b->graph = g;
+  return TOMO_OK;
+}
+
+// Persistent capture.  Every member's orientations are cut into units of its launch shape's
+// orientations in flight; consecutive units are packed into rounds whose slot tables lie side by
+// side in the shared block (which is as large as the largest single unit needs, so no round asks
+// for more memory than the per-member form).  Each round is swept by one persistent work-queue
+// launch per stage kernel over all its units' items (mesh, orientation-tile), and the counters
+// of every queue are cleared by one kernel at the start of the graph.
+static int batch_capture_queue(tomo_batch *b) {
+  struct Unit {
+    size_t m;
+    int32_t b0, n;
+    size_t bytes;
+  };
+  auto align = [](size_t v) { return (v + 255) & ~(size_t)255; };
+  std::vector<Unit> units;
+  size_t budget = 0;
+  for (size_t m = 0; m < b->members.size(); m++) {
+    tomo_handle *h = b->members[m];
+    const int32_t bs = std::max<int32_t>(1, b->tune[m].batch);
+    budget = std::max(budget, align((size_t)h->lay.bytes * (size_t)bs));
+    for (int32_t b0 = 0; b0 < h->n_orient; b0 += bs) {
+      const int32_t n = std::min(bs, h->n_orient - b0);
+      units.push_back({m, b0, n, align((size_t)h->lay.bytes * (size_t)n)});
+    }
+  }
+  std::vector<std::pair<size_t, size_t>> rounds;  // [first unit, end)
+  size_t hmax = 0;
+  for (size_t i = 0; i < units.size();) {
+    size_t j = i, bytes = 0, hc = 0;
+    while (j < units.size() && (j == i || bytes + units[j].bytes <= budget)) {
+      bytes += units[j].bytes;
+      hc += (size_t)units[j].n;
+      j++;
+    }
+    rounds.push_back({i, j});
+    hmax = std::max(hmax, hc);
+    i = j;
+  }
+  // work-groups of each persistent kernel resident on the whole device
+  int ncu = 0;
+  HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, b->device));
+  int32_t resident[QK_COUNT];
+  for (int k = 0; k < QK_COUNT; k++) {
+    int per = 0;
+    HIP_TRY(queue_occupancy(k, &per));
+    resident[k] = std::max(8, (std::max(per, 1) * ncu) / 8 * 8);
+  }
+  if (scratch_alloc(budget, b->device, &b->d_scratch, &b->scratch_cap) != hipSuccess) {
+    b->d_scratch = nullptr;
+    return fail(TOMO_E_OOM, "shared scratch allocation failed");
+  }
+  const size_t nfirst = (units.size() + rounds.size()) * QK_COUNT;
+  const size_t nheads = rounds.size() * QK_COUNT * 8;
+  if (hipMalloc(&b->d_hdr, sizeof(OrientHdr) * hmax) != hipSuccess ||
+      hipMalloc(&b->d_units, sizeof(SweepParams) * units.size()) != hipSuccess ||
+      hipMalloc(&b->d_first, sizeof(int32_t) * nfirst) != hipSuccess ||
+      hipMalloc(&b->d_heads, sizeof(uint32_t) * nheads) != hipSuccess) {
+    (void)hipGetLastError();
+    batch_unshare(b);
+    return fail(TOMO_E_OOM, "shared buffer allocation failed");
+  }
+  b->scratch_bytes = budget;
+  b->hdr_count = hmax;
+  // unit parameters and, per round and kernel, the first item of every unit
+  std::vector<SweepParams> P(units.size());
+  std::vector<int32_t> F;
+  F.reserve(nfirst);
+  std::vector<size_t> foff(rounds.size() * QK_COUNT);
+  std::vector<int32_t> items(rounds.size() * QK_COUNT);
+  for (size_t r = 0; r < rounds.size(); r++) {
+    size_t off = 0, hoff = 0;
+    for (size_t i = rounds[r].first; i < rounds[r].second; i++) {
+      const Unit &u = units[i];
+      tomo_handle *h = b->members[u.m];
+      SweepParams p = base_params(h);
+      p.scratch = b->d_scratch + off;
+      p.hdr = b->d_hdr + hoff;
+      p.n_batch = u.n;
+      p.rot = h->d_rot + (size_t)u.b0 * 9;
+      p.rec = h->d_rec + u.b0;
+      P[i] = p;
+      off += u.bytes;
+      hoff += (size_t)u.n;
+    }
+    for (int k = 0; k < QK_COUNT; k++) {
+      foff[r * QK_COUNT + k] = F.size();
+      int64_t acc = 0;
+      F.push_back(0);
+      for (size_t i = rounds[r].first; i < rounds[r].second; i++) {
+        acc += queue_items(k, P[i]);
+        if (acc > INT32_MAX) {
+          batch_unshare(b);
+          return fail(TOMO_E_INVALID_ARG, "too many work items in one round");
+        }
+        F.push_back((int32_t)acc);
+      }
+      items[r * QK_COUNT + k] = (int32_t)acc;
+    }
+  }
+  HIP_TRY(hipMemcpy(b->d_units, P.data(), sizeof(SweepParams) * P.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(b->d_first, F.data(), sizeof(int32_t) * F.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemsetAsync(b->d_scratch, 0, budget, b->stream));
+  HIP_TRY(hipMemsetAsync(b->d_hdr, 0, sizeof(OrientHdr) * hmax, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  batch_share(b);
+  HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t e = launch_clear_heads(b->d_heads, (int32_t)nheads, b->stream);
+  for (size_t r = 0; r < rounds.size() && e == hipSuccess; r++) {
+    const int32_t nu = (int32_t)(rounds[r].second - rounds[r].first);
+    for (int k = 0; k < QK_COUNT && e == hipSuccess; k++) {
+      const int32_t it = items[r * QK_COUNT + k];
+      if (it == 0) continue;
+      UnitQueue q;
+      q.units = b->d_units + rounds[r].first;
+      q.first = b->d_first + foff[r * QK_COUNT + k];
+      q.n_units = nu;
+      q.heads = b->d_heads + (r * QK_COUNT + (size_t)k) * 8;
+      const int32_t groups = std::min(resident[k], (it + 7) / 8 * 8);
+      e = launch_queue(k, q, groups, b->stream);
+    }
+  }
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(b->stream, &g);
+  if (e != hipSuccess || ec != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return fail(TOMO_E_HIP, std::string("graph capture: ") + hipGetErrorString(e != hipSuccess ? e : ec));
+  }
+  b->graph = g;
+  b->rounds = (int32_t)rounds.size();
+  b->units = (int32_t)units.size();
+  return TOMO_OK;
+}
+
+int tomo_batch_capture(tomo_batch *b) {
+  if (!b) return fail(TOMO_E_INVALID_ARG, "null batch");
+  if (b->members.empty()) return fail(TOMO_E_STATE, "empty batch");
+  batch_unshare(b);
+  for (size_t m = 0; m < b->members.size(); m++) {
+    tomo_handle *h = b->members[m];
+    release_private_scratch(h);
+    apply_tune(h, b->tune[m]);
+  }
+  const int rc = b->mode == TOMO_BATCH_QUEUE ? batch_capture_queue(b) : batch_capture_chain(b);
+  if (rc) return rc;
   HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
   for (tomo_handle *h : b->members) {
     h->last_begin = 0;
     h->last_end = h->n_orient;
     h->timed = false;
   }
+  return TOMO_OK;
+}
+
+int tomo_batch_set_mode(tomo_batch *b, int32_t mode) {
+  if (!b) return fail(TOMO_E_INVALID_ARG, "null batch");
+  if (mode != TOMO_BATCH_QUEUE && mode != TOMO_BATCH_CHAIN) return fail(TOMO_E_INVALID_ARG, "bad batch mode");
+  if (mode != b->mode) {
+    batch_unshare(b);
+    b->mode = mode;
+  }
+  return TOMO_OK;
+}
+
+int tomo_batch_get_plan(tomo_batch *b, int32_t *rounds, int32_t *units) {
+  if (!b) return fail(TOMO_E_INVALID_ARG, "null batch");
+  if (rounds) *rounds = b->rounds;
+  if (units) *units = b->units;
   return TOMO_OK;
 }
 

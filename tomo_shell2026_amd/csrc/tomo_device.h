@@ -105,4 +105,20 @@ struct SweepParams {
   int32_t exact_fsum;      // keep per-slot values and rebuild float totals when they round
 };
 
+// The stage kernels of the batched launch as persistent work-queue kernels: one launch per stage
+// serves every unit (a slice of one mesh's orientations) of a round.  An item is one work-group's
+// share of one unit (an orientation-tile for the rasteriser, an orientation for aabb), numbered
+// unit after unit from first[u]; every first[u] is a multiple of 8 and item g belongs to the
+// queue of XCD g & 7 (the unit's xcd_map), taken by the work-groups with blockIdx & 7 == g & 7.
+enum QueueKernel : int {
+  QK_AABB = 0, QK_RASTER, QK_COMPACT, QK_PAIR, QK_BIG, QK_HUGE, QK_BED, QK_FSUM, QK_COUNT
+};
+
+struct UnitQueue {
+  const SweepParams *units;  // [n_units] each unit's parameters (device memory)
+  const int32_t *first;      // [n_units + 1] first item of each unit; first[n_units] = items
+  int32_t n_units;
+  uint32_t *heads;           // [8] items taken per XCD queue; zero when the launch starts
+};
+
 }  // namespace tomo

@@ -664,8 +664,8 @@ __device__ void mid_slot(int32_t s, uint32_t c, const Tables &t, int32_t G2, int
 // orientation b = (i mod 8) + 8 * ((i / 8) / per) and part (i / 8) mod per, so every part of an
 // orientation runs on the same XCD (its slot table stays in that XCD's L2) and each XCD walks
 // its orientations one after another.  The grid is padded to a multiple of 8 orientations.
-__device__ __forceinline__ bool xcd_map(int32_t per, int32_t nb, int32_t &b, int32_t &part) {
-  const int32_t i = (int32_t)blockIdx.x, x = i & 7, j = i >> 3;
+__device__ __forceinline__ bool xcd_map(int32_t blk, int32_t per, int32_t nb, int32_t &b, int32_t &part) {
+  const int32_t x = blk & 7, j = blk >> 3;
   b = x + 8 * (j / per);
   part = j % per;
   return b < nb;
@@ -739,10 +739,11 @@ __device__ __forceinline__ int32_t group_or(int32_t f, int32_t *red, int lane, i
 // Rotates every vertex and reduces the rotated AABB minimum (STomoSh_Base.cpp:77-109) and
 // resets the orientation's header and record.  The slot counters need no reset: the rasteriser
 // writes every counter of its tile, zeros included.
-__global__ __launch_bounds__(kThreads) void aabb_kernel(SweepParams p) {
+__device__ __forceinline__ void aabb_body(const SweepParams &p, int32_t blk) {
   __shared__ float red[kWaves][3];
   const int tid = threadIdx.x, lane = tid & 63, wave = wave_index();
-  const int32_t b = blockIdx.x;
+  const int32_t b = blk;
+  if (b >= p.n_batch) return;
   const float *Rg = p.rot + (int64_t)b * 9;
   const float r0 = Rg[0], r1 = Rg[1], r2 = Rg[2], r3 = Rg[3], r4 = Rg[4], r5 = Rg[5], r6 = Rg[6],
               r7 = Rg[7], r8 = Rg[8];
@@ -849,11 +850,11 @@ __device__ __forceinline__ void chunk_span(const float *bmin, const float *bmax,
   }
 }
 
-__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6))) void raster_kernel(SweepParams p) {
+__device__ __forceinline__ void raster_body(const SweepParams &p, int32_t blk) {
   __shared__ RasterLds L;
   const int tid = threadIdx.x, lane = tid & 63, wave = wave_index();
   int32_t b, tile;
-  if (!xcd_map(p.tiles, p.n_batch, b, tile)) return;
+  if (!xcd_map(blk, p.tiles, p.n_batch, b, tile)) return;
   const int32_t G = p.grid;
   const int32_t ti = tile / p.tiles_y, tj = tile - ti * p.tiles_y;
   const int32_t xa = ti * p.tile_x, xb = min(xa + p.tile_x, G);
@@ -1212,11 +1213,12 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6)
 // arrival order; group them per slot (exclusive scan over the large-slot list, then scatter).
 constexpr int kCompactCap = 4096;  // large slots whose overflow cursors fit in LDS
 
-__global__ __launch_bounds__(kThreads) void compact_kernel(SweepParams p) {
+__device__ __forceinline__ void compact_body(const SweepParams &p, int32_t blk) {
   __shared__ uint32_t scan[kWaves];
   __shared__ uint32_t lcur[kCompactCap];  // overflow cursor of each large-slot list entry
   const int tid = threadIdx.x, lane = tid & 63, wave = wave_index();
-  const int32_t b = blockIdx.x;
+  const int32_t b = blk;
+  if (b >= p.n_batch) return;
   OrientHdr &h = p.hdr[b];
   const Tables t = tables_for(p, b);
   const uint32_t nspill = h.spill_top;
@@ -1288,7 +1290,7 @@ __global__ __launch_bounds__(kThreads) void compact_kernel(SweepParams p) {
 // of the touched box; pair_big_kernel (PART 1, a second launch with its own register budget)
 // pairs the large-slot list.
 template <int PART>
-__device__ __forceinline__ void pair_body(const SweepParams &p) {
+__device__ __forceinline__ void pair_body(const SweepParams &p, int32_t blk) {
   __shared__ uint64_t sortbuf[kWaves][kBigCap];
   __shared__ uint32_t pairq[3][kWaves][128];  // queued slots: s | entries << 24
   __shared__ uint32_t pairk[2][kWaves][128];  // keys of the queued slots of at most 2 entries
@@ -1297,7 +1299,7 @@ __device__ __forceinline__ void pair_body(const SweepParams &p) {
   const int tid = threadIdx.x, lane = tid & 63, wave = wave_index();
   int32_t b, tile;
   const int32_t groups = PART == 0 ? p.pair_tiles : p.big_groups;  // work-groups per orientation
-  if (!xcd_map(groups, p.n_batch, b, tile)) return;
+  if (!xcd_map(blk, groups, p.n_batch, b, tile)) return;
   const int32_t G = p.grid, G2 = G * G;
   const OrientHdr &h = p.hdr[b];
   const int32_t bx0 = h.bbox[0], bx1 = h.bbox[1], by0 = h.bbox[2], by1 = h.bbox[3];
@@ -1559,12 +1561,6 @@ __device__ __forceinline__ void pair_body(const SweepParams &p) {
   if (tid == 0 && f) atomicOr(&R.flags, f);
 }
 
-__global__ __launch_bounds__(kThreads, 5) void pair_kernel(SweepParams p) { pair_body<0>(p); }
-// Four waves per SIMD (128 VGPRs, a few spilled) beat three without spills: the kernel waits on
-// memory far more than it computes.
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void pair_big_kernel(SweepParams p) {
-  pair_body<1>(p);
-}
 
 // ---- pair_huge_kernel: one work-group per orientation -------------------------------------
 // Pairs the slots pair_big_kernel listed: those with more than kBigCap entries (rows from many
@@ -1669,7 +1665,7 @@ __device__ int wave_distinct_by_rank(uint64_t *buf, int n, int lane) {
   return D;
 }
 
-__global__ __launch_bounds__(kThreads) void pair_huge_kernel(SweepParams p) {
+__device__ __forceinline__ void pair_huge_body(const SweepParams &p, int32_t blk) {
   __shared__ uint64_t hbuf[kHugeLds];
   __shared__ uint32_t wcnt[kWaves];
   __shared__ uint32_t sel[64];  // kept keys, ascending (selection in passes)
@@ -1678,7 +1674,7 @@ __global__ __launch_bounds__(kThreads) void pair_huge_kernel(SweepParams p) {
   __shared__ int64_t red[kWaves][18];
   __shared__ int32_t fred[kWaves];
   const int tid = threadIdx.x, lane = tid & 63, wave = wave_index();
-  const int32_t b = blockIdx.x;
+  const int32_t b = blk;
   if (b >= p.n_batch) return;
   const OrientHdr &h = p.hdr[b];
   const uint32_t nh = min(h.huge_top, (uint32_t)p.lay.nhuge);
@@ -1855,7 +1851,7 @@ __global__ __launch_bounds__(kThreads) void pair_huge_kernel(SweepParams p) {
 // ---- bed_kernel: band_rows grid rows of the bed scan region per work-group ----------------
 // GenerateBed / IsBedCandidate (STomoSh_INT3.cpp:1331-1443) with the raster-order effect: a raft
 // source slot that takes its own bed row stops being a source for slots scanned after it.
-__global__ __launch_bounds__(kThreads) void bed_kernel(SweepParams p) {
+__device__ __forceinline__ void bed_body(const SweepParams &p, int32_t blk) {
   __shared__ uint16_t nb[kBedLdsSlots];  // bed bookkeeping (bed_info bits) of the band and halo
   // Source bits of the halo rows (one bit per column, filled by ballots), then their OR over
   // each column's half-open window [y - r, y + r): an exact "no source can be near" test.
@@ -1866,7 +1862,7 @@ __global__ __launch_bounds__(kThreads) void bed_kernel(SweepParams p) {
   __shared__ int32_t fred[kWaves];
   const int tid = threadIdx.x, lane = tid & 63, wave = wave_index();
   int32_t b, band;
-  if (!xcd_map(p.bed_bands, p.n_batch, b, band)) return;
+  if (!xcd_map(blk, p.bed_bands, p.n_batch, b, band)) return;
   const int32_t G = p.grid;
   const OrientHdr &h = p.hdr[b];
   const int32_t bx0 = h.bbox[0], bx1 = h.bbox[1], by0 = h.bbox[2], by1 = h.bbox[3];
@@ -2049,12 +2045,13 @@ __global__ __launch_bounds__(kThreads) void bed_kernel(SweepParams p) {
 // exactly in integers, lane-parallel, while the bound holds for all of them, and the serial
 // replay (lane q owns total q) starts at the first block where it fails for one, from the exact
 // sums before it.
-__global__ __launch_bounds__(64) void fsum_kernel(SweepParams p) {
+__device__ __forceinline__ void fsum_body(const SweepParams &p, int32_t blk) {
   constexpr int kAhead = 8;                 // chunks of 64 slots in flight per lane
   constexpr int kBlock = 8;                 // chunks per block of the exact first pass
   __shared__ int32_t vals[6][65];           // padded: lanes 0..5 read one column each
   const int lane = threadIdx.x;
-  const int32_t b = blockIdx.x;
+  const int32_t b = blk;
+  if (b >= p.n_batch) return;
   const OrientHdr &h = p.hdr[b];
   DevRecord &R = p.rec[b];
   const int64_t lim = (int64_t)1 << 24;
@@ -2161,6 +2158,96 @@ __global__ __launch_bounds__(64) void fsum_kernel(SweepParams p) {
   if (lane == 0) R.fsum_done = 1;
 }
 
+// ---- kernel entry points ------------------------------------------------------------------
+// One sweep slice per launch: work-group i serves block i.
+__global__ __launch_bounds__(kThreads) void aabb_kernel(SweepParams p) { aabb_body(p, (int32_t)blockIdx.x); }
+__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6))) void raster_kernel(SweepParams p) {
+  raster_body(p, (int32_t)blockIdx.x);
+}
+__global__ __launch_bounds__(kThreads) void compact_kernel(SweepParams p) { compact_body(p, (int32_t)blockIdx.x); }
+__global__ __launch_bounds__(kThreads, 5) void pair_kernel(SweepParams p) { pair_body<0>(p, (int32_t)blockIdx.x); }
+// Four waves per SIMD (128 VGPRs, a few spilled) beat three without spills: the kernel waits on
+// memory far more than it computes.
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void pair_big_kernel(SweepParams p) {
+  pair_body<1>(p, (int32_t)blockIdx.x);
+}
+__global__ __launch_bounds__(kThreads) void pair_huge_kernel(SweepParams p) { pair_huge_body(p, (int32_t)blockIdx.x); }
+__global__ __launch_bounds__(kThreads) void bed_kernel(SweepParams p) { bed_body(p, (int32_t)blockIdx.x); }
+__global__ __launch_bounds__(64) void fsum_kernel(SweepParams p) { fsum_body(p, (int32_t)blockIdx.x); }
+
+// Persistent form: each work-group takes items from its XCD's queue (thread 0 counts them off
+// with one atomic, the unit is found by binary search over first[]) until the queue is empty.
+// Every work-group leaves the loop when its queue runs out; no item waits on another.  The unit
+// table comes in as a read-only, unaliased kernel argument, so its fields are read with scalar
+// loads wherever the body needs them, as from a kernel's own parameter block.
+#define TOMO_QUEUE_ARGS                                                                              \
+  const SweepParams *__restrict__ units, const int32_t *__restrict__ first, int32_t n_units, \
+      uint32_t *__restrict__ heads
+template <typename Body>
+__device__ __forceinline__ void drain(const SweepParams *__restrict__ units, const int32_t *__restrict__ first,
+                                      int32_t n_units, uint32_t *__restrict__ heads, Body body) {
+  __shared__ int32_t s_item, s_unit;
+  const int32_t x = (int32_t)blockIdx.x & 7;
+  const int32_t items = first[n_units];
+  for (;;) {
+    __syncthreads();  // the previous item is done in every thread (its LDS and s_item are free)
+    if (threadIdx.x == 0) {
+      const int64_t g = (int64_t)x + 8 * (int64_t)atomicAdd(&heads[x], 1u);
+      int32_t u = -1;
+      if (g < items) {  // the last unit whose first item is at most g (empty units are skipped)
+        int32_t lo = 0, hi = n_units - 1;
+        while (lo < hi) {
+          const int32_t mid = (lo + hi + 1) >> 1;
+          if (first[mid] <= g) lo = mid;
+          else hi = mid - 1;
+        }
+        u = lo;
+      }
+      s_item = u >= 0 ? (int32_t)g : -1;
+      s_unit = u;
+    }
+    __syncthreads();
+    const int32_t g = __builtin_amdgcn_readfirstlane(s_item), u = __builtin_amdgcn_readfirstlane(s_unit);
+    if (g < 0) break;
+    // The table stays unchanged while the kernel runs: read it through the constant address
+    // space, as the parameter block is, so its fields load into scalar registers after barriers.
+    const __attribute__((address_space(4))) SweepParams *cu =
+        (const __attribute__((address_space(4))) SweepParams *)units;
+    body(*(const SweepParams *)&cu[u], g - first[u]);
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void aabb_queue(TOMO_QUEUE_ARGS) {
+  drain(units, first, n_units, heads, [](const SweepParams &p, int32_t blk) { aabb_body(p, blk); });
+}
+__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6))) void raster_queue(TOMO_QUEUE_ARGS) {
+  drain(units, first, n_units, heads, [](const SweepParams &p, int32_t blk) { raster_body(p, blk); });
+}
+__global__ __launch_bounds__(kThreads) void compact_queue(TOMO_QUEUE_ARGS) {
+  drain(units, first, n_units, heads, [](const SweepParams &p, int32_t blk) { compact_body(p, blk); });
+}
+__global__ __launch_bounds__(kThreads, 5) void pair_queue(TOMO_QUEUE_ARGS) {
+  drain(units, first, n_units, heads, [](const SweepParams &p, int32_t blk) { pair_body<0>(p, blk); });
+}
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void pair_big_queue(TOMO_QUEUE_ARGS) {
+  drain(units, first, n_units, heads, [](const SweepParams &p, int32_t blk) { pair_body<1>(p, blk); });
+}
+__global__ __launch_bounds__(kThreads) void pair_huge_queue(TOMO_QUEUE_ARGS) {
+  drain(units, first, n_units, heads, [](const SweepParams &p, int32_t blk) { pair_huge_body(p, blk); });
+}
+__global__ __launch_bounds__(kThreads) void bed_queue(TOMO_QUEUE_ARGS) {
+  drain(units, first, n_units, heads, [](const SweepParams &p, int32_t blk) { bed_body(p, blk); });
+}
+__global__ __launch_bounds__(64) void fsum_queue(TOMO_QUEUE_ARGS) {
+  drain(units, first, n_units, heads, [](const SweepParams &p, int32_t blk) { fsum_body(p, blk); });
+}
+
+// Zeroes n queue counters (a kernel, not a memset node, so a captured graph replays it exactly).
+__global__ void clear_heads_kernel(uint32_t *heads, int32_t n) {
+  const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < n) heads[i] = 0u;
+}
+
 // Device address of `scratch` on `device`, which must be the current device the first time it
 // is asked for: looked up once per device with hipGetSymbolAddress and kept (every GPU holds its
 // own copy of the array at its own address).
@@ -2178,6 +2265,58 @@ hipError_t static_scratch_ptr(int device, uint8_t **out) {
   }
   *out = (uint8_t *)addr[device];
   return hipSuccess;
+}
+
+static const void *queue_kernel(int k) {
+  switch (k) {
+    case QK_AABB: return (const void *)aabb_queue;
+    case QK_RASTER: return (const void *)raster_queue;
+    case QK_COMPACT: return (const void *)compact_queue;
+    case QK_PAIR: return (const void *)pair_queue;
+    case QK_BIG: return (const void *)pair_big_queue;
+    case QK_HUGE: return (const void *)pair_huge_queue;
+    case QK_BED: return (const void *)bed_queue;
+    case QK_FSUM: return (const void *)fsum_queue;
+    default: return nullptr;
+  }
+}
+
+int queue_threads(int k) { return k == QK_RASTER ? kTileThreads : (k == QK_FSUM ? 64 : kThreads); }
+
+// Items of unit p for queue kernel k (xcd_map pads the orientation count to whole XCD rounds).
+int32_t queue_items(int k, const SweepParams &p) {
+  const int32_t nb8 = (p.n_batch + 7) / 8 * 8;
+  switch (k) {
+    case QK_AABB: case QK_COMPACT: case QK_HUGE: return nb8;
+    case QK_RASTER: return nb8 * p.tiles;
+    case QK_PAIR: return nb8 * p.pair_tiles;
+    case QK_BIG: return nb8 * p.big_groups;
+    case QK_BED: return p.bed_on ? nb8 * p.bed_bands : 0;
+    case QK_FSUM: return p.exact_fsum ? nb8 : 0;
+    default: return 0;
+  }
+}
+
+// Work-groups of queue kernel k resident on one CU at once.
+hipError_t queue_occupancy(int k, int *per_cu) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, queue_kernel(k), queue_threads(k), 0);
+}
+
+hipError_t launch_queue(int k, const UnitQueue &q, int32_t groups, hipStream_t stream) {
+  const void *f = queue_kernel(k);
+  if (!f || groups < 8 || groups % 8 != 0) return hipErrorInvalidValue;
+  const SweepParams *units = q.units;
+  const int32_t *first = q.first;
+  int32_t n_units = q.n_units;
+  uint32_t *heads = q.heads;
+  void *args[] = {(void *)&units, (void *)&first, (void *)&n_units, (void *)&heads};
+  return hipLaunchKernel(f, dim3((unsigned)groups), dim3((unsigned)queue_threads(k)), args, 0, stream);
+}
+
+hipError_t launch_clear_heads(uint32_t *heads, int32_t n, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(clear_heads_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, heads, n);
+  return hipGetLastError();
 }
 
 hipError_t launch_stage(int stage, const SweepParams &p, hipStream_t stream) {

@@ -115,14 +115,10 @@ TOMO_RHD bool scan_candidate(const TriScan &t, float rax, float rbx, float e2y) 
   return !(a < t.lo || b < t.lo || a + b > t.his);
 }
 
-// Upper bound on 1 / x for finite x > 0 (at least 2^-10 above it).
-TOMO_RHD float rcp_up(float x) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  return __builtin_amdgcn_rcpf(x) * 1.0009765625f;
-#else
-  return (1.0f / x) * 1.0009765625f;
-#endif
-}
+// Upper bound on 1 / x for finite x > 0: the correctly rounded quotient (within 2^-24 of 1 / x)
+// raised by 2^-10, the same operations on the host and the device, so the host tests of the
+// accept box cover the kernel's arithmetic exactly.
+TOMO_RHD float rcp_up(float x) { return (1.0f / x) * 1.0009765625f; }
 
 // Offsets from v0 that an accepted pixel centre can have: a tightening of the pixel box.
 // Every accepted pixel satisfies a >= lo, b >= lo and a + b <= his in real arithmetic (the
