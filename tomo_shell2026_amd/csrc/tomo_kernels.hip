@@ -113,12 +113,17 @@ struct Tables {
   int32_t nhuge;  // huge list capacity
 };
 
-struct Acc {
-  int64_t Va, Vb, Vo, Vss, Vnv, Vtc, nbed, nent, nslot, nfinal;
+// Per-lane totals.  T = int32_t where a lane accounts for a bounded number of slots: a slot's
+// values stay below 2^22 in magnitude (at most 64 rows of |z| <= 2^15), so up to 2^9 slots per
+// lane fit; the work-group reduction widens them to 64 bits.
+template <typename T>
+struct AccT {
+  T Va, Vb, Vo, Vss, Vnv, Vtc, nbed, nent, nslot, nfinal;
   uint64_t chkpre, chkfinal;
-  int64_t aVo, aVss, aVa, aVb, aVnv, aVtc;
+  T aVo, aVss, aVa, aVb, aVnv, aVtc;
   int32_t flags;
 };
+using Acc = AccT<int64_t>;
 
 __device__ __forceinline__ int64_t iabs64(int32_t v) { return v < 0 ? -(int64_t)v : (int64_t)v; }
 
@@ -139,10 +144,15 @@ __device__ __forceinline__ bool totals_may_round(const OrientHdr &h, const PairP
   return (pp.explicit_ss ? h.zabs : 2ull * h.zabs) > (1ull << 24);
 }
 
-__device__ __forceinline__ void account(Acc &a, const SlotResult &r, int32_t s, const SweepParams &p) {
+// The sums of |per-slot value| only matter when they can pass 2^24 (the float totals may round,
+// and the record says so); otherwise they are left at zero (track false).
+template <typename T>
+__device__ __forceinline__ void account(AccT<T> &a, const SlotResult &r, int32_t s, const SweepParams &p, bool track) {
   a.Va += r.va; a.Vb += r.vb; a.Vo += r.vo; a.Vss += r.vss; a.Vnv += r.vnv; a.Vtc += r.vtc;
-  a.aVo += iabs64(r.vo); a.aVss += iabs64(r.vss); a.aVa += iabs64(r.va); a.aVb += iabs64(r.vb);
-  a.aVnv += iabs64(r.vnv); a.aVtc += iabs64(r.vtc);
+  if (track) {
+    a.aVo += (T)iabs64(r.vo); a.aVss += (T)iabs64(r.vss); a.aVa += (T)iabs64(r.va); a.aVb += (T)iabs64(r.vb);
+    a.aVnv += (T)iabs64(r.vnv); a.aVtc += (T)iabs64(r.vtc);
+  }
   a.nent += r.npre;
   a.chkpre += (uint64_t)(s + 1) * (uint64_t)r.npre;
   a.flags |= r.flags;
@@ -1317,8 +1327,12 @@ __device__ __forceinline__ void pair_body(const SweepParams &p, int32_t blk) {
   if (PART == 0 ? i0 >= nbb : (uint32_t)(tile * kWaves) >= nbig) return;  // nothing for this group
   const Tables t = tables_for(p, b);
   const bool final_here = !p.bed_on;  // without a bed the final row count is known here
-  int32_t *const sv = (p.exact_fsum && totals_may_round(h, p.pp)) ? t.sv : nullptr;
-  Acc acc = {};
+  const bool track = totals_may_round(h, p.pp);
+  int32_t *const sv = (p.exact_fsum && track) ? t.sv : nullptr;
+  // pair_kernel: a lane pairs at most one slot per batch of its wave's span (at most 2,048
+  // slots, so at most 2^6 slots per lane in 3 classes); pair_big_kernel's lanes can account for
+  // a whole wave's large slots and keep 64-bit totals.
+  AccT<typename std::conditional<PART == 0, int32_t, int64_t>::type> acc = {};
   uint64_t *const dbg_rows = p.dbg_rows;
   // Pair one slot of at most N entries in registers (one lane).  Slots of at most 2 entries
   // come with their keys (ka, kb: 0 past the count); larger ones read theirs here.
@@ -1335,7 +1349,7 @@ __device__ __forceinline__ void pair_body(const SweepParams &p, int32_t blk) {
     }
     SlotResult r;
     pair_keys<N>(k, (int)c, p.pp, r, dbg_rows ? dbg_rows + (int64_t)s * 64 : nullptr);
-    account(acc, r, s, p);
+    account(acc, r, s, p, track);
     t.bedinfo[s] = (uint16_t)bed_info(r, p.pp);
     if (sv) put_sv(sv, G2, s, r);
     if (final_here) {
@@ -1472,7 +1486,7 @@ __device__ __forceinline__ void pair_body(const SweepParams &p, int32_t blk) {
     SlotResult r;
     pair_keys<16>(k, (int)c, p.pp, r, p.dbg_rows ? p.dbg_rows + (int64_t)s * 64 : nullptr);
     acc.nslot++;
-    account(acc, r, s, p);
+    account(acc, r, s, p, track);
     t.bedinfo[s] = (uint16_t)bed_info(r, p.pp);
     if (sv) put_sv(sv, G2, s, r);
     if (final_here) {
@@ -1485,7 +1499,7 @@ __device__ __forceinline__ void pair_body(const SweepParams &p, int32_t blk) {
   // most 32 entries are paired two per wave (one per 32-lane half), the rest one per wave.
   auto finish_slot = [&](int32_t s, const SlotResult &r) {
     acc.nslot++;
-    account(acc, r, s, p);
+    account(acc, r, s, p, track);
     t.bedinfo[s] = (uint16_t)bed_info(r, p.pp);
     if (sv) put_sv(sv, G2, s, r);
     if (final_here) {
@@ -1682,7 +1696,8 @@ __device__ __forceinline__ void pair_huge_body(const SweepParams &p, int32_t blk
   const Tables t = tables_for(p, b);
   const int32_t G2 = p.grid * p.grid;
   const bool final_here = !p.bed_on;
-  int32_t *const sv = (p.exact_fsum && totals_may_round(h, p.pp)) ? t.sv : nullptr;
+  const bool track = totals_may_round(h, p.pp);
+  int32_t *const sv = (p.exact_fsum && track) ? t.sv : nullptr;
   Acc acc = {};
   for (uint32_t k = 0; k < nh; k++) {
     const int32_t s = t.huge[k];
@@ -1827,7 +1842,7 @@ __device__ __forceinline__ void pair_huge_body(const SweepParams &p, int32_t blk
       r.flags |= fl;
       if (lane == 0) {
         acc.nslot++;
-        account(acc, r, s, p);
+        account(acc, r, s, p, track);
         t.bedinfo[s] = (uint16_t)bed_info(r, p.pp);
         if (sv) put_sv(sv, G2, s, r);
         if (final_here) {
