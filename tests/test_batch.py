@@ -84,6 +84,37 @@ def test_batch_graph_matches_single_sweeps_and_oracle(oracle_mod, mode):
 
 
 @pytest.mark.gpu
+def test_tuned_shapes_cache(tmp_path):
+    """Batch.tune(cache=...) writes the tuned shapes per group; a later batch of the same groups
+    applies them without timing anything and sweeps to the same records."""
+    from tomo_shell2026_amd import _lib
+    ws = _small_members()[:2]
+    path = str(tmp_path / "shapes.json")
+    print("[gpu-case] tuned-shape cache", flush=True)
+    with _lib.Batch() as B:
+        for w in ws:
+            B.add(w.f32, w.i32, w.grid, w.tri, w.vtx, w.vnrm, w.ypr)
+        B.tune(sample=256, cache=path)
+        first = [B.tuning(m) for m in range(len(ws))]
+        B.launch()
+        ref = [B.records(m) for m in range(len(ws))]
+    assert all(t["us_per_orientation"] > 0 for t in first)
+    with _lib.Batch() as B:
+        for w in ws:
+            B.add(w.f32, w.i32, w.grid, w.tri, w.vtx, w.vnrm, w.ypr)
+        B.tune(sample=256, cache=path)
+        again = [B.tuning(m) for m in range(len(ws))]
+        B.launch()
+        got = [B.records(m) for m in range(len(ws))]
+    for a, b in zip(first, again):
+        assert (a["batch"], a["tile_y"], a["inline"]) == (b["batch"], b["tile_y"], b["inline"])
+        assert b["us_per_orientation"] < 0  # applied from the cache, not timed
+    for m, w in enumerate(ws):
+        bad = C.compare_records(got[m], ref[m], f"{w.name} cached shapes: ")
+        assert not bad, "\n".join(bad)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("mode", ["queue", "chain"])
 def test_members_reuse_larger_layouts_in_any_order(oracle_mod, mode):
     """Members share one scratch block and each lays its slot table out over the previous

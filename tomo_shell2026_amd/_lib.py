@@ -271,6 +271,7 @@ class Batch:
         check(load().tomo_batch_create(ct.byref(h)), "tomo_batch_create")
         self._h = h
         self._n = []
+        self._keys = []
 
     def add(self, f32, i32, grid, tri, vtx, vnrm, ypr):
         f = np.ascontiguousarray(f32, np.float32)
@@ -283,10 +284,41 @@ class Batch:
         check(load().tomo_batch_add(self._h, ptr(f), ptr(i), int(grid), t.shape[0], ptr(t), v.shape[0], ptr(v),
                                     ptr(n), y.shape[0], ptr(y), ct.byref(m)), "tomo_batch_add")
         self._n.append(y.shape[0])
+        # the tuner's group (grid, ceil(log2 nTri)) and the options that change the work per slot
+        lg = int(np.ceil(np.log2(max(t.shape[0], 1))))
+        self._keys.append(f"grid{int(grid)}_lg{lg}_shell{int(i[2])}_ex{int(i[1])}_bed{int(i[9])}")
         return m.value
 
-    def tune(self, sample=4096):
+    def tune(self, sample=4096, cache=None):
+        """Pick every member's launch shape (tomo_batch_tune).  With `cache` (a JSON file path),
+        shapes found there for every member's group are applied without timing anything, and a
+        fresh tuning is written back for the next process."""
+        import json
+        import os
+        if cache and os.path.exists(cache):
+            with open(cache) as fh:
+                known = json.load(fh)
+            if all(k in known for k in self._keys):
+                for m, k in enumerate(self._keys):
+                    bs, ty, inl = known[k]
+                    self.set_tuning(m, batch=min(int(bs), self._n[m]), tile_y=int(ty), inline=int(inl))
+                return
         check(load().tomo_batch_tune(self._h, int(sample)), "tomo_batch_tune")
+        if cache:
+            known = {}
+            if os.path.exists(cache):
+                with open(cache) as fh:
+                    known = json.load(fh)
+            fresh = {}
+            for m, k in enumerate(self._keys):  # a group shares one shape; batch clipped per member
+                t = self.tuning(m)
+                bs = max(t["batch"], fresh[k][0]) if k in fresh else t["batch"]
+                fresh[k] = [bs, t["tile_y"], t["inline"]]
+            known.update(fresh)
+            tmp = cache + ".tmp"
+            with open(tmp, "w") as fh:
+                json.dump(known, fh, indent=1, sort_keys=True)
+            os.replace(tmp, cache)
 
     def tuning(self, member):
         b, t, i = ct.c_int32(), ct.c_int32(), ct.c_int32()

@@ -1256,7 +1256,6 @@ int tomo_batch_set_tuning(tomo_batch *b, int32_t member, int32_t batch, int32_t 
   return TOMO_OK;
 }
 
-// Record every member's sweep (launch shapes from the tuner, or the defaults) into one graph.
 // Attach the members to the batch's stream and shared buffers.
 static void batch_share(tomo_batch *b) {
   for (size_t m = 0; m < b->members.size(); m++) {

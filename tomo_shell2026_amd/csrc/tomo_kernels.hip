@@ -2180,7 +2180,9 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6)
   raster_body(p, (int32_t)blockIdx.x);
 }
 __global__ __launch_bounds__(kThreads) void compact_kernel(SweepParams p) { compact_body(p, (int32_t)blockIdx.x); }
-__global__ __launch_bounds__(kThreads, 5) void pair_kernel(SweepParams p) { pair_body<0>(p, (int32_t)blockIdx.x); }
+// Six waves per SIMD (80 VGPRs): the few registers spilled are the accumulators' epilogue, after
+// the step loop; measured 4.5% faster than five waves without spills.
+__global__ __launch_bounds__(kThreads, 6) void pair_kernel(SweepParams p) { pair_body<0>(p, (int32_t)blockIdx.x); }
 // Four waves per SIMD (128 VGPRs, a few spilled) beat three without spills: the kernel waits on
 // memory far more than it computes.
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void pair_big_kernel(SweepParams p) {
