@@ -432,9 +432,10 @@ def main():
     if not args.no_batch and args.workload == "dsphere":
         b = bench_batch70(argparse.Namespace(steps=2, warmup=1))
         progress(f"batch70: {b['value']:.0f} orientations/s")
-        out["batch70"] = {k: b[k] for k in ("value", "unit", "config", "graph_ms", "plain_per_mesh_ms",
-                                            "graph_vs_plain_speedup", "tune_s", "capture_s", "tuned_shapes",
-                                            "flags", "matches_oracle_on_sample", "oracle_sample")}
+        out["batch70"] = {k: b[k] for k in ("value", "unit", "config", "graph_ms", "rounds", "units",
+                                            "chain_graph_ms", "chain_matches_queue", "queue_vs_chain_speedup",
+                                            "plain_per_mesh_ms", "graph_vs_plain_speedup", "tune_s", "capture_s",
+                                            "tuned_shapes", "flags", "matches_oracle_on_sample", "oracle_sample")}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, rec, args.cpu_stride)
         progress(f"cpu baseline: {out['cpu_baseline']['value']} orientations/s")

@@ -445,10 +445,13 @@ SweepParams base_params(tomo_handle *h) {
   p.tiles = p.tiles_x * p.tiles_y;
   // Pairing work-groups: 8,192 touched-box slots each up to a 256 grid, 1,024 above (measured on
   // the 69k-triangle sphere at 256 against 4,096 and 16,384: 3% faster than either; on the
-  // thin-shell manikin at 512, 3% faster than 2,048); an eighth as many for the large-slot list.
+  // thin-shell manikin at 512, 3% faster than 2,048).  The large-slot list gets a quarter as
+  // many up to a 256 grid (two per orientation at 256: 10% faster than one, four and eight were
+  // slower) and an eighth above.
   p.pair_span = h->opt.grid <= 256 ? 8192 : 1024;
   p.pair_tiles = (g2 + p.pair_span - 1) / p.pair_span;
-  p.big_groups = p.pair_tiles >= 8 ? p.pair_tiles / 8 : 1;
+  const int32_t big_div = h->opt.grid <= 256 ? 4 : 8;
+  p.big_groups = p.pair_tiles >= big_div ? p.pair_tiles / big_div : 1;
   // Bed bands of 40 rows up to a 256 grid, where a band and its halo fit the bed kernel's LDS
   // staging for bed radii up to 6 (on the 69k-triangle sphere 15% faster than 16 rows; 32 and
   // 48 were slower), 16 rows above.
