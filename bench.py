@@ -166,11 +166,11 @@ def cpu_baseline(w, records, stride, runs=3):
     dt1 = time.perf_counter() - t0
     same = same and bool(np.array_equal(ref1["Vss"], records["Vss"][idx1]))
     return {"value": round(len(idx) / med, 2), "unit": "orientations/s", "cores": threads, "kind": "port",
-            "sample": f"every {stride}th orientation of the same sweep ({len(idx)} orientations), median of "
+            "sample": f"one orientation in {stride} of the same sweep ({len(idx)} orientations), median of "
                       f"{runs} runs ({', '.join(f'{t:.2f}' for t in times)} s wall), oracle in C, "
                       f"orientation-parallel pthreads",
             "one_thread": {"value": round(len(idx1) / dt1, 2), "unit": "orientations/s",
-                           "sample": f"every {stride * 20}th orientation ({len(idx1)}, {dt1:.1f} s wall)"},
+                           "sample": f"one orientation in {stride * 20} ({len(idx1)}, {dt1:.1f} s wall)"},
             "cpu_model": model, "nproc": nproc, "cpus_available": avail,
             "matches_gpu_on_sample": same}
 
