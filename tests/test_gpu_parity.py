@@ -169,6 +169,31 @@ def test_footprint_out_of_grid(oracle_mod):
     assert ((r["flags"] & 4) != 0).all()
 
 
+@pytest.mark.parametrize("kind", ["tall_box", "tall_stack"])
+def test_int16_wrap_is_flagged(oracle_mod, kind):
+    """Where the reference's int16 slot rows overflow the record must say so (INT16_WRAP) and
+    still equal the oracle bit for bit: a box 40,000 voxels tall (its z values leave int16) and
+    40 squares 50 voxels apart (each slot's sum of |z| passes 32,767)."""
+    if kind == "tall_box":
+        v, f = C.box((0, 0, 0), (12, 9, 40000))
+        case = C.make(v, f, 32, synth.ypr_grid([0, 90], [0, 90]), bed=synth.BED_RAFT)
+    else:
+        v, f = C.stacked_planes(n_layers=40, size=6.0, dz=50.0)
+        case = C.make(v, f, 32, synth.ypr_grid([0, 180], [0, 180]), bed=synth.BED_NONE)
+    g, r = assert_parity(oracle_mod, case, f"int16 wrap: {kind}")
+    assert ((r["flags"] & 2) != 0).any()
+
+
+def test_large_mesh_600k(oracle_mod):
+    """A 599,808-triangle displaced sphere on a 512 grid (the size of the reference's bug-report
+    meshes): about 600,000 entries per orientation, slots at the 63-row cap, dropped floor
+    sentinels and float totals past 2^24 (flags 25 at several orientations)."""
+    v, f = synth.displaced_sphere_n(600000, 240.0, seed=21)
+    case = C.make(v, f, 512, synth.ypr_grid([0, 37, 90, 200], [0, 45]), bed=synth.BED_RAFT)
+    g, r = assert_parity(oracle_mod, case, f"displaced sphere {len(f)} triangles, 512 grid")
+    assert ((r["flags"] & 8) != 0).any() and ((r["flags"] & 16) != 0).any()
+
+
 def test_empty_mesh(oracle_mod):
     announce("empty mesh")
     case = C.make(np.zeros((0, 3)), np.zeros((0, 3), np.int64), 32, C.coarse_ypr(90))
