@@ -446,12 +446,11 @@ SweepParams base_params(tomo_handle *h) {
   // Pairing work-groups: 8,192 touched-box slots each up to a 256 grid, 1,024 above (measured on
   // the 69k-triangle sphere at 256 against 4,096 and 16,384: 3% faster than either; on the
   // thin-shell manikin at 512, 3% faster than 2,048).  The large-slot list gets a quarter as
-  // many up to a 256 grid (two per orientation at 256: 10% faster than one, four and eight were
-  // slower) and an eighth above.
+  // many (two per orientation at 256: 10% faster than one, four and eight were slower; on the
+  // manikin 1% faster than an eighth, a sixteenth was 3% slower).
   p.pair_span = h->opt.grid <= 256 ? 8192 : 1024;
   p.pair_tiles = (g2 + p.pair_span - 1) / p.pair_span;
-  const int32_t big_div = h->opt.grid <= 256 ? 4 : 8;
-  p.big_groups = p.pair_tiles >= big_div ? p.pair_tiles / big_div : 1;
+  p.big_groups = p.pair_tiles >= 4 ? p.pair_tiles / 4 : 1;
   // Bed bands of 40 rows up to a 256 grid, where a band and its halo fit the bed kernel's LDS
   // staging for bed radii up to 6 (on the 69k-triangle sphere 15% faster than 16 rows; 32 and
   // 48 were slower), 16 rows above.
