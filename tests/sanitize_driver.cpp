@@ -6,7 +6,7 @@
 //   sanitize_driver CASES OUT
 //
 // CASES holds cases back to back: int32 {nTri, nVtx, nYPR}, the orc_opts bytes, tri[3 nTri]
-// (int32), vtx[3 nVtx], vnrm[3 nVtx], ypr[3 nYPR] (float32).  For each case the driver runs
+// (int32), vtx[3 nVtx], vnrm[3 nVtx], ypr[3 nYPR] (float32).  For each case the program runs
 // orc_sweep (two threads), orc_one_rows and orc_insert_log on the first orientation, and writes
 // the sweep's records to OUT.  It then pairs seeded random slots with every register network
 // and with the oracle's vslotPair, and walks seeded random triangles (ordinary, sliver, tiny,
