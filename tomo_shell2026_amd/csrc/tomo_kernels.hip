@@ -2050,7 +2050,6 @@ __device__ __forceinline__ void bed_body(const SweepParams &p, int32_t blk) {
   if (tid == 0 && f) atomicOr(&R.flags, f);
 }
 
-// Stage k of a batch: 0 aabb, 1 raster, 2 compact, 3 pair, 4 bed (skipped without a bed).
 // ---- fsum_kernel: one 64-lane work-group per orientation ----------------------------------
 // The reference accumulates per-slot integer totals into float running sums in ascending slot
 // order: Vo and Vss in Pairing (STomoSh_INT3.cpp:386-392), Va, Vb, Vnv and Vtc in Calculate
@@ -2336,6 +2335,8 @@ hipError_t launch_clear_heads(uint32_t *heads, int32_t n, hipStream_t stream) {
   return hipGetLastError();
 }
 
+// Stage k of one sweep slice: 0 aabb, 1 raster, 2 compact, 3 pair (small, large and huge
+// slots), 4 bed (skipped without a bed), 5 ordered float totals (skipped unless kept).
 hipError_t launch_stage(int stage, const SweepParams &p, hipStream_t stream) {
   const int32_t nb = p.n_batch;
   const int32_t nb8 = (nb + 7) / 8 * 8;  // xcd_map pads the orientation count to whole XCD rounds
