@@ -2062,7 +2062,7 @@ __device__ __forceinline__ void bed_body(const SweepParams &p, int32_t blk) {
 __device__ __forceinline__ void fsum_body(const SweepParams &p, int32_t blk) {
   constexpr int kAhead = 8;                 // chunks of 64 slots in flight per lane
   constexpr int kBlock = 8;                 // chunks per block of the exact first pass
-  __shared__ int32_t vals[6][65];           // padded: lanes 0..5 read one column each
+  __shared__ int32_t vals[6][65];           // padded: lanes 0..5 read one row each
   const int lane = threadIdx.x;
   const int32_t b = blk;
   if (b >= p.n_batch) return;
@@ -2079,26 +2079,24 @@ __device__ __forceinline__ void fsum_body(const SweepParams &p, int32_t blk) {
   const int32_t wy = by1 - by0 + 1;
   const int32_t nbb = bx1 >= bx0 ? (bx1 - bx0 + 1) * wy : 0;
   const int32_t nchunk = (nbb + 63) >> 6;
+  // Totals whose sum of |value| passes 2^24 need the replay; only their per-slot values are read.
+  bool need[6];
+#pragma unroll
+  for (int k = 0; k < 6; k++) need[k] = absv[k] > lim;
   // Slots in ascending id order: row-major over the bounding box (id = x * G + y).
   auto load = [&](int32_t c, int32_t (&v)[6]) {
     const int32_t i = (c << 6) + lane;
-    if (c < nchunk && i < nbb) {
-      const int32_t s = (bx0 + i / wy) * G + by0 + i % wy;
+    const bool in = c < nchunk && i < nbb;
+    const int32_t s = in ? (bx0 + i / wy) * G + by0 + i % wy : 0;
 #pragma unroll
-      for (int k = 0; k < 6; k++) v[k] = t.sv[(int64_t)k * G2 + s];
-    } else {
-#pragma unroll
-      for (int k = 0; k < 6; k++) v[k] = 0;
-    }
+    for (int k = 0; k < 6; k++) v[k] = (in && need[k]) ? t.sv[(int64_t)k * G2 + s] : 0;
   };
   // First pass (wave-uniform): exact integer sums S[k] of whole blocks while, for every total
   // that needs the replay, the sum of |value| so far stays within 2^24 (then every partial sum
   // before the block is exact); cmin is the first block where that fails for some total.
-  bool need[6];
   int64_t S[6], A[6];
 #pragma unroll
   for (int k = 0; k < 6; k++) {
-    need[k] = absv[k] > lim;
     S[k] = 0;
     A[k] = 0;
   }
@@ -2162,7 +2160,17 @@ __device__ __forceinline__ void fsum_body(const SweepParams &p, int32_t blk) {
         load(c + kAhead, buf[j]);
         if (lane < 6 && c >= mystart) {
           const int32_t n = min(64, nbb - (c << 6));
-          for (int32_t i = 0; i < n; i++) acc = acc + (float)vals[q][i];
+          if (n == 64) {
+            // a whole chunk: the 64 values are read and converted first, so the LDS reads are
+            // in flight together and only the additions form the serial chain
+            float f[64];
+#pragma unroll
+            for (int i = 0; i < 64; i++) f[i] = (float)vals[q][i];
+#pragma unroll
+            for (int i = 0; i < 64; i++) acc = acc + f[i];
+          } else {
+            for (int32_t i = 0; i < n; i++) acc = acc + (float)vals[q][i];
+          }
         }
         __syncthreads();
       }
