@@ -46,6 +46,40 @@ int harness_pair2(const uint32_t *keys, int c, int cs, int cn, int ex, int bt, i
 }
 uint32_t harness_make_key(int z, int nz) { return make_key(z, nz); }
 
+// pair_two against pair_keys<2> for every ordered pair of keys from a pool whose z and nZ values
+// straddle every decision the pairing makes (signs, the SS and NV cuts, the (1, 0) row, the
+// int16 ends), every count 0..2, both SS modes and every bed type and radius: returns the number
+// of disagreeing results (all SlotResult fields and the bed bits).
+int harness_pair_two_mismatches(int cs, int cn) {
+  const int zs[] = {-32768, -3, -1, 0, 1, 2, 5, 32767};
+  const int ns[] = {-32768, -1000, cs - 1, cs, cs + 1, cn - 1, cn, cn + 1, -2, -1, 0, 1, 2, 999, 32767};
+  uint32_t pool[sizeof zs / sizeof zs[0] * sizeof ns / sizeof ns[0]];
+  int np = 0;
+  for (int z : zs)
+    for (int n : ns) pool[np++] = make_key(z, std::max(-32768, std::min(32767, n)));
+  int bad = 0;
+  for (int ex = 0; ex < 2; ex++)
+    for (int bt = 0; bt < 4; bt++)
+      for (int br = 0; br < 4; br += 3) {
+        const PairParams pp{cs, cn, ex, bt, br};
+        for (int i = 0; i < np; i++)
+          for (int j = 0; j < np; j++)
+            for (int c = 0; c <= 2; c++) {
+              const uint32_t ka = c >= 1 ? pool[i] : 0u, kb = c >= 2 ? pool[j] : 0u;
+              uint32_t k[2] = {ka, kb};
+              SlotResult a, b;
+              pair_keys<2>(k, c, pp, a);
+              pair_two(ka, kb, c, pp, b);
+              const bool same = a.vo == b.vo && a.vss == b.vss && a.va == b.va && a.vb == b.vb &&
+                                a.vnv == b.vnv && a.vtc == b.vtc && a.npre == b.npre && a.n == b.n &&
+                                a.sabs == b.sabs && a.last_z == b.last_z && a.last_type == b.last_type &&
+                                a.has10 == b.has10 && a.flags == b.flags && bed_info(a, pp) == bed_info(b, pp);
+              bad += same ? 0 : 1;
+            }
+      }
+  return bad;
+}
+
 // Coverage test of the rasteriser over the pixel box [x0, x1] x [y0, y1] of one margined
 // triangle, pixels x-major as the kernel walks them: out[i*5] = prefilter candidate,
 // out[i*5+1] = exact accept, out[i*5+2..4] = u, v, w bits.

@@ -30,6 +30,8 @@ def harness():
         fn.restype = ct.c_int
     L.harness_make_key.argtypes = (ct.c_int, ct.c_int)
     L.harness_make_key.restype = ct.c_uint32
+    L.harness_pair_two_mismatches.argtypes = (ct.c_int, ct.c_int)
+    L.harness_pair_two_mismatches.restype = ct.c_int
     return L
 
 
@@ -56,6 +58,14 @@ def test_sort_network_zero_one(harness, n):
         out = hpair(harness, ks, n, -867, -866, True)
         srt = out[14:14 + n].astype(np.int64) & 0xffffffff
         assert list(srt) == sorted(srt, reverse=True)
+
+
+@pytest.mark.parametrize("theta_deg", [30.0, 45.0, 60.0, 75.0, 89.0])
+def test_two_entry_pairing_matches_network(harness, oracle_mod, theta_deg):
+    # the kernels' straight-line pairing of slots with at most two entries against the 2-key
+    # network, on every key pair of a pool straddling each decision, all counts, modes and beds
+    cs, cn = oracle_mod.theta_cuts(np.float32(theta_deg * np.pi / 180.0))
+    assert harness.harness_pair_two_mismatches(cs, cn) == 0
 
 
 def test_known_answer_slot_product(harness):

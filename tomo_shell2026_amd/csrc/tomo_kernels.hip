@@ -1348,7 +1348,8 @@ __device__ __forceinline__ void pair_body(const SweepParams &p, int32_t blk) {
       for (int q = 0; q < N; q++) k[q] = (q < (int)c) ? t.ent[(int64_t)q * G2 + s].x : 0u;
     }
     SlotResult r;
-    pair_keys<N>(k, (int)c, p.pp, r, dbg_rows ? dbg_rows + (int64_t)s * 64 : nullptr);
+    if (N == 2 && !dbg_rows) pair_two(k[0], k[1], (int)c, p.pp, r);  // straight-line form
+    else pair_keys<N>(k, (int)c, p.pp, r, dbg_rows ? dbg_rows + (int64_t)s * 64 : nullptr);
     account(acc, r, s, p, track);
     t.bedinfo[s] = (uint16_t)bed_info(r, p.pp);
     if (sv) put_sv(sv, G2, s, r);

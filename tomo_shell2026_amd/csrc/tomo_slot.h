@@ -256,4 +256,40 @@ TOMO_HD void pair_keys(uint32_t (&k)[N], int c, const PairParams &pp, SlotResult
   }
 }
 
+// pair_keys<2> without branches, for the slots of at most two entries (most slots of a closed
+// surface): the keys ka, kb (0 past the count c <= 2), the same SlotResult.  With two distinct
+// rows, the top one (larger key) is a kept alpha when nZ >= 1; the bottom one is then a kept
+// beta when nZ <= -1 (opening SSB / NVB at the cuts, each closed by a z = 0 floor row, SSA
+// before NVA), or else a kept alpha when the top one was not.  The first kept alpha is the TC
+// row of implicit SS.
+TOMO_HD void pair_two(uint32_t ka, uint32_t kb, int c, const PairParams &pp, SlotResult &r) {
+  const uint32_t k0 = ka > kb ? ka : kb, k1 = ka > kb ? kb : ka;
+  const uint32_t k10 = make_key(1, 0);
+  const bool v0 = c >= 1, two = c >= 2 && k1 != k0;  // row 0 exists; row 1 is distinct
+  const int32_t z0 = key_z(k0), n0 = key_nz(k0), z1 = key_z(k1), n1 = key_nz(k1);
+  r.npre = (v0 ? 1 : 0) + (two ? 1 : 0);
+  r.sabs = (v0 ? (z0 < 0 ? -z0 : z0) : 0) + (two ? (z1 < 0 ? -z1 : z1) : 0);
+  r.has10 = ((v0 && k0 == k10) || (two && k1 == k10)) ? 1 : 0;
+  r.flags = 0;
+  const bool a0 = two && n0 >= 1;          // row 0 a kept alpha
+  const bool a1 = two && !a0 && n1 >= 1;   // row 1 a kept alpha
+  const bool b1 = a0 && n1 <= -1;          // row 1 a kept beta
+  const bool ssb = b1 && n1 <= pp.cut_ssb, nvb = b1 && n1 >= pp.cut_nvb;
+  const int32_t al = a0 ? z0 : (a1 ? z1 : 0);
+  const int32_t be = b1 ? z1 : 0;
+  const int32_t tc = pp.explicit_ss ? 0 : al;
+  const int32_t tc_bit = pp.explicit_ss ? 0 : T_TC;
+  const int32_t ty1 = a1 ? (T_AL | tc_bit) : (b1 ? (T_BE | (ssb ? T_SSB : 0) | (nvb ? T_NVB : 0)) : 0);
+  r.n = two ? 2 + (ssb ? 1 : 0) + (nvb ? 1 : 0) : r.npre;
+  r.last_z = two ? ((ssb || nvb) ? 0 : z1) : (v0 ? z0 : 0);
+  r.last_type = two ? (nvb ? T_NVA : (ssb ? T_SSA : ty1)) : 0;
+  const int32_t zssb = ssb ? z1 : 0, znvb = nvb ? z1 : 0;
+  r.vo = al > be ? al - be : 0;
+  r.va = al;
+  r.vb = be;
+  r.vnv = znvb;
+  r.vss = pp.explicit_ss ? zssb : -al + be + tc - znvb;
+  r.vtc = tc;
+}
+
 }  // namespace tomo
