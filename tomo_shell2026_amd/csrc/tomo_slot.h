@@ -172,87 +172,163 @@ TOMO_HD void pair_keys(uint32_t (&k)[N], int c, const PairParams &pp, SlotResult
       }
     }
   }
-  const uint32_t k10 = make_key(1, 0);
-  int npre = 0;
-  int32_t sabs = 0, has10 = 0;
-#pragma unroll
-  for (int i = 0; i < N; i++) {
-    bool d = (i < c) && (i == 0 || k[i] != k[i - 1]);
-    if (d) {
-      npre++;
-      int32_t z = key_z(k[i]);
-      sabs += z < 0 ? -z : z;
-      if (k[i] == k10) has10 = 1;
-    }
-  }
-  r.npre = npre;
-  r.sabs = sabs;
-  r.has10 = has10;
-  r.flags = 0;
-  r.vo = r.vss = r.va = r.vb = r.vnv = r.vtc = 0;
-  if (npre <= 1) {  // Pairing skips slots with a single pixel (STomoSh_INT3.cpp:386)
-    r.n = npre;
-    r.last_z = npre ? key_z(k[0]) : 0;
-    r.last_type = 0;
-    if (rows && npre) rows[0] = make_row(0, k[0]);
-    return;
-  }
-  bool started = false, ex = false, im = false, tc_done = false;
-  int32_t al = 0, be = 0, ssb = 0, ssa = 0, nvb = 0, nva = 0, tc = 0;
-  int32_t last_z = 0, last_type = 0;
-  int row = 0;
-#pragma unroll
-  for (int i = 0; i < N; i++) {
-    bool d = (i < c) && (i == 0 || k[i] != k[i - 1]);
-    if (d) {
-      int32_t z = key_z(k[i]), nz = key_nz(k[i]);
-      bool isA = nz >= 1, isB = nz <= -1;
-      bool kA = false, kB = false;
-      if (!started && isA) { kA = true; started = true; }
-      else if (started && isB) { kB = true; started = false; }
-      int32_t ty = kA ? T_AL : (kB ? T_BE : 0);
-      if (kA) al += z;
-      if (kB) be += z;
-      if (kB) {
-        if (!ex && nz <= pp.cut_ssb) { ty |= T_SSB; ex = true; ssb += z; }
-        if (!im && nz >= pp.cut_nvb) { ty |= T_NVB; im = true; nvb += z; }
-      } else if (kA) {
-        if (ex) { ty |= T_SSA; ex = false; ssa += z; }
-        if (im) { ty |= T_NVA; im = false; nva += z; }
-        if (!pp.explicit_ss && !tc_done) { ty |= T_TC; tc_done = true; tc += z; }
+  if constexpr (N <= 4) {
+    // The rows in descending order, written with selects only (no divergent branches): a row
+    // is distinct (d) when it is valid and differs from the one before.
+    const uint32_t k10 = make_key(1, 0);
+    int npre = 0;
+    int32_t sabs = 0;
+    bool has10 = false;
+    bool started = false, ex = false, im = false, tc_done = false;
+    int32_t al = 0, be = 0, ssb = 0, ssa = 0, nvb = 0, nva = 0, tc = 0;
+    int32_t last_z = 0, last_type = 0;
+    const bool imp = !pp.explicit_ss;
+  #pragma unroll
+    for (int i = 0; i < N; i++) {
+      const bool d = (i < c) && (i == 0 || k[i] != k[i - 1]);
+      const int32_t z = key_z(k[i]), nz = key_nz(k[i]);
+      // an alpha is kept when no kept alpha is open, a beta when one is
+      const bool kA = d && !started && nz >= 1;
+      const bool kB = d && started && nz <= -1;
+      started = (started || kA) && !kB;
+      // a kept beta opens SSB / NVB below / above the cuts; the next kept alpha closes them
+      const bool sb = kB && !ex && nz <= pp.cut_ssb, nb = kB && !im && nz >= pp.cut_nvb;
+      const bool sa = kA && ex, na = kA && im;
+      const bool tcb = kA && imp && !tc_done;  // the first kept alpha is the TC row
+      ex = (ex || sb) && !sa;
+      im = (im || nb) && !na;
+      tc_done = tc_done || tcb;
+      al += kA ? z : 0;
+      be += kB ? z : 0;
+      ssb += sb ? z : 0;
+      nvb += nb ? z : 0;
+      ssa += sa ? z : 0;
+      nva += na ? z : 0;
+      tc += tcb ? z : 0;
+      const int32_t ty = (kA ? T_AL : 0) | (kB ? T_BE : 0) | (sb ? T_SSB : 0) | (nb ? T_NVB : 0) |
+                         (sa ? T_SSA : 0) | (na ? T_NVA : 0) | (tcb ? T_TC : 0);
+      if (rows) {
+        if (d) rows[npre] = make_row(ty, k[i]);
       }
-      last_z = z;
-      last_type = ty;
-      if (rows) rows[row] = make_row(ty, k[i]);
-      row++;
+      npre += d ? 1 : 0;
+      sabs += d ? (z < 0 ? -z : z) : 0;
+      has10 = has10 || (d && k[i] == k10);
+      last_z = d ? z : last_z;
+      last_type = d ? ty : last_type;
     }
-  }
-  int n = npre;
-  if (ex) {
-    if (n < kMaxPix) {
-      if (rows) rows[n] = make_row(T_SSA, make_key(0, 0));
-      n++; last_z = 0; last_type = T_SSA;
-    } else r.flags |= F_SENTINEL_DROP;
-  }
-  if (im) {
-    if (n < kMaxPix) {
-      if (rows) rows[n] = make_row(T_NVA, make_key(0, 0));
-      n++; last_z = 0; last_type = T_NVA;
-    } else r.flags |= F_SENTINEL_DROP;
-  }
-  r.n = n;
-  r.last_z = last_z;
-  r.last_type = last_type;
-  r.vo = al > be ? al - be : 0;
-  r.va = al;
-  r.vb = be;
-  r.vnv = nvb - nva;
-  if (pp.explicit_ss) {
-    r.vss = ssb - ssa;
-    r.vtc = 0;
+    r.npre = npre;
+    r.sabs = sabs;
+    r.has10 = has10 ? 1 : 0;
+    // Pairing skips slots with a single pixel (STomoSh_INT3.cpp:386): no types, no sums.
+    const bool paired = npre >= 2;
+    ex = ex && paired;
+    im = im && paired;
+    // open shadows get a z = 0 floor row (SSA, then NVA) while the slot has fewer than 63 rows
+    const bool fa = ex && npre < kMaxPix, fb = im && npre + (fa ? 1 : 0) < kMaxPix;
+    if (rows) {
+      if (!paired && npre) rows[0] = make_row(0, k[0]);
+      if (fa) rows[npre] = make_row(T_SSA, make_key(0, 0));
+      if (fb) rows[npre + (fa ? 1 : 0)] = make_row(T_NVA, make_key(0, 0));
+    }
+    r.flags = ((ex && !fa) || (im && !fb)) ? F_SENTINEL_DROP : 0;
+    r.n = npre + (fa ? 1 : 0) + (fb ? 1 : 0);
+    r.last_z = (fa || fb) ? 0 : (npre ? last_z : 0);
+    r.last_type = fb ? T_NVA : (fa ? T_SSA : (paired ? last_type : 0));
+    al = paired ? al : 0;
+    be = paired ? be : 0;
+    r.vo = al > be ? al - be : 0;
+    r.va = al;
+    r.vb = be;
+    r.vnv = paired ? nvb - nva : 0;
+    r.vss = !paired ? 0 : (pp.explicit_ss ? ssb - ssa : -al + be + tc - nvb + nva);
+    r.vtc = (paired && imp) ? tc : 0;
+
   } else {
-    r.vss = -al + be + tc - nvb + nva;
-    r.vtc = tc;
+  // Larger networks keep the branches: the selects of the form above hold every row's
+  // conditions as lane masks at once, and at 8 rows and more they spill.
+    const uint32_t k10 = make_key(1, 0);
+    int npre = 0;
+    int32_t sabs = 0, has10 = 0;
+  #pragma unroll
+    for (int i = 0; i < N; i++) {
+      bool d = (i < c) && (i == 0 || k[i] != k[i - 1]);
+      if (d) {
+        npre++;
+        int32_t z = key_z(k[i]);
+        sabs += z < 0 ? -z : z;
+        if (k[i] == k10) has10 = 1;
+      }
+    }
+    r.npre = npre;
+    r.sabs = sabs;
+    r.has10 = has10;
+    r.flags = 0;
+    r.vo = r.vss = r.va = r.vb = r.vnv = r.vtc = 0;
+    if (npre <= 1) {  // Pairing skips slots with a single pixel (STomoSh_INT3.cpp:386)
+      r.n = npre;
+      r.last_z = npre ? key_z(k[0]) : 0;
+      r.last_type = 0;
+      if (rows && npre) rows[0] = make_row(0, k[0]);
+      return;
+    }
+    bool started = false, ex = false, im = false, tc_done = false;
+    int32_t al = 0, be = 0, ssb = 0, ssa = 0, nvb = 0, nva = 0, tc = 0;
+    int32_t last_z = 0, last_type = 0;
+    int row = 0;
+  #pragma unroll
+    for (int i = 0; i < N; i++) {
+      bool d = (i < c) && (i == 0 || k[i] != k[i - 1]);
+      if (d) {
+        int32_t z = key_z(k[i]), nz = key_nz(k[i]);
+        bool isA = nz >= 1, isB = nz <= -1;
+        bool kA = false, kB = false;
+        if (!started && isA) { kA = true; started = true; }
+        else if (started && isB) { kB = true; started = false; }
+        int32_t ty = kA ? T_AL : (kB ? T_BE : 0);
+        if (kA) al += z;
+        if (kB) be += z;
+        if (kB) {
+          if (!ex && nz <= pp.cut_ssb) { ty |= T_SSB; ex = true; ssb += z; }
+          if (!im && nz >= pp.cut_nvb) { ty |= T_NVB; im = true; nvb += z; }
+        } else if (kA) {
+          if (ex) { ty |= T_SSA; ex = false; ssa += z; }
+          if (im) { ty |= T_NVA; im = false; nva += z; }
+          if (!pp.explicit_ss && !tc_done) { ty |= T_TC; tc_done = true; tc += z; }
+        }
+        last_z = z;
+        last_type = ty;
+        if (rows) rows[row] = make_row(ty, k[i]);
+        row++;
+      }
+    }
+    int n = npre;
+    if (ex) {
+      if (n < kMaxPix) {
+        if (rows) rows[n] = make_row(T_SSA, make_key(0, 0));
+        n++; last_z = 0; last_type = T_SSA;
+      } else r.flags |= F_SENTINEL_DROP;
+    }
+    if (im) {
+      if (n < kMaxPix) {
+        if (rows) rows[n] = make_row(T_NVA, make_key(0, 0));
+        n++; last_z = 0; last_type = T_NVA;
+      } else r.flags |= F_SENTINEL_DROP;
+    }
+    r.n = n;
+    r.last_z = last_z;
+    r.last_type = last_type;
+    r.vo = al > be ? al - be : 0;
+    r.va = al;
+    r.vb = be;
+    r.vnv = nvb - nva;
+    if (pp.explicit_ss) {
+      r.vss = ssb - ssa;
+      r.vtc = 0;
+    } else {
+      r.vss = -al + be + tc - nvb + nva;
+      r.vtc = tc;
+    }
+
   }
 }
 
