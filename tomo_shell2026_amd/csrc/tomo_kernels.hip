@@ -146,7 +146,7 @@ __device__ __forceinline__ bool totals_may_round(const OrientHdr &h, const PairP
 
 // The sums of |per-slot value| only matter when they can pass 2^24 (the float totals may round,
 // and the record says so); otherwise they are left at zero (track false).
-template <typename T>
+template <bool DBG = true, typename T>
 __device__ __forceinline__ void account(AccT<T> &a, const SlotResult &r, int32_t s, const SweepParams &p, bool track) {
   a.Va += r.va; a.Vb += r.vb; a.Vo += r.vo; a.Vss += r.vss; a.Vnv += r.vnv; a.Vtc += r.vtc;
   if (track) {
@@ -157,7 +157,7 @@ __device__ __forceinline__ void account(AccT<T> &a, const SlotResult &r, int32_t
   a.chkpre += (uint64_t)(s + 1) * (uint64_t)r.npre;
   a.flags |= r.flags;
   if (r.sabs > 32767) a.flags |= F_INT16_WRAP;
-  if (p.dbg_pre) {
+  if (DBG && p.dbg_pre) {
     p.dbg_pre[s] = r.npre;
     p.dbg_vo[s] = r.vo;
     p.dbg_vss[s] = r.vss;
@@ -1299,7 +1299,7 @@ __device__ __forceinline__ void compact_body(const SweepParams &p, int32_t blk) 
 // pair_kernel (PART 0) pairs the slots of at most kSmall entries found in the work-group's part
 // of the touched box; pair_big_kernel (PART 1, a second launch with its own register budget)
 // pairs the large-slot list.
-template <int PART>
+template <int PART, bool DBG>
 __device__ __forceinline__ void pair_body(const SweepParams &p, int32_t blk) {
   __shared__ uint64_t sortbuf[kWaves][kBigCap];
   __shared__ uint32_t pairq[3][kWaves][128];  // queued slots: s | entries << 24
@@ -1333,7 +1333,10 @@ __device__ __forceinline__ void pair_body(const SweepParams &p, int32_t blk) {
   // slots, so at most 2^6 slots per lane in 3 classes); pair_big_kernel's lanes can account for
   // a whole wave's large slots and keep 64-bit totals.
   AccT<typename std::conditional<PART == 0, int32_t, int64_t>::type> acc = {};
-  uint64_t *const dbg_rows = p.dbg_rows;
+  // The debug dumps (rows, final row counts, per-slot sums) exist only in the DBG instances,
+  // which the host launches when a dump was asked for: the others keep no registers for them.
+  uint64_t *const dbg_rows = DBG ? p.dbg_rows : nullptr;
+  int32_t *const dbg_final = DBG ? p.dbg_final : nullptr;
   // Pair one slot of at most N entries in registers (one lane).  Slots of at most 2 entries
   // come with their keys (ka, kb: 0 past the count); larger ones read theirs here.
   auto pair_small = [&](auto nt, int32_t s, uint32_t c, uint32_t ka, uint32_t kb) {
@@ -1350,13 +1353,13 @@ __device__ __forceinline__ void pair_body(const SweepParams &p, int32_t blk) {
     SlotResult r;
     if (N == 2 && !dbg_rows) pair_two(k[0], k[1], (int)c, p.pp, r);  // straight-line form
     else pair_keys<N>(k, (int)c, p.pp, r, dbg_rows ? dbg_rows + (int64_t)s * 64 : nullptr);
-    account(acc, r, s, p, track);
+    account<DBG>(acc, r, s, p, track);
     t.bedinfo[s] = (uint16_t)bed_info(r, p.pp);
     if (sv) put_sv(sv, G2, s, r);
     if (final_here) {
       acc.nfinal += r.n;
       acc.chkfinal += (uint64_t)(s + 1) * (uint64_t)r.n;
-      if (p.dbg_final) p.dbg_final[s] = r.n;
+      if (dbg_final) dbg_final[s] = r.n;
     }
   };
   // The touched box is walked 64 slots per wave step; slots of at most kSmall entries are
@@ -1485,28 +1488,28 @@ __device__ __forceinline__ void pair_body(const SweepParams &p, int32_t blk) {
 #pragma unroll
     for (int q = 0; q < 16; q++) k[q] = (q < (int)c) ? t.ent[(int64_t)q * G2 + s].x : 0u;
     SlotResult r;
-    pair_keys<16>(k, (int)c, p.pp, r, p.dbg_rows ? p.dbg_rows + (int64_t)s * 64 : nullptr);
+    pair_keys<16>(k, (int)c, p.pp, r, dbg_rows ? dbg_rows + (int64_t)s * 64 : nullptr);
     acc.nslot++;
-    account(acc, r, s, p, track);
+    account<DBG>(acc, r, s, p, track);
     t.bedinfo[s] = (uint16_t)bed_info(r, p.pp);
     if (sv) put_sv(sv, G2, s, r);
     if (final_here) {
       acc.nfinal += r.n;
       acc.chkfinal += (uint64_t)(s + 1) * (uint64_t)r.n;
-      if (p.dbg_final) p.dbg_final[s] = r.n;
+      if (dbg_final) dbg_final[s] = r.n;
     }
   }
   // Larger ones.  Each wave reads its share of the big list 64 entries at a time; slots of at
   // most 32 entries are paired two per wave (one per 32-lane half), the rest one per wave.
   auto finish_slot = [&](int32_t s, const SlotResult &r) {
     acc.nslot++;
-    account(acc, r, s, p, track);
+    account<DBG>(acc, r, s, p, track);
     t.bedinfo[s] = (uint16_t)bed_info(r, p.pp);
     if (sv) put_sv(sv, G2, s, r);
     if (final_here) {
       acc.nfinal += r.n;
       acc.chkfinal += (uint64_t)(s + 1) * (uint64_t)r.n;
-      if (p.dbg_final) p.dbg_final[s] = r.n;
+      if (dbg_final) dbg_final[s] = r.n;
     }
   };
   // A slot left to pair_huge_kernel (see big_slot).
@@ -1542,7 +1545,7 @@ __device__ __forceinline__ void pair_body(const SweepParams &p, int32_t blk) {
       const uint32_t cs = __shfl(lc, src);
       const uint32_t c = (upper && lb < 0) ? 0u : cs;  // an idle upper half pairs nothing
       SlotResult r;
-      mid_slot<32>(s, c, t, G2, lane, p.pp, r, (p.dbg_rows && c) ? p.dbg_rows + (int64_t)s * 64 : nullptr);
+      mid_slot<32>(s, c, t, G2, lane, p.pp, r, (dbg_rows && c) ? dbg_rows + (int64_t)s * 64 : nullptr);
       if ((lane & 31) == 0 && c) finish_slot(s, r);
     }
     while (m64) {
@@ -1554,7 +1557,7 @@ __device__ __forceinline__ void pair_body(const SweepParams &p, int32_t blk) {
         if (lane == 0) defer(s);
         continue;
       }
-      uint64_t *const rows = p.dbg_rows ? p.dbg_rows + (int64_t)s * 64 : nullptr;
+      uint64_t *const rows = dbg_rows ? dbg_rows + (int64_t)s * 64 : nullptr;
       SlotResult r;
       if (c < 64u && (c <= (uint32_t)t.inl || pool_ok)) {
         mid_slot<64>(s, c, t, G2, lane, p.pp, r, rows);
@@ -2190,11 +2193,16 @@ __global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6)
 __global__ __launch_bounds__(kThreads) void compact_kernel(SweepParams p) { compact_body(p, (int32_t)blockIdx.x); }
 // Six waves per SIMD (80 VGPRs): the few registers spilled are the accumulators' epilogue, after
 // the step loop; measured 4.5% faster than five waves without spills.
-__global__ __launch_bounds__(kThreads, 6) void pair_kernel(SweepParams p) { pair_body<0>(p, (int32_t)blockIdx.x); }
+__global__ __launch_bounds__(kThreads, 6) void pair_kernel(SweepParams p) { pair_body<0, false>(p, (int32_t)blockIdx.x); }
 // Four waves per SIMD (128 VGPRs, a few spilled) beat three without spills: the kernel waits on
 // memory far more than it computes.
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void pair_big_kernel(SweepParams p) {
-  pair_body<1>(p, (int32_t)blockIdx.x);
+  pair_body<1, false>(p, (int32_t)blockIdx.x);
+}
+// The same two with the debug dumps, launched only when one was asked for.
+__global__ __launch_bounds__(kThreads, 6) void pair_kernel_dbg(SweepParams p) { pair_body<0, true>(p, (int32_t)blockIdx.x); }
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void pair_big_kernel_dbg(SweepParams p) {
+  pair_body<1, true>(p, (int32_t)blockIdx.x);
 }
 __global__ __launch_bounds__(kThreads) void pair_huge_kernel(SweepParams p) { pair_huge_body(p, (int32_t)blockIdx.x); }
 __global__ __launch_bounds__(kThreads) void bed_kernel(SweepParams p) { bed_body(p, (int32_t)blockIdx.x); }
@@ -2252,10 +2260,10 @@ __global__ __launch_bounds__(kThreads) void compact_queue(TOMO_QUEUE_ARGS) {
   drain(units, first, n_units, heads, [](const SweepParams &p, int32_t blk) { compact_body(p, blk); });
 }
 __global__ __launch_bounds__(kThreads, 5) void pair_queue(TOMO_QUEUE_ARGS) {
-  drain(units, first, n_units, heads, [](const SweepParams &p, int32_t blk) { pair_body<0>(p, blk); });
+  drain(units, first, n_units, heads, [](const SweepParams &p, int32_t blk) { pair_body<0, false>(p, blk); });
 }
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void pair_big_queue(TOMO_QUEUE_ARGS) {
-  drain(units, first, n_units, heads, [](const SweepParams &p, int32_t blk) { pair_body<1>(p, blk); });
+  drain(units, first, n_units, heads, [](const SweepParams &p, int32_t blk) { pair_body<1, false>(p, blk); });
 }
 __global__ __launch_bounds__(kThreads) void pair_huge_queue(TOMO_QUEUE_ARGS) {
   drain(units, first, n_units, heads, [](const SweepParams &p, int32_t blk) { pair_huge_body(p, blk); });
@@ -2353,11 +2361,14 @@ hipError_t launch_stage(int stage, const SweepParams &p, hipStream_t stream) {
     case 0: hipLaunchKernelGGL(aabb_kernel, dim3(nb), dim3(kThreads), 0, stream, p); break;
     case 1: hipLaunchKernelGGL(raster_kernel, dim3(nb8 * p.tiles), dim3(kTileThreads), 0, stream, p); break;
     case 2: hipLaunchKernelGGL(compact_kernel, dim3(nb), dim3(kThreads), 0, stream, p); break;
-    case 3:
-      hipLaunchKernelGGL(pair_kernel, dim3(nb8 * p.pair_tiles), dim3(kThreads), 0, stream, p);
-      hipLaunchKernelGGL(pair_big_kernel, dim3(nb8 * p.big_groups), dim3(kThreads), 0, stream, p);
+    case 3: {
+      const bool dbg = p.dbg_pre || p.dbg_final || p.dbg_rows;
+      hipLaunchKernelGGL(dbg ? pair_kernel_dbg : pair_kernel, dim3(nb8 * p.pair_tiles), dim3(kThreads), 0, stream, p);
+      hipLaunchKernelGGL(dbg ? pair_big_kernel_dbg : pair_big_kernel, dim3(nb8 * p.big_groups), dim3(kThreads), 0,
+                         stream, p);
       hipLaunchKernelGGL(pair_huge_kernel, dim3(nb), dim3(kThreads), 0, stream, p);
       break;
+    }
     case 4:
       if (p.bed_on) hipLaunchKernelGGL(bed_kernel, dim3(nb8 * p.bed_bands), dim3(kThreads), 0, stream, p);
       break;
