@@ -19,6 +19,9 @@ constexpr int kHugeLds = 8192;         // entries of a slot sorted in LDS by pai
                                        // are selected in passes over global memory
 constexpr int kHugeList = 1024;        // one triangle's entries of one slot (pair_huge_kernel)
 constexpr int kMaxBedR = 16;           // int(BedOuterBound) supported by the bed kernel
+constexpr int kBedThreads = 512;       // bed work-group: 8 waves; three 40 KB groups per CU give 6 waves
+                                       // per SIMD (measured faster than 4 or 16 waves per group)
+constexpr int kBedWaves = kBedThreads / 64;
 constexpr int kTileThreads = 768;      // rasteriser work-group: 12 waves
 constexpr int kTileWaves = kTileThreads / 64;
 constexpr int kTileSlots = 16384;      // rasteriser tile: slots with a 16-bit LDS counter each

@@ -709,7 +709,7 @@ __device__ __forceinline__ Tables tables_for(const SweepParams &p, int32_t b) {
 
 // Sum N per-thread values over the work-group and add them to the N consecutive 64-bit
 // counters at dst (zero sums are skipped).
-template <int N>
+template <int N, int W = kWaves>
 __device__ __forceinline__ void group_add(int64_t (&v)[N], int64_t (*red)[N], int lane, int wave,
                                           int tid, unsigned long long *dst) {
 #pragma unroll
@@ -722,7 +722,7 @@ __device__ __forceinline__ void group_add(int64_t (&v)[N], int64_t (*red)[N], in
   if (tid < N) {
     int64_t s = 0;
 #pragma unroll
-    for (int w = 0; w < kWaves; w++) s += red[w][tid];
+    for (int w = 0; w < W; w++) s += red[w][tid];
     if (s != 0) atomicAdd(dst + tid, (unsigned long long)s);
   }
 }
@@ -732,6 +732,7 @@ static_assert(offsetof(DevRecord, Vbed) == 6 * 8 && offsetof(DevRecord, n_final)
                   offsetof(DevRecord, chk_final) == 11 * 8 && offsetof(DevRecord, aVtc) == 17 * 8,
               "DevRecord totals must be consecutive 64-bit fields");
 
+template <int W = kWaves>
 __device__ __forceinline__ int32_t group_or(int32_t f, int32_t *red, int lane, int wave) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) f |= __shfl_xor(f, off);
@@ -739,7 +740,7 @@ __device__ __forceinline__ int32_t group_or(int32_t f, int32_t *red, int lane, i
   __syncthreads();
   int32_t r = 0;
 #pragma unroll
-  for (int w = 0; w < kWaves; w++) r |= red[w];
+  for (int w = 0; w < W; w++) r |= red[w];
   return r;
 }
 
@@ -1877,8 +1878,8 @@ __device__ __forceinline__ void bed_body(const SweepParams &p, int32_t blk) {
   __shared__ uint32_t srcw[kBedWords], rowany[kBedWords], convw[kBedWords];
   // dtab[dx][m]: the least bed_dist[dx][j] over the window columns j set in m (1e5 if none)
   __shared__ float dtab[2 * kBedTabR][1 << (2 * kBedTabR)];
-  __shared__ int64_t red[kWaves][6];
-  __shared__ int32_t fred[kWaves];
+  __shared__ int64_t red[kBedWaves][6];
+  __shared__ int32_t fred[kBedWaves];
   const int tid = threadIdx.x, lane = tid & 63, wave = wave_index();
   int32_t b, band;
   if (!xcd_map(blk, p.bed_bands, p.n_batch, b, band)) return;
@@ -1905,7 +1906,7 @@ __device__ __forceinline__ void bed_body(const SweepParams &p, int32_t blk) {
   const bool wtab = bits && br >= 1 && br <= kBedTabR;  // bit-window neighbour search
   if (wtab) {
     const int32_t wd = 2 * br, nm = 1 << wd;
-    for (int32_t i = tid; i < wd * nm; i += kThreads) {
+    for (int32_t i = tid; i < wd * nm; i += kBedThreads) {
       const int32_t dx = i / nm, m = i - dx * nm;
       float md = 1e5f;
       for (int32_t j = 0; j < wd; j++)
@@ -1916,7 +1917,7 @@ __device__ __forceinline__ void bed_body(const SweepParams &p, int32_t blk) {
   if (lds) {
     // One halo row per wave step, its loads (up to 8 column chunks of 64) issued together.
     constexpr int kChunks = 8;
-    for (int32_t r0 = wave; r0 < nr; r0 += kWaves) {
+    for (int32_t r0 = wave; r0 < nr; r0 += kBedWaves) {
       const int32_t x = hx0 + r0;
       const bool xin = x >= bx0 && x <= bx1;
       for (int32_t c00 = 0; c00 < hw; c00 += 64 * kChunks) {
@@ -1950,7 +1951,7 @@ __device__ __forceinline__ void bed_body(const SweepParams &p, int32_t blk) {
     __syncthreads();
   }
   if (bits) {
-    for (int32_t i = tid; i < nr * nw; i += kThreads) {
+    for (int32_t i = tid; i < nr * nw; i += kBedThreads) {
       const int32_t w0 = i % nw;
       const uint32_t cur = srcw[i];
       const uint32_t prev = w0 > 0 ? srcw[i - 1] : 0u;
@@ -1966,7 +1967,7 @@ __device__ __forceinline__ void bed_body(const SweepParams &p, int32_t blk) {
   int64_t nbed = 0, nfinal = 0;
   uint64_t chk = 0;
   int32_t flags = 0;
-  for (int32_t xr = wave; xr < X1 - X0; xr += kWaves)
+  for (int32_t xr = wave; xr < X1 - X0; xr += kBedWaves)
   for (int32_t c0 = 0; c0 < rwy; c0 += 64) {
     if (c0 + lane >= rwy) continue;
     const int32_t X = X0 + xr, Y = ry0 + c0 + lane;
@@ -2049,8 +2050,8 @@ __device__ __forceinline__ void bed_body(const SweepParams &p, int32_t blk) {
   DevRecord &R = p.rec[b];
   // Vbed, n_entries, n_slots, n_final, chk_pre, chk_final
   int64_t v[6] = {nbed, 0, 0, nfinal, 0, (int64_t)chk};
-  group_add<6>(v, red, lane, wave, tid, (unsigned long long *)&R.Vbed);
-  const int32_t f = group_or(flags, fred, lane, wave);
+  group_add<6, kBedWaves>(v, red, lane, wave, tid, (unsigned long long *)&R.Vbed);
+  const int32_t f = group_or<kBedWaves>(flags, fred, lane, wave);
   if (tid == 0 && f) atomicOr(&R.flags, f);
 }
 
@@ -2205,7 +2206,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
   pair_body<1, true>(p, (int32_t)blockIdx.x);
 }
 __global__ __launch_bounds__(kThreads) void pair_huge_kernel(SweepParams p) { pair_huge_body(p, (int32_t)blockIdx.x); }
-__global__ __launch_bounds__(kThreads) void bed_kernel(SweepParams p) { bed_body(p, (int32_t)blockIdx.x); }
+__global__ __launch_bounds__(kBedThreads) void bed_kernel(SweepParams p) { bed_body(p, (int32_t)blockIdx.x); }
 __global__ __launch_bounds__(64) void fsum_kernel(SweepParams p) { fsum_body(p, (int32_t)blockIdx.x); }
 
 // Persistent form: each work-group takes items from its XCD's queue (thread 0 counts them off
@@ -2268,7 +2269,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
 __global__ __launch_bounds__(kThreads) void pair_huge_queue(TOMO_QUEUE_ARGS) {
   drain(units, first, n_units, heads, [](const SweepParams &p, int32_t blk) { pair_huge_body(p, blk); });
 }
-__global__ __launch_bounds__(kThreads) void bed_queue(TOMO_QUEUE_ARGS) {
+__global__ __launch_bounds__(kBedThreads) void bed_queue(TOMO_QUEUE_ARGS) {
   drain(units, first, n_units, heads, [](const SweepParams &p, int32_t blk) { bed_body(p, blk); });
 }
 __global__ __launch_bounds__(64) void fsum_queue(TOMO_QUEUE_ARGS) {
@@ -2314,7 +2315,9 @@ static const void *queue_kernel(int k) {
   }
 }
 
-int queue_threads(int k) { return k == QK_RASTER ? kTileThreads : (k == QK_FSUM ? 64 : kThreads); }
+int queue_threads(int k) {
+  return k == QK_RASTER ? kTileThreads : (k == QK_FSUM ? 64 : (k == QK_BED ? kBedThreads : kThreads));
+}
 
 // Items of unit p for queue kernel k (xcd_map pads the orientation count to whole XCD rounds).
 int32_t queue_items(int k, const SweepParams &p) {
@@ -2370,7 +2373,7 @@ hipError_t launch_stage(int stage, const SweepParams &p, hipStream_t stream) {
       break;
     }
     case 4:
-      if (p.bed_on) hipLaunchKernelGGL(bed_kernel, dim3(nb8 * p.bed_bands), dim3(kThreads), 0, stream, p);
+      if (p.bed_on) hipLaunchKernelGGL(bed_kernel, dim3(nb8 * p.bed_bands), dim3(kBedThreads), 0, stream, p);
       break;
     case 5:
       if (p.exact_fsum) hipLaunchKernelGGL(fsum_kernel, dim3(nb), dim3(64), 0, stream, p);
