@@ -937,14 +937,16 @@ __device__ __forceinline__ void raster_body(const SweepParams &p, int32_t blk) {
       const int32_t ch = wbase + __builtin_amdgcn_readfirstlane((int)L.clist[idx]);
       const int32_t tr = ch * kChunk + lane;
       const bool active = tr < p.n_tri;
-      int32_t x0 = 0, x1 = -1, y0 = 0, y1 = -1;
-      bool owner = false;
-      TriScan sc = {};
-      float v0x = 0.0f, v0y = 0.0f;
-      uint32_t my_rank = 0u;
-      if (active) {
+      // Lanes past the last triangle (only in the last chunk) set up that chunk's last triangle
+      // again and walk nothing: no branch around the setup.
+      int32_t x0, x1, y0, y1;
+      bool owner;
+      TriScan sc;
+      float v0x, v0y;
+      uint32_t my_rank;
+      {
         float a[3][3], nzv[3];
-        const int4 ix = p.tidx[tr];
+        const int4 ix = p.tidx[active ? tr : p.n_tri - 1];
         my_rank = 2u * (uint32_t)ix.w;
         const int32_t ids[3] = {ix.x, ix.y, ix.z};
         // Rotate the three vertices (stored relative to the rotation centre, so shared by every
@@ -979,7 +981,7 @@ __device__ __forceinline__ void raster_body(const SweepParams &p, int32_t blk) {
           x0 = max(x0, lo); y0 = max(y0, lo); x1 = min(x1, hi); y1 = min(y1, hi);
         }
         const int32_t ox = min(max(x0, 0), G - 1), oy = min(max(y0, 0), G - 1);
-        owner = ox >= xa && ox < xb && oy >= ya && oy < yb;
+        owner = active && ox >= xa && ox < xb && oy >= ya && oy < yb;
         // The owner walks the whole box when it leaves the tile (hits decide the fallback);
         // every other walk is clipped to the pixels this tile tests.
         if (!(owner && (x0 < exlo || x1 > exhi || y0 < eylo || y1 > eyhi))) {
