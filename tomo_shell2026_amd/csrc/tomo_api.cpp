@@ -15,7 +15,8 @@
 #include "tomo_device.h"
 
 namespace tomo {
-hipError_t launch_stage(int stage, const SweepParams &p, hipStream_t stream);
+hipError_t launch_stage(int stage, const SweepParams &p, hipStream_t stream, hipStream_t side = nullptr,
+                        hipEvent_t fork = nullptr, hipEvent_t join = nullptr);
 hipError_t static_scratch_ptr(int device, uint8_t **out);
 int32_t queue_items(int k, const SweepParams &p);
 hipError_t queue_occupancy(int k, int *per_cu);
@@ -200,6 +201,8 @@ struct tomo_handle {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipStream_t side = nullptr;  // second stream: large-slot pairing beside the small-slot pairing
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool timed = false;
   Options opt{};
   // mesh
@@ -464,6 +467,11 @@ SweepParams base_params(tomo_handle *h) {
 // scr / hdr, when given, replace the handle's slot-table scratch and orientation headers.
 int enqueue(tomo_handle *h, int32_t begin, int32_t end, int32_t batch, int32_t *dbg4, size_t g2,
             uint64_t *dbg_rows = nullptr, uint8_t *scr = nullptr, OrientHdr *hdr = nullptr) {
+  // The side stream is left out while the stream is being captured into a graph.
+  hipStream_t side = h->shared ? nullptr : h->side;
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (side && (hipStreamIsCapturing(h->stream, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone))
+    side = nullptr;
   for (int32_t b0 = begin; b0 < end; b0 += batch) {
     SweepParams p = base_params(h);
     if (scr) {
@@ -488,7 +496,7 @@ int enqueue(tomo_handle *h, int32_t begin, int32_t end, int32_t batch, int32_t *
         h->stage_ev.push_back(e);
         h->stage_of.push_back(stage);
       }
-      HIP_TRY(launch_stage(stage, p, h->stream));
+      HIP_TRY(launch_stage(stage, p, h->stream, side, h->ev_fork, h->ev_join));
     }
   }
   if (h->phases) {
@@ -608,7 +616,10 @@ int tomo_create(const float *f32, const int32_t *i32, int32_t grid, tomo_handle 
   h->opt = o;
   if (hipGetDevice(&h->device) != hipSuccess ||
       hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
+      hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess ||
+      hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess) {
     tomo_destroy(h);
     return fail(TOMO_E_HIP, "stream/event creation failed");
   }
@@ -1021,6 +1032,12 @@ int tomo_destroy(tomo_handle *h) {
   if (h->d_bed_dist) (void)hipFree(h->d_bed_dist);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+  if (h->ev_join) (void)hipEventDestroy(h->ev_join);
+  if (h->side) {
+    (void)hipStreamSynchronize(h->side);
+    (void)hipStreamDestroy(h->side);
+  }
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return TOMO_OK;

@@ -2359,7 +2359,8 @@ hipError_t launch_clear_heads(uint32_t *heads, int32_t n, hipStream_t stream) {
 
 // Stage k of one sweep slice: 0 aabb, 1 raster, 2 compact, 3 pair (small, large and huge
 // slots), 4 bed (skipped without a bed), 5 ordered float totals (skipped unless kept).
-hipError_t launch_stage(int stage, const SweepParams &p, hipStream_t stream) {
+hipError_t launch_stage(int stage, const SweepParams &p, hipStream_t stream, hipStream_t side, hipEvent_t fork,
+                        hipEvent_t join) {
   const int32_t nb = p.n_batch;
   const int32_t nb8 = (nb + 7) / 8 * 8;  // xcd_map pads the orientation count to whole XCD rounds
   switch (stage) {
@@ -2368,9 +2369,24 @@ hipError_t launch_stage(int stage, const SweepParams &p, hipStream_t stream) {
     case 2: hipLaunchKernelGGL(compact_kernel, dim3(nb), dim3(kThreads), 0, stream, p); break;
     case 3: {
       const bool dbg = p.dbg_pre || p.dbg_final || p.dbg_rows;
-      hipLaunchKernelGGL(dbg ? pair_kernel_dbg : pair_kernel, dim3(nb8 * p.pair_tiles), dim3(kThreads), 0, stream, p);
+      // The small and the large slots are disjoint sets and both kernels add their totals
+      // atomically: with a side stream the large-slot pairing runs beside the small-slot one
+      // (its few, latency-bound work-groups fill the gaps), joined before pair_huge_kernel.
+      hipStream_t bs = stream;
+      if (side) {
+        hipError_t e = hipEventRecord(fork, stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(side, fork, 0);
+        if (e != hipSuccess) return e;
+        bs = side;
+      }
       hipLaunchKernelGGL(dbg ? pair_big_kernel_dbg : pair_big_kernel, dim3(nb8 * p.big_groups), dim3(kThreads), 0,
-                         stream, p);
+                         bs, p);
+      hipLaunchKernelGGL(dbg ? pair_kernel_dbg : pair_kernel, dim3(nb8 * p.pair_tiles), dim3(kThreads), 0, stream, p);
+      if (side) {
+        hipError_t e = hipEventRecord(join, side);
+        if (e == hipSuccess) e = hipStreamWaitEvent(stream, join, 0);
+        if (e != hipSuccess) return e;
+      }
       hipLaunchKernelGGL(pair_huge_kernel, dim3(nb), dim3(kThreads), 0, stream, p);
       break;
     }
