@@ -175,7 +175,7 @@ __device__ __forceinline__ int32_t to_i16(float f, int32_t &flags) {
 __device__ __forceinline__ void place_entry(const Tables &t, int32_t G2, uint32_t *pool_top, int32_t s,
                                             uint32_t k, uint32_t key, uint32_t rank) {
   if (k < (uint32_t)t.inl) {
-    t.ent[(int64_t)k * G2 + s] = make_uint2(key, rank);
+    t.ent[k * (uint32_t)G2 + (uint32_t)s] = make_uint2(key, rank);  // < 32 * 4096^2 + 4096^2
   } else {
     const uint32_t j = atomicAdd(pool_top, 1u);
     if (j < (uint32_t)t.pool) {
@@ -200,18 +200,16 @@ struct Emit {
   int32_t flags;
   uint64_t zabs;  // sum of |z| of this lane's entries
 
-  // coord2vID (STomoVoxelSpaceInfo.cpp:97-105) with the footprint check.
+  // coord2vID (STomoVoxelSpaceInfo.cpp:97-105) with the footprint check: true when (px, py) lies
+  // on the grid.  The touched box grows only for such points (the selects keep it otherwise).
   __device__ __forceinline__ bool slot_xy(float px, float py, int32_t &sx, int32_t &sy) {
     sx = (int32_t)px;
     sy = (int32_t)py;
-    if (!(px > -2147483648.0f && px < 2147483648.0f && py > -2147483648.0f && py < 2147483648.0f) ||
-        sx < 0 || sy < 0 || sx >= G || sy >= G) {
-      flags |= F_FOOTPRINT_OOB;
-      return false;
-    }
-    bx0 = min(bx0, sx); bx1 = max(bx1, sx);
-    by0 = min(by0, sy); by1 = max(by1, sy);
-    return true;
+    const bool ok = __builtin_fabsf(px) < 2147483648.0f && __builtin_fabsf(py) < 2147483648.0f &&
+                    (uint32_t)sx < (uint32_t)G && (uint32_t)sy < (uint32_t)G;
+    bx0 = min(bx0, ok ? sx : G); bx1 = max(bx1, ok ? sx : -1);
+    by0 = min(by0, ok ? sy : G); by1 = max(by1, ok ? sy : -1);
+    return ok;
   }
 
   __device__ __forceinline__ void spill(int32_t s, uint32_t key, uint32_t rank) {
@@ -237,40 +235,47 @@ struct Emit {
     return sx >= xa && sx < xb && sy >= ya && sy < yb;
   }
 
-  __device__ __forceinline__ int32_t put(float px, float py, float pz, float nz, uint32_t rank) {
-    int32_t sx, sy;
-    if (!slot_xy(px, py, sx, sy)) return -1;
-    const int32_t z16 = to_i16(pz, flags);
-    const int32_t nz16 = to_i16(nz * kNormFactor, flags);
-    zabs += (uint64_t)(z16 < 0 ? -z16 : z16);
+  // An entry for the point (px, py, pz) with normal z nz.  Points off the grid only raise the
+  // footprint flag; the arithmetic runs for them too (its flags and sums are dropped), so that
+  // no lane takes a path of its own before the append.
+  __device__ __forceinline__ void put(float px, float py, float pz, float nz, uint32_t rank) {
+    int32_t sx, sy, f = 0;
+    const bool ok = slot_xy(px, py, sx, sy);
+    const int32_t z16 = to_i16(pz, f);
+    const int32_t nz16 = to_i16(nz * kNormFactor, f);
+    flags |= ok ? f : F_FOOTPRINT_OOB;
+    zabs += ok ? (uint32_t)(z16 < 0 ? -z16 : z16) : 0u;
     const uint32_t key = make_key(z16, nz16);
     const int32_t s = sx * G + sy;
-    if (in_tile(sx, sy))
-      place_entry(t, G2, pool_top, s, reserve(sx, sy, 1u), key, rank);
-    else
-      spill(s, key, rank);
-    return s;
+    if (ok) {
+      if (in_tile(sx, sy))
+        place_entry(t, G2, pool_top, s, reserve(sx, sy, 1u), key, rank);
+      else
+        spill(s, key, rank);
+    }
   }
 
   // A shell pixel and its sub-pixel (same slot, ranks r and r + 1) with one counter update.
-  __device__ __forceinline__ int32_t put2(float px, float py, float pz, float nz, float pz2, float nz2,
-                                          uint32_t rank) {
-    int32_t sx, sy;
-    if (!slot_xy(px, py, sx, sy)) return -1;
-    const int32_t z1 = to_i16(pz, flags), n1 = to_i16(nz * kNormFactor, flags);
-    const int32_t z2 = to_i16(pz2, flags), n2 = to_i16(nz2 * kNormFactor, flags);
-    zabs += (uint64_t)(z1 < 0 ? -z1 : z1) + (uint64_t)(z2 < 0 ? -z2 : z2);
+  __device__ __forceinline__ void put2(float px, float py, float pz, float nz, float pz2, float nz2,
+                                       uint32_t rank) {
+    int32_t sx, sy, f = 0;
+    const bool ok = slot_xy(px, py, sx, sy);
+    const int32_t z1 = to_i16(pz, f), n1 = to_i16(nz * kNormFactor, f);
+    const int32_t z2 = to_i16(pz2, f), n2 = to_i16(nz2 * kNormFactor, f);
+    flags |= ok ? f : F_FOOTPRINT_OOB;
+    zabs += ok ? (uint32_t)(z1 < 0 ? -z1 : z1) + (uint32_t)(z2 < 0 ? -z2 : z2) : 0u;
     const int32_t s = sx * G + sy;
     const uint32_t k1 = make_key(z1, n1), k2 = make_key(z2, n2);
-    if (in_tile(sx, sy)) {
-      const uint32_t k = reserve(sx, sy, 2u);
-      place_entry(t, G2, pool_top, s, k, k1, rank);
-      place_entry(t, G2, pool_top, s, k + 1u, k2, rank + 1u);
-    } else {
-      spill(s, k1, rank);
-      spill(s, k2, rank + 1u);
+    if (ok) {
+      if (in_tile(sx, sy)) {
+        const uint32_t k = reserve(sx, sy, 2u);
+        place_entry(t, G2, pool_top, s, k, k1, rank);
+        place_entry(t, G2, pool_top, s, k + 1u, k2, rank + 1u);
+      } else {
+        spill(s, k1, rank);
+        spill(s, k2, rank + 1u);
+      }
     }
-    return s;
   }
 };
 
@@ -1037,10 +1042,18 @@ __device__ __forceinline__ void raster_body(const SweepParams &p, int32_t blk) {
             m[1][q] = rec[TR_A1 + q] + kEps;
             m[2][q] = rec[TR_A2 + q] + kEps;
           }
-          const TriSetup ts = tri_setup(m[0][0], m[0][1], m[1][0], m[1][1], m[2][0], m[2][1]);
-          float nv, nw, u, v, w;
-          bary_num(ts, ((float)px_i + kEps) + p.h2, ((float)py_i + kEps) + p.h2, nv, nw);
-          if (bary_exact(ts.den, nv, nw, u, v, w)) {
+          // tri_setup and bary_num without their "+ 0" z terms: those only turn a -0 dot product
+          // into +0, and the sign of a zero there reaches no decision and no stored value (the
+          // quotients' zeros enter sums, comparisons and truncations to integers only).
+          const float e0x = m[1][0] - m[0][0], e0y = m[1][1] - m[0][1];
+          const float e1x = m[2][0] - m[0][0], e1y = m[2][1] - m[0][1];
+          const float d00 = e0x * e0x + e0y * e0y, d01 = e0x * e1x + e0y * e1y, d11 = e1x * e1x + e1y * e1y;
+          const float den = d00 * d11 - d01 * d01;
+          const float e2x = (((float)px_i + kEps) + p.h2) - m[0][0], e2y = (((float)py_i + kEps) + p.h2) - m[0][1];
+          const float d20 = e2x * e0x + e2y * e0y, d21 = e2x * e1x + e2y * e1y;
+          const float nv = d11 * d20 - d01 * d21, nw = d00 * d21 - d01 * d20;
+          float u, v, w;
+          if (bary_exact(den, nv, nw, u, v, w)) {
             L.hits[wave][owner_lane] = 1;
             if (px_i >= exlo && px_i <= exhi && py_i >= eylo && py_i <= eyhi) {
               // _bary_product (Tomo_types.cpp:75-84) of the margined vertices and the normals
