@@ -992,6 +992,9 @@ __device__ __forceinline__ void raster_body(const SweepParams &p, int32_t blk) {
         if (!(owner && (x0 < exlo || x1 > exhi || y0 < eylo || y1 > eyhi))) {
           x0 = max(x0, exlo); x1 = min(x1, exhi); y0 = max(y0, eylo); y1 = min(y1, eyhi);
         }
+        // The chunk's culling box is padded: when no triangle of it reaches the tile (no box
+        // left and no owner), the rest of the set-up and the walk have nothing to do.
+        if (!ballot(active && (owner || (x1 >= x0 && y1 >= y0)))) continue;
         const TriSetup ts = tri_setup(v0x, v0y, v1x, v1y, v2x, v2y);
         if (tri_degenerate(ts)) { x1 = x0 - 1; }  // no pixel can pass
         const float ex = fmaxf(fabsf(centre_offset(x0, p.h2, v0x)), fabsf(centre_offset(x1, p.h2, v0x)));
