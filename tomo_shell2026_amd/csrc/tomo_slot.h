@@ -76,27 +76,21 @@ enum : uint32_t { BI_HAS10 = 1u << 7, BI_SRC = 1u << 8, BI_POSS = 1u << 9, BI_CO
 //             beta/SSA bottom;
 //  conv     : a raft source that will take a bed row on its own turn and so stops being a
 //             source for every slot scanned after it.
+// Written with bitwise operators and no branches (slot pairing calls it once per slot, lane by
+// lane, and a chain of ifs there compiles to a branch tree); the truth table is the one above.
 TOMO_HD uint32_t bed_info(const SlotResult &r, const PairParams &pp) {
-  uint32_t b = (uint32_t)r.n;
-  if (r.has10) b |= BI_HAS10;
+  const int32_t lt = r.last_type;
   const int32_t solid = T_AL | T_BE | T_SSA | T_SS;
-  bool src = r.n > 0 && r.last_z == 0 && (r.last_type & solid);
-  bool poss;
-  if (r.n == 0) {
-    poss = true;
-  } else if (r.last_z == 0) {
-    poss = (r.last_type == T_NVA || r.last_type == T_VO || r.last_type == T_VSS);
-    if (r.last_type & solid) poss = false;
-    if (pp.bed_type == 2 && (r.last_type & (T_BE | T_SSA))) poss = true;
-  } else {
-    poss = false;
-  }
-  bool conv = src && pp.bed_type == 2 && (r.last_type & (T_BE | T_SSA)) && pp.bed_r >= 1 &&
-              r.n < kMaxPix && !r.has10;
-  if (src) b |= BI_SRC;
-  if (poss) b |= BI_POSS;
-  if (conv) b |= BI_CONV;
-  return b;
+  const bool n0 = r.n == 0, lz = r.last_z == 0;
+  const bool lsolid = (lt & solid) != 0, lbs = (lt & (T_BE | T_SSA)) != 0, bt2 = pp.bed_type == 2;
+  const bool open = (lt == T_NVA) | (lt == T_VO) | (lt == T_VSS);
+  const bool src = !n0 & lz & lsolid;
+  // possible: an empty slot; a bottom row at z 0 that is open (NVA/Vo/Vss, no solid bit) or, on
+  // a raft, a beta/SSA row; nothing else
+  const bool poss = n0 | (lz & ((open & !lsolid) | (bt2 & lbs)));
+  const bool conv = src & bt2 & lbs & (pp.bed_r >= 1) & (r.n < kMaxPix) & !r.has10;
+  return (uint32_t)r.n | (r.has10 ? BI_HAS10 : 0u) | (src ? BI_SRC : 0u) | (poss ? BI_POSS : 0u) |
+         (conv ? BI_CONV : 0u);
 }
 
 // A final slot row for the render export: (type << 32) | key.
