@@ -1321,8 +1321,7 @@ __device__ __forceinline__ void compact_body(const SweepParams &p, int32_t blk) 
 template <int PART, bool DBG>
 __device__ __forceinline__ void pair_body(const SweepParams &p, int32_t blk) {
   __shared__ uint64_t sortbuf[kWaves][kBigCap];
-  __shared__ uint32_t pairq[3][kWaves][128];  // queued slots: s | entries << 24
-  __shared__ uint32_t pairk[2][kWaves][128];  // keys of the queued slots of at most 2 entries
+  __shared__ uint32_t pairq[2][kWaves][128];  // queued slots of 3 .. kSmall entries: s | entries << 24
   __shared__ int64_t red[kWaves][18];
   __shared__ int32_t fred[kWaves];
   const int tid = threadIdx.x, lane = tid & 63, wave = wave_index();
@@ -1348,9 +1347,10 @@ __device__ __forceinline__ void pair_body(const SweepParams &p, int32_t blk) {
   const bool final_here = !p.bed_on;  // without a bed the final row count is known here
   const bool track = totals_may_round(h, p.pp);
   int32_t *const sv = (p.exact_fsum && track) ? t.sv : nullptr;
-  // pair_kernel: a lane pairs at most one slot per batch of its wave's span (at most 2,048
-  // slots, so at most 2^6 slots per lane in 3 classes); pair_big_kernel's lanes can account for
-  // a whole wave's large slots and keep 64-bit totals.
+  // pair_kernel: a wave's span holds at most 2,048 slots (32 steps); a lane pairs at most one
+  // slot of at most 2 entries per step and one slot per batch of the two queued classes, so at
+  // most 2^7 slots of |value| < 2^22: 32-bit totals do not overflow.  pair_big_kernel's lanes
+  // can account for a whole wave's large slots and keep 64-bit totals.
   AccT<typename std::conditional<PART == 0, int32_t, int64_t>::type> acc = {};
   // The debug dumps (rows, final row counts, per-slot sums) exist only in the DBG instances,
   // which the host launches when a dump was asked for: the others keep no registers for them.
@@ -1381,17 +1381,16 @@ __device__ __forceinline__ void pair_body(const SweepParams &p, int32_t blk) {
       if (dbg_final) dbg_final[s] = r.n;
     }
   };
-  // The touched box is walked 64 slots per wave step; slots of at most kSmall entries are
-  // queued by size class (<= 2, <= 4, <= 8) and paired 64 at a time with that class's register
-  // network, so no wave runs a larger network than its slots need.  Each step first issues the
-  // loads of the next step's counters and of its own slots' keys when they have at most 2
-  // entries (most slots), then pairs the batches earlier steps filled while those loads are in
-  // flight, and only then queues its slots with their keys.  One extra step with no slots
-  // pairs what remains.
+  // The touched box is walked 64 slots per wave step.  A step issues the loads of the next
+  // step's counters and of its own slots' keys when they have at most 2 entries (most slots),
+  // then pairs the previous step's slots of at most 2 entries (their keys have arrived) and
+  // writes their bed bookkeeping, and the empty slots', together: one store per step covers a
+  // run of consecutive slots.  Slots of 3 .. kSmall entries are queued by size class (<= 4,
+  // <= 8) and paired 64 at a time with that class's register network, so no wave runs a larger
+  // network than its slots need.  One extra step with no slots pairs what remains.
   if constexpr (PART == 0) {
-  uint32_t n2 = 0u, n4 = 0u, n8 = 0u;  // queue fills (wave-uniform)
-  uint32_t *const q2 = pairq[0][wave], *const q4 = pairq[1][wave], *const q8 = pairq[2][wave];
-  uint32_t *const qa = pairk[0][wave], *const qb = pairk[1][wave];
+  uint32_t n4 = 0u, n8 = 0u;  // queue fills (wave-uniform)
+  uint32_t *const q4 = pairq[0][wave], *const q8 = pairq[1][wave];
   const int32_t i1 = min(i0 + p.pair_span, nbb);
   if (i0 + wave * 64 < i1) {
     // slot i of the box is (bx0 + i / wy, by0 + i % wy); step kThreads slots at a time
@@ -1399,6 +1398,9 @@ __device__ __forceinline__ void pair_body(const SweepParams &p, int32_t blk) {
     int32_t sx = bx0 + first / wy, sy = by0 + first % wy;
     const int32_t dq = kThreads / wy, dr = kThreads % wy;
     uint32_t c_next = first < i1 ? t.count[sx * G + sy] : 0u;
+    // the previous step's slot of this lane when it has at most 2 entries (pc = 3: none)
+    int32_t ps = 0;
+    uint32_t pc = 3u, pka = 0u, pkb = 0u;
     for (int32_t base = i0 + wave * 64;; base += kThreads) {
       const bool tail = base >= i1;  // the extra step
       const int32_t i = base + lane;
@@ -1416,82 +1418,65 @@ __device__ __forceinline__ void pair_body(const SweepParams &p, int32_t blk) {
         ka = t.ent[s].x;
         if (c == 2u) kb = t.ent[(int64_t)G2 + s].x;
       }
+      if (pc == 0u) {
+        t.bedinfo[ps] = (uint16_t)BI_POSS;
+        if (sv) {
+          const SlotResult z = {};
+          put_sv(sv, G2, ps, z);
+        }
+      } else if (pc <= 2u) {
+        pair_small(std::integral_constant<int, 2>{}, ps, pc, pka, pkb);
+      }
       // Pair full batches (in the extra step, everything queued), one class at a time: a single
       // call site per network keeps the code (and the register demand) small.
       for (;;) {
         const uint32_t lim = tail ? 1u : 64u;
-        const int fc = n2 >= lim ? 0 : (n4 >= lim ? 1 : (n8 >= lim ? 2 : -1));
+        const int fc = n4 >= lim ? 0 : (n8 >= lim ? 1 : -1);
         if (fc < 0) break;
-        uint32_t *const q = fc == 0 ? q2 : (fc == 1 ? q4 : q8);
-        const uint32_t n = fc == 0 ? n2 : (fc == 1 ? n4 : n8);
+        uint32_t *const q = fc == 0 ? q4 : q8;
+        const uint32_t n = fc == 0 ? n4 : n8;
         const uint32_t np = n < 64u ? n : 64u;
         wave_sync();
         if ((uint32_t)lane < np) {
           const uint32_t qe = q[lane];
           const int32_t qs = (int32_t)(qe & 0xffffffu);
           const uint32_t qc = qe >> 24;
-          if (fc == 0) pair_small(std::integral_constant<int, 2>{}, qs, qc, qa[lane], qb[lane]);
-          else if (fc == 1) pair_small(std::integral_constant<int, 4>{}, qs, qc, 0u, 0u);
+          if (fc == 0) pair_small(std::integral_constant<int, 4>{}, qs, qc, 0u, 0u);
           else pair_small(std::integral_constant<int, kSmall>{}, qs, qc, 0u, 0u);
         }
         wave_sync();
         if (n > 64u) {
           const uint32_t rest = n - 64u;
-          uint32_t mm = 0u, ma = 0u, mb = 0u;
-          if ((uint32_t)lane < rest) {
-            mm = q[64 + lane];
-            if (fc == 0) {
-              ma = qa[64 + lane];
-              mb = qb[64 + lane];
-            }
-          }
+          uint32_t mm = 0u;
+          if ((uint32_t)lane < rest) mm = q[64 + lane];
           wave_sync();
-          if ((uint32_t)lane < rest) {
-            q[lane] = mm;
-            if (fc == 0) {
-              qa[lane] = ma;
-              qb[lane] = mb;
-            }
-          }
+          if ((uint32_t)lane < rest) q[lane] = mm;
           wave_sync();
         }
-        if (fc == 0) n2 = n - np;
-        else if (fc == 1) n4 = n - np;
+        if (fc == 0) n4 = n - np;
         else n8 = n - np;
       }
       if (tail) break;
+      // this step's slots: at most 2 entries wait for the next step, larger ones are queued
+      // (more than kSmall: paired from the large-slot list)
+      ps = s;
+      pc = i < i1 && c <= 2u ? c : 3u;
+      pka = ka;
+      pkb = kb;
       int cls = -1;
       uint32_t e = 0u;
-      if (i < i1) {
-        if (c == 0u) {
-          t.bedinfo[s] = (uint16_t)BI_POSS;
-          if (sv) {
-            const SlotResult z = {};
-            put_sv(sv, G2, s, z);
-          }
-        } else if (c <= (uint32_t)kSmall) {  // larger ones are paired below from the big list
-          cls = c <= 2u ? 0 : (c <= 4u ? 1 : 2);
-          e = (uint32_t)s | (c << 24);
-        }
+      if (i < i1 && c > 2u && c <= (uint32_t)kSmall) {
+        cls = c <= 4u ? 0 : 1;
+        e = (uint32_t)s | (c << 24);
       }
       {
         const uint64_t m = ballot(cls == 0);
-        if (cls == 0) {
-          const uint32_t at = n2 + lanes_below(m);
-          q2[at] = e;
-          qa[at] = ka;
-          qb[at] = kb;
-        }
-        n2 += (uint32_t)__popcll(m);
-      }
-      {
-        const uint64_t m = ballot(cls == 1);
-        if (cls == 1) q4[n4 + lanes_below(m)] = e;
+        if (cls == 0) q4[n4 + lanes_below(m)] = e;
         n4 += (uint32_t)__popcll(m);
       }
       {
-        const uint64_t m = ballot(cls == 2);
-        if (cls == 2) q8[n8 + lanes_below(m)] = e;
+        const uint64_t m = ballot(cls == 1);
+        if (cls == 1) q8[n8 + lanes_below(m)] = e;
         n8 += (uint32_t)__popcll(m);
       }
     }
