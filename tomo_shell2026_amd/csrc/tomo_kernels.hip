@@ -650,15 +650,21 @@ __device__ bool big_slot(const SweepParams &p, int32_t b, int32_t s, uint32_t c,
 // A slot with kSmall < c < W entries in a W-lane segment (W = 64: the whole wave; W = 32: two
 // slots per wave): one key per lane, register bitonic sort over the segment with lane
 // shuffles, duplicates masked out.  No ranks are needed: at most 63 distinct keys.
+// This lane's key of such a slot (0 past its count).
 template <int W = 64>
-__device__ void mid_slot(int32_t s, uint32_t c, const Tables &t, int32_t G2, int lane,
-                         const PairParams &pp, SlotResult &r, uint64_t *rows) {
+__device__ __forceinline__ uint32_t mid_key(int32_t s, uint32_t c, const Tables &t, int32_t G2, int lane) {
   const int i = lane & (W - 1);
   uint32_t key = 0u;
   if (i < (int)c) {
     const uint32_t off = (c > (uint32_t)t.inl) ? t.off[s] : 0u;
     key = (i < t.inl) ? t.ent[(int64_t)i * G2 + s].x : t.cmp_key[off + (uint32_t)(i - t.inl)];
   }
+  return key;
+}
+
+template <int W = 64>
+__device__ void mid_sort(uint32_t key, uint32_t c, int lane, const PairParams &pp, SlotResult &r, uint64_t *rows) {
+  const int i = lane & (W - 1);
 #pragma unroll
   for (int size = 2; size <= W; size <<= 1) {
 #pragma unroll
@@ -673,6 +679,12 @@ __device__ void mid_slot(int32_t s, uint32_t c, const Tables &t, int32_t G2, int
   const uint32_t prev = __shfl_up(key, 1);
   const bool first = i < (int)c && (i == 0 || key != prev);
   wave_pair<W>(key, ballot(first), lane, pp, r, rows);
+}
+
+template <int W = 64>
+__device__ __forceinline__ void mid_slot(int32_t s, uint32_t c, const Tables &t, int32_t G2, int lane,
+                                         const PairParams &pp, SlotResult &r, uint64_t *rows) {
+  mid_sort<W>(mid_key<W>(s, c, t, G2, lane), c, lane, pp, r, rows);
 }
 
 // Work-groups are dispatched to the 8 XCDs round-robin by block index.  Map block i to
@@ -1535,7 +1547,9 @@ __device__ __forceinline__ void pair_body(const SweepParams &p, int32_t blk) {
     const bool half = lc > lane_cap && lc <= 32u && fits;
     uint64_t m32 = ballot(half);
     uint64_t m64 = ballot(lc > lane_cap && !half);
-    while (m32) {
+    // Two slots per step, one per 32-lane half; the next step's keys are loaded before this
+    // step sorts, so their latency overlaps the shuffles.
+    auto take2 = [&](int32_t &s, uint32_t &c) {
       const int la = __ffsll((unsigned long long)m32) - 1;
       m32 &= m32 - 1ull;
       int lb = -1;
@@ -1545,12 +1559,32 @@ __device__ __forceinline__ void pair_body(const SweepParams &p, int32_t blk) {
       }
       const bool upper = lane >= 32;
       const int src = (upper && lb >= 0) ? lb : la;
-      const int32_t s = __shfl(ls, src);
+      s = __shfl(ls, src);
       const uint32_t cs = __shfl(lc, src);
-      const uint32_t c = (upper && lb < 0) ? 0u : cs;  // an idle upper half pairs nothing
+      c = (upper && lb < 0) ? 0u : cs;  // an idle upper half pairs nothing
+    };
+    int32_t s = 0;
+    uint32_t c = 0u, key = 0u;
+    bool have = m32 != 0ull;
+    if (have) {
+      take2(s, c);
+      key = mid_key<32>(s, c, t, G2, lane);
+    }
+    while (have) {
+      int32_t s2 = 0;
+      uint32_t c2 = 0u, key2 = 0u;
+      const bool have2 = m32 != 0ull;
+      if (have2) {
+        take2(s2, c2);
+        key2 = mid_key<32>(s2, c2, t, G2, lane);
+      }
       SlotResult r;
-      mid_slot<32>(s, c, t, G2, lane, p.pp, r, (dbg_rows && c) ? dbg_rows + (int64_t)s * 64 : nullptr);
+      mid_sort<32>(key, c, lane, p.pp, r, (dbg_rows && c) ? dbg_rows + (int64_t)s * 64 : nullptr);
       if ((lane & 31) == 0 && c) finish_slot(s, r);
+      s = s2;
+      c = c2;
+      key = key2;
+      have = have2;
     }
     while (m64) {
       const int la = __ffsll((unsigned long long)m64) - 1;
