@@ -1396,8 +1396,8 @@ __device__ __forceinline__ void pair_body(const SweepParams &p, int32_t blk) {
   // The touched box is walked 64 slots per wave step.  A step issues the loads of the next
   // step's counters and of its own slots' keys when they have at most 2 entries (most slots),
   // then pairs the previous step's slots of at most 2 entries (their keys have arrived) and
-  // writes their bed bookkeeping, and the empty slots', together: one store per step covers a
-  // run of consecutive slots.  Slots of 3 .. kSmall entries are queued by size class (<= 4,
+  // writes their bed bookkeeping, and the empty slots', together: the step's stores cover a run
+  // of consecutive slots.  Slots of 3 .. kSmall entries are queued by size class (<= 4,
   // <= 8) and paired 64 at a time with that class's register network, so no wave runs a larger
   // network than its slots need.  One extra step with no slots pairs what remains.
   if constexpr (PART == 0) {
