@@ -203,6 +203,7 @@ struct tomo_handle {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipStream_t side = nullptr;  // second stream: large-slot pairing beside the small-slot pairing
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_pair = nullptr, ev_bed = nullptr;  // a batch's pairing done; its bed and sums done
   bool timed = false;
   Options opt{};
   // mesh
@@ -329,13 +330,14 @@ int ensure_scratch(tomo_handle *h, int32_t batch) {
     h->d_scratch = nullptr;
     return fail(TOMO_E_OOM, "scratch allocation failed");
   }
-  if (hipMalloc(&h->d_hdr, sizeof(OrientHdr) * (size_t)batch) != hipSuccess) {
+  // two header blocks: consecutive batches alternate between them (enqueue)
+  if (hipMalloc(&h->d_hdr, sizeof(OrientHdr) * 2 * (size_t)batch) != hipSuccess) {
     h->d_hdr = nullptr;
     return fail(TOMO_E_OOM, "header allocation failed");
   }
   // Defined contents from the start (each sweep writes every counter and header it reads).
   HIP_TRY(hipMemsetAsync(h->d_scratch, 0, bytes, h->stream));
-  HIP_TRY(hipMemsetAsync(h->d_hdr, 0, sizeof(OrientHdr) * (size_t)batch, h->stream));
+  HIP_TRY(hipMemsetAsync(h->d_hdr, 0, sizeof(OrientHdr) * 2 * (size_t)batch, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
   h->scratch_batch = batch;
   h->scratch_region = h->lay.bytes;
@@ -472,11 +474,19 @@ int enqueue(tomo_handle *h, int32_t begin, int32_t end, int32_t batch, int32_t *
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
   if (side && (hipStreamIsCapturing(h->stream, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone))
     side = nullptr;
-  for (int32_t b0 = begin; b0 < end; b0 += batch) {
+  // With the side stream, a batch's bed and ordered sums run on it after the batch's pairing,
+  // beside the next batch's placement, rasteriser and compaction (which touch neither the bed
+  // bookkeeping nor the per-slot sums, and whose headers are the other of two blocks); the next
+  // batch's pairing waits for them.  Not for debug dumps, stage timing or the static array.
+  const bool overlap = side && !scr && !dbg4 && !dbg_rows && !h->phases;
+  int32_t k = 0;
+  for (int32_t b0 = begin; b0 < end; b0 += batch, k++) {
     SweepParams p = base_params(h);
     if (scr) {
       p.scratch = scr;
       p.hdr = hdr;
+    } else if (overlap) {
+      p.hdr = h->d_hdr + (size_t)(k & 1) * (size_t)batch;
     }
     p.n_batch = (end - b0) < batch ? (end - b0) : batch;
     p.rot = h->d_rot + (size_t)b0 * 9;
@@ -488,6 +498,17 @@ int enqueue(tomo_handle *h, int32_t begin, int32_t end, int32_t batch, int32_t *
       p.dbg_vss = dbg4 + 3 * g2;
     }
     p.dbg_rows = dbg_rows;
+    if (overlap) {
+      for (int stage = 0; stage < 3; stage++) HIP_TRY(launch_stage(stage, p, h->stream));
+      if (k > 0) HIP_TRY(hipStreamWaitEvent(h->stream, h->ev_bed, 0));
+      HIP_TRY(launch_stage(3, p, h->stream, side, h->ev_fork, h->ev_join));
+      HIP_TRY(hipEventRecord(h->ev_pair, h->stream));
+      HIP_TRY(hipStreamWaitEvent(side, h->ev_pair, 0));
+      HIP_TRY(launch_stage(4, p, side));
+      HIP_TRY(launch_stage(5, p, side));
+      HIP_TRY(hipEventRecord(h->ev_bed, side));
+      continue;
+    }
     for (int stage = 0; stage < 6; stage++) {
       if (h->phases) {
         hipEvent_t e;
@@ -499,6 +520,7 @@ int enqueue(tomo_handle *h, int32_t begin, int32_t end, int32_t batch, int32_t *
       HIP_TRY(launch_stage(stage, p, h->stream, side, h->ev_fork, h->ev_join));
     }
   }
+  if (overlap && k > 0) HIP_TRY(hipStreamWaitEvent(h->stream, h->ev_bed, 0));
   if (h->phases) {
     hipEvent_t e;
     HIP_TRY(hipEventCreate(&e));
@@ -619,7 +641,9 @@ int tomo_create(const float *f32, const int32_t *i32, int32_t grid, tomo_handle 
       hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess ||
       hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_pair, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_bed, hipEventDisableTiming) != hipSuccess) {
     tomo_destroy(h);
     return fail(TOMO_E_HIP, "stream/event creation failed");
   }
@@ -1034,6 +1058,8 @@ int tomo_destroy(tomo_handle *h) {
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
   if (h->ev_join) (void)hipEventDestroy(h->ev_join);
+  if (h->ev_pair) (void)hipEventDestroy(h->ev_pair);
+  if (h->ev_bed) (void)hipEventDestroy(h->ev_bed);
   if (h->side) {
     (void)hipStreamSynchronize(h->side);
     (void)hipStreamDestroy(h->side);
