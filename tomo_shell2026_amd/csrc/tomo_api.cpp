@@ -204,6 +204,7 @@ struct tomo_handle {
   hipStream_t side = nullptr;  // second stream: large-slot pairing beside the small-slot pairing
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipEvent_t ev_pair = nullptr, ev_bed = nullptr;  // a batch's pairing done; its bed and sums done
+  hipEvent_t ev_aabb = nullptr;                     // the next batch's placement done
   bool timed = false;
   Options opt{};
   // mesh
@@ -499,8 +500,26 @@ int enqueue(tomo_handle *h, int32_t begin, int32_t end, int32_t batch, int32_t *
     }
     p.dbg_rows = dbg_rows;
     if (overlap) {
-      for (int stage = 0; stage < 3; stage++) HIP_TRY(launch_stage(stage, p, h->stream));
+      // placement: the first batch's here; each later one's ran on the side stream during the
+      // previous batch's pairing
+      if (k == 0) HIP_TRY(launch_stage(0, p, h->stream));
+      else HIP_TRY(hipStreamWaitEvent(h->stream, h->ev_aabb, 0));
+      for (int stage = 1; stage < 3; stage++) HIP_TRY(launch_stage(stage, p, h->stream));
       if (k > 0) HIP_TRY(hipStreamWaitEvent(h->stream, h->ev_bed, 0));
+      if (b0 + batch < end) {
+        // the next batch's placement beside this batch's pairing, before the small-slot pairing
+        // on the side stream (the large-slot kernel on the main stream is the longer of the two);
+        // its headers are the block this batch's predecessor used, whose bed is done
+        SweepParams pn = base_params(h);
+        pn.hdr = h->d_hdr + (size_t)((k + 1) & 1) * (size_t)batch;
+        pn.n_batch = (end - b0 - batch) < batch ? (end - b0 - batch) : batch;
+        pn.rot = h->d_rot + (size_t)(b0 + batch) * 9;
+        pn.rec = h->d_rec + b0 + batch;
+        HIP_TRY(hipEventRecord(h->ev_pair, h->stream));
+        HIP_TRY(hipStreamWaitEvent(side, h->ev_pair, 0));
+        HIP_TRY(launch_stage(0, pn, side));
+        HIP_TRY(hipEventRecord(h->ev_aabb, side));
+      }
       HIP_TRY(launch_stage(3, p, h->stream, side, h->ev_fork, h->ev_join));
       HIP_TRY(hipEventRecord(h->ev_pair, h->stream));
       HIP_TRY(hipStreamWaitEvent(side, h->ev_pair, 0));
@@ -643,7 +662,8 @@ int tomo_create(const float *f32, const int32_t *i32, int32_t grid, tomo_handle 
       hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&h->ev_pair, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&h->ev_bed, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&h->ev_bed, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_aabb, hipEventDisableTiming) != hipSuccess) {
     tomo_destroy(h);
     return fail(TOMO_E_HIP, "stream/event creation failed");
   }
@@ -1060,6 +1080,7 @@ int tomo_destroy(tomo_handle *h) {
   if (h->ev_join) (void)hipEventDestroy(h->ev_join);
   if (h->ev_pair) (void)hipEventDestroy(h->ev_pair);
   if (h->ev_bed) (void)hipEventDestroy(h->ev_bed);
+  if (h->ev_aabb) (void)hipEventDestroy(h->ev_aabb);
   if (h->side) {
     (void)hipStreamSynchronize(h->side);
     (void)hipStreamDestroy(h->side);
