@@ -454,7 +454,7 @@ SweepParams base_params(tomo_handle *h) {
   // thin-shell manikin at 512, 3% faster than 2,048).  The large-slot list gets a quarter as
   // many (two per orientation at 256: 10% faster than one, four and eight were slower; on the
   // manikin 1% faster than an eighth, a sixteenth was 3% slower).
-  p.pair_span = h->opt.grid <= 256 ? 8192 : 1024;
+  p.pair_span = h->opt.grid <= 256 ? kPairSpanMax : 1024;
   p.pair_tiles = (g2 + p.pair_span - 1) / p.pair_span;
   p.big_groups = p.pair_tiles >= 4 ? p.pair_tiles / 4 : 1;
   // Bed bands of 40 rows up to a 256 grid, where a band and its halo fit the bed kernel's LDS

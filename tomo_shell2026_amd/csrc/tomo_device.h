@@ -15,6 +15,7 @@ constexpr int kSmall = 8;              // slots with <= 8 entries are paired in 
 constexpr int kInlineDefault = 16;     // entries stored inline per slot (layer-major); tunable
 constexpr int kInlineMax = 32;         // in [kSmall, kInlineMax]
 constexpr int kBigCap = 512;           // entries of a slot sorted by one wave in pair_big_kernel
+constexpr int kPairSpanMax = 8192;     // touched-box slots per pair_kernel work-group (LDS window)
 constexpr int kHugeLds = 8192;         // entries of a slot sorted in LDS by pair_huge_kernel; more
                                        // are selected in passes over global memory
 constexpr int kHugeList = 1024;        // one triangle's entries of one slot (pair_huge_kernel)

@@ -1333,7 +1333,10 @@ __device__ __forceinline__ void compact_body(const SweepParams &p, int32_t blk) 
 template <int PART, bool DBG>
 __device__ __forceinline__ void pair_body(const SweepParams &p, int32_t blk) {
   __shared__ uint64_t sortbuf[kWaves][kBigCap];
-  __shared__ uint32_t pairq[2][kWaves][128];  // queued slots of 3 .. kSmall entries: s | entries << 24
+  __shared__ uint32_t pairq[2][kWaves][128];  // queued slots of 3 .. kSmall entries: i - i0 | entries << 24
+  // pair_kernel's bed bookkeeping for its part of the box, two 16-bit values a word (0xffff: a
+  // slot of pair_big_kernel's), written out in slot order at the end
+  __shared__ uint32_t bwin2[PART == 0 ? kPairSpanMax / 2 : 1];
   __shared__ int64_t red[kWaves][18];
   __shared__ int32_t fred[kWaves];
   const int tid = threadIdx.x, lane = tid & 63, wave = wave_index();
@@ -1370,7 +1373,8 @@ __device__ __forceinline__ void pair_body(const SweepParams &p, int32_t blk) {
   int32_t *const dbg_final = DBG ? p.dbg_final : nullptr;
   // Pair one slot of at most N entries in registers (one lane).  Slots of at most 2 entries
   // come with their keys (ka, kb: 0 past the count); larger ones read theirs here.
-  auto pair_small = [&](auto nt, int32_t s, uint32_t c, uint32_t ka, uint32_t kb) {
+  uint16_t *const bwin = (uint16_t *)bwin2;
+  auto pair_small = [&](auto nt, int32_t s, int32_t wi, uint32_t c, uint32_t ka, uint32_t kb) {
     constexpr int N = decltype(nt)::value;
     acc.nslot++;
     uint32_t k[N];
@@ -1385,7 +1389,7 @@ __device__ __forceinline__ void pair_body(const SweepParams &p, int32_t blk) {
     if (N == 2 && !dbg_rows) pair_two(k[0], k[1], (int)c, p.pp, r);  // straight-line form
     else pair_keys<N>(k, (int)c, p.pp, r, dbg_rows ? dbg_rows + (int64_t)s * 64 : nullptr);
     account<DBG>(acc, r, s, p, track);
-    t.bedinfo[s] = (uint16_t)bed_info(r, p.pp);
+    bwin[wi] = (uint16_t)bed_info(r, p.pp);
     if (sv) put_sv(sv, G2, s, r);
     if (final_here) {
       acc.nfinal += r.n;
@@ -1395,15 +1399,20 @@ __device__ __forceinline__ void pair_body(const SweepParams &p, int32_t blk) {
   };
   // The touched box is walked 64 slots per wave step.  A step issues the loads of the next
   // step's counters and of its own slots' keys when they have at most 2 entries (most slots),
-  // then pairs the previous step's slots of at most 2 entries (their keys have arrived) and
-  // writes their bed bookkeeping, and the empty slots', together: the step's stores cover a run
-  // of consecutive slots.  Slots of 3 .. kSmall entries are queued by size class (<= 4,
-  // <= 8) and paired 64 at a time with that class's register network, so no wave runs a larger
-  // network than its slots need.  One extra step with no slots pairs what remains.
+  // then pairs the previous step's slots of at most 2 entries (their keys have arrived).  Slots
+  // of 3 .. kSmall entries are queued by size class (<= 4, <= 8) and paired 64 at a time with
+  // that class's register network, so no wave runs a larger network than its slots need.  One
+  // extra step with no slots pairs what remains.  The bed bookkeeping goes to LDS and leaves in
+  // slot order once the work-group is done: stored as each slot is paired, the queued slots'
+  // values reached a line long after its neighbours' and L2 wrote it out in pieces, three times
+  // the bytes.
   if constexpr (PART == 0) {
   uint32_t n4 = 0u, n8 = 0u;  // queue fills (wave-uniform)
   uint32_t *const q4 = pairq[0][wave], *const q8 = pairq[1][wave];
   const int32_t i1 = min(i0 + p.pair_span, nbb);
+  __syncthreads();  // a work-queue launch reuses the window for its next unit
+  for (int32_t j = tid; j < (i1 - i0 + 1) / 2; j += kThreads) bwin2[j] = ~0u;
+  __syncthreads();
   if (i0 + wave * 64 < i1) {
     // slot i of the box is (bx0 + i / wy, by0 + i % wy); step kThreads slots at a time
     const int32_t first = i0 + wave * 64 + lane;
@@ -1430,14 +1439,15 @@ __device__ __forceinline__ void pair_body(const SweepParams &p, int32_t blk) {
         ka = t.ent[s].x;
         if (c == 2u) kb = t.ent[(int64_t)G2 + s].x;
       }
+      const int32_t pwi = base - kThreads + lane - i0;  // the previous step's slot in the window
       if (pc == 0u) {
-        t.bedinfo[ps] = (uint16_t)BI_POSS;
+        bwin[pwi] = (uint16_t)BI_POSS;
         if (sv) {
           const SlotResult z = {};
           put_sv(sv, G2, ps, z);
         }
       } else if (pc <= 2u) {
-        pair_small(std::integral_constant<int, 2>{}, ps, pc, pka, pkb);
+        pair_small(std::integral_constant<int, 2>{}, ps, pwi, pc, pka, pkb);
       }
       // Pair full batches (in the extra step, everything queued), one class at a time: a single
       // call site per network keeps the code (and the register demand) small.
@@ -1451,10 +1461,11 @@ __device__ __forceinline__ void pair_body(const SweepParams &p, int32_t blk) {
         wave_sync();
         if ((uint32_t)lane < np) {
           const uint32_t qe = q[lane];
-          const int32_t qs = (int32_t)(qe & 0xffffffu);
+          const int32_t qwi = (int32_t)(qe & 0xffffffu), qi = i0 + qwi;
+          const int32_t qs = (bx0 + qi / wy) * G + by0 + qi % wy;
           const uint32_t qc = qe >> 24;
-          if (fc == 0) pair_small(std::integral_constant<int, 4>{}, qs, qc, 0u, 0u);
-          else pair_small(std::integral_constant<int, kSmall>{}, qs, qc, 0u, 0u);
+          if (fc == 0) pair_small(std::integral_constant<int, 4>{}, qs, qwi, qc, 0u, 0u);
+          else pair_small(std::integral_constant<int, kSmall>{}, qs, qwi, qc, 0u, 0u);
         }
         wave_sync();
         if (n > 64u) {
@@ -1479,7 +1490,7 @@ __device__ __forceinline__ void pair_body(const SweepParams &p, int32_t blk) {
       uint32_t e = 0u;
       if (i < i1 && c > 2u && c <= (uint32_t)kSmall) {
         cls = c <= 4u ? 0 : 1;
-        e = (uint32_t)s | (c << 24);
+        e = (uint32_t)(i - i0) | (c << 24);
       }
       {
         const uint64_t m = ballot(cls == 0);
@@ -1490,6 +1501,21 @@ __device__ __forceinline__ void pair_body(const SweepParams &p, int32_t blk) {
         const uint64_t m = ballot(cls == 1);
         if (cls == 1) q8[n8 + lanes_below(m)] = e;
         n8 += (uint32_t)__popcll(m);
+      }
+    }
+  }
+  __syncthreads();
+  {
+    int32_t wsx = bx0 + (i0 + tid) / wy, wsy = by0 + (i0 + tid) % wy;
+    const int32_t dq = kThreads / wy, dr = kThreads % wy;
+    for (int32_t wi = tid; wi < i1 - i0; wi += kThreads) {
+      const uint32_t v = bwin[wi];
+      if (v != 0xffffu) t.bedinfo[wsx * G + wsy] = (uint16_t)v;
+      wsx += dq;
+      wsy += dr;
+      if (wsy > by1) {
+        wsy -= wy;
+        wsx += 1;
       }
     }
   }
