@@ -729,11 +729,11 @@ __device__ __forceinline__ Tables tables_for(const SweepParams &p, int32_t b) {
 template <int N, int W = kWaves>
 __device__ __forceinline__ void group_add(int64_t (&v)[N], int64_t (*red)[N], int lane, int wave,
                                           int tid, unsigned long long *dst) {
+  // one total at a time, each stored as soon as it is reduced: few registers stay live
 #pragma unroll
-  for (int q = 0; q < N; q++) v[q] = wave_sum(v[q]);
-  if (lane == 0) {
-#pragma unroll
-    for (int q = 0; q < N; q++) red[wave][q] = v[q];
+  for (int q = 0; q < N; q++) {
+    const int64_t w = wave_sum(v[q]);
+    if (lane == 0) red[wave][q] = w;
   }
   __syncthreads();
   if (tid < N) {
